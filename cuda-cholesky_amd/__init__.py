@@ -1,0 +1,313 @@
+"""Python mirror of the MI355X Cholesky library's C ABI.
+
+The product is libcucholesky.so (built from csrc/ for gfx950), whose entry
+points are the reference's include/blas.h and include/lapack.h device-pointer
+API.  This module binds those entry points with ctypes under the same names,
+argument order and meaning, so Python callers (the tests, the benchmark) drive
+exactly the code a C caller links against.  LAPACK-style functions return
+``(status, info)``; BLAS functions return ``status`` (a CUresult value).
+
+Device memory is plain torch tensors: a column-major m x n matrix with leading
+dimension ld is stored as a contiguous tensor of shape (n, ld) -- element
+(i, j) at offset j * ld + i -- and passed by its device byte address.
+
+There is no fallback: if the library is missing or fails to load, importing
+this module raises.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB_PATH = os.path.join(HERE, "libcucholesky.so")
+
+CUDA_SUCCESS = 0
+CUDA_ERROR_INVALID_VALUE = 1
+CUDA_ERROR_OUT_OF_MEMORY = 2
+
+# char-valued enums (reference include/blas.h:24-27)
+NoTrans, Trans, ConjTrans = ord("N"), ord("T"), ord("C")
+Lower, Upper = ord("L"), ord("U")
+Left, Right = ord("L"), ord("R")
+NonUnit, Unit = ord("N"), ord("U")
+
+
+def build(jobs=8):
+    """Compile every HIP source for gfx950 and link libcucholesky.so."""
+    subprocess.run(["make", "-C", CSRC, "-B", f"-j{jobs}", "-s"], check=True)
+
+
+class _Z(ctypes.Structure):
+    """C99 `double complex` passed by value."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+def zscalar(v):
+    v = complex(v)
+    return _Z(v.real, v.imag)
+
+
+_sz = ctypes.c_size_t
+_ptr = ctypes.c_uint64     # CUdeviceptr: device byte address
+_enum = ctypes.c_int       # enums are int-sized
+_res = ctypes.c_int        # CUresult
+_stream = ctypes.c_void_p  # CUstream
+_longp = ctypes.POINTER(ctypes.c_long)
+_h = ctypes.c_void_p
+
+# name -> argument types (restype CUresult unless noted); mirrors
+# include/blas.h and include/lapack.h
+SIGNATURES = {
+    "cuBLASCreate": [ctypes.POINTER(_h)],
+    "cuBLASDestroy": [_h],
+    "cuDsyrk": [_h, _enum, _enum, _sz, _sz, ctypes.c_double, _ptr, _sz, ctypes.c_double, _ptr, _sz,
+                _stream],
+    "cuZherk": [_h, _enum, _enum, _sz, _sz, ctypes.c_double, _ptr, _sz, ctypes.c_double, _ptr, _sz,
+                _stream],
+    "cuDgemm2": [_h, _enum, _enum, _sz, _sz, _sz, ctypes.c_double, _ptr, _sz, _ptr, _sz,
+                 ctypes.c_double, _ptr, _sz, _ptr, _sz, _stream],
+    "cuZgemm2": [_h, _enum, _enum, _sz, _sz, _sz, _Z, _ptr, _sz, _ptr, _sz, _Z, _ptr, _sz, _ptr,
+                 _sz, _stream],
+    "cuDtrmm2": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_double, _ptr, _sz, _ptr, _sz,
+                 _ptr, _sz, _stream],
+    "cuZtrmm2": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _Z, _ptr, _sz, _ptr, _sz, _ptr, _sz,
+                 _stream],
+    "cuDtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_double, _ptr, _sz, _ptr, _sz,
+                _stream],
+    "cuZtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _Z, _ptr, _sz, _ptr, _sz, _stream],
+    "cuLAPACKCreate": [ctypes.POINTER(_h)],
+    "cuLAPACKDestroy": [_h],
+    "cuDpotrf": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuZpotrf": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuDtrtri": [_h, _enum, _enum, _sz, _ptr, _sz, _longp],
+    "cuZtrtri": [_h, _enum, _enum, _sz, _ptr, _sz, _longp],
+    "cuDtrti22": [_h, _enum, _enum, _sz, _ptr, _sz, _ptr, _sz, _ptr, _stream],
+    "cuZtrti22": [_h, _enum, _enum, _sz, _ptr, _sz, _ptr, _sz, _ptr, _stream],
+    "cuDlauum": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuZlauum": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuDpotri": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuZpotri": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuDlogdet": [_h, _ptr, _sz, _sz, ctypes.POINTER(ctypes.c_double), _stream],
+    "cuZlogdet": [_h, _ptr, _sz, _sz, ctypes.POINTER(ctypes.c_double), _stream],
+    "cuGetErrorString": [_res],
+}
+
+XERBLA_T = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_long)
+ERRHANDLER_T = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                               "(or make -C cuda-cholesky_amd/csrc) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _res
+        L.cuGetErrorString.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def error_string(code):
+    return lib().cuGetErrorString(code).decode()
+
+
+class Xerbla:
+    """Context manager capturing xerbla reports (function, parameter index)."""
+
+    def __init__(self):
+        self.calls = []
+        self._cb = XERBLA_T(lambda f, i: self.calls.append((f.decode(), i)))
+        self._slot = ctypes.c_void_p.in_dll(lib(), "xerbla")
+
+    def __enter__(self):
+        self._old = self._slot.value
+        self._slot.value = ctypes.cast(self._cb, ctypes.c_void_p).value
+        return self
+
+    def __exit__(self, *exc):
+        self._slot.value = self._old
+        return False
+
+
+# ---- handles ----
+
+class BLASHandle:
+    def __init__(self):
+        self.h = _h()
+        st = lib().cuBLASCreate(ctypes.byref(self.h))
+        if st != CUDA_SUCCESS:
+            raise RuntimeError(f"cuBLASCreate failed: {st}")
+
+    def close(self):
+        if self.h:
+            lib().cuBLASDestroy(self.h)
+            self.h = _h()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class LAPACKHandle:
+    def __init__(self):
+        self.h = _h()
+        st = lib().cuLAPACKCreate(ctypes.byref(self.h))
+        if st != CUDA_SUCCESS:
+            raise RuntimeError(f"cuLAPACKCreate failed: {st}")
+
+    def close(self):
+        if self.h:
+            lib().cuLAPACKDestroy(self.h)
+            self.h = _h()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _hv(handle):
+    return handle.h if hasattr(handle, "h") else handle
+
+
+# ---- LAPACK (status, info) ----
+
+def _info_call(fn, *args):
+    info = ctypes.c_long(0)
+    st = fn(*args, ctypes.byref(info))
+    return st, info.value
+
+
+def cuDpotrf(handle, uplo, n, A, lda):
+    return _info_call(lib().cuDpotrf, _hv(handle), uplo, n, A, lda)
+
+
+def cuZpotrf(handle, uplo, n, A, lda):
+    return _info_call(lib().cuZpotrf, _hv(handle), uplo, n, A, lda)
+
+
+def cuDtrtri(handle, uplo, diag, n, A, lda):
+    return _info_call(lib().cuDtrtri, _hv(handle), uplo, diag, n, A, lda)
+
+
+def cuZtrtri(handle, uplo, diag, n, A, lda):
+    return _info_call(lib().cuZtrtri, _hv(handle), uplo, diag, n, A, lda)
+
+
+def cuDlauum(handle, uplo, n, A, lda):
+    return _info_call(lib().cuDlauum, _hv(handle), uplo, n, A, lda)
+
+
+def cuZlauum(handle, uplo, n, A, lda):
+    return _info_call(lib().cuZlauum, _hv(handle), uplo, n, A, lda)
+
+
+def cuDpotri(handle, uplo, n, A, lda):
+    return _info_call(lib().cuDpotri, _hv(handle), uplo, n, A, lda)
+
+
+def cuZpotri(handle, uplo, n, A, lda):
+    return _info_call(lib().cuZpotri, _hv(handle), uplo, n, A, lda)
+
+
+def cuDtrti22(handle, uplo, diag, n, A, lda, B, ldb, info_ptr, stream=None):
+    return lib().cuDtrti22(_hv(handle), uplo, diag, n, A, lda, B, ldb, info_ptr, stream)
+
+
+def cuZtrti22(handle, uplo, diag, n, A, lda, B, ldb, info_ptr, stream=None):
+    return lib().cuZtrti22(_hv(handle), uplo, diag, n, A, lda, B, ldb, info_ptr, stream)
+
+
+def cuDlogdet(handle, x, incx, n, stream=None):
+    r = ctypes.c_double(0.0)
+    st = lib().cuDlogdet(_hv(handle), x, incx, n, ctypes.byref(r), stream)
+    return st, r.value
+
+
+def cuZlogdet(handle, x, incx, n, stream=None):
+    r = ctypes.c_double(0.0)
+    st = lib().cuZlogdet(_hv(handle), x, incx, n, ctypes.byref(r), stream)
+    return st, r.value
+
+
+# ---- BLAS (status) ----
+
+def cuDgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, D, ldd,
+             stream=None):
+    return lib().cuDgemm2(_hv(handle), transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                          D, ldd, stream)
+
+
+def cuZgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, D, ldd,
+             stream=None):
+    return lib().cuZgemm2(_hv(handle), transA, transB, m, n, k, zscalar(alpha), A, lda, B, ldb,
+                          zscalar(beta), C, ldc, D, ldd, stream)
+
+
+def cuDgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=None):
+    return cuDgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc,
+                    stream)
+
+
+def cuZgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=None):
+    return cuZgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc,
+                    stream)
+
+
+def cuDsyrk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream=None):
+    return lib().cuDsyrk(_hv(handle), uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream)
+
+
+def cuZherk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream=None):
+    return lib().cuZherk(_hv(handle), uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream)
+
+
+def cuDtrmm2(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx, stream=None):
+    return lib().cuDtrmm2(_hv(handle), side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx,
+                          stream)
+
+
+def cuZtrmm2(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx, stream=None):
+    return lib().cuZtrmm2(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), A, lda, B,
+                          ldb, X, ldx, stream)
+
+
+def cuDtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
+    return lib().cuDtrsm(_hv(handle), side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream)
+
+
+def cuZtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
+    return lib().cuZtrsm(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), A, lda, B, ldb,
+                         stream)
+
+
+# ---- device matrices (torch is only the allocator here) ----
+
+def to_device(M, device="cuda"):
+    """Column-major copy of the 2-D numpy array M on the GPU: a tensor of
+    shape (n, ld) with M's leading dimension ld = M.shape[0]."""
+    import numpy as np
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(M.T)).to(device)
+
+
+def from_device(t):
+    """numpy (ld, n) Fortran-ordered view of a device matrix made by to_device."""
+    return t.detach().cpu().numpy().T
+
+
+def ptr(t):
+    return t.data_ptr()

@@ -1,0 +1,380 @@
+// Diagonal-block kernels and reductions, double and double complex.
+//
+// A diagonal block of order n <= 128 (double) or n <= 64 (complex) is held in
+// LDS by one workgroup of 512 threads and processed in "lower form": for an
+// upper-triangle caller the block is loaded conjugate-transposed, so
+// U^H U = L L^H, inv(U) = inv(L)^H and U U^H = L^H L reduce every case to a
+// lower-triangular kernel, and stored back conjugate-transposed.
+//
+//   diag_potrf   factor + inverse (reference lapack/dpotrf.cu:23-148 and the
+//                host dpotrf of lapack/dpotrf.c:268; here it never leaves the GPU)
+//   diag_trtri   triangular inverse (reference lapack/dtrtri.cu:23-167)
+//   diag_lauum   L^H L / U U^H      (reference lapack/dlauum.cu:23-91)
+//   trsm_diag_inverses  batched inverses of op(A)'s diagonal blocks
+//   logdet       2 sum log(Re x_i)  (reference lapack/dlogdet.cu:53-118)
+#include "internal.h"
+
+#include <climits>
+
+namespace hc {
+namespace {
+
+constexpr int DIAG_THREADS = 512;
+
+template <typename T> __device__ __forceinline__ T shfl_xor_t(T v, int m);
+template <> __device__ __forceinline__ double shfl_xor_t<double>(double v, int m) {
+  return __shfl_xor(v, m, 64);
+}
+template <> __device__ __forceinline__ zd shfl_xor_t<zd>(zd v, int m) {
+  return make_zd(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+// Sum over groups of G consecutive lanes (G a power of two dividing 64).
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v = v + shfl_xor_t(v, off);
+  return v;
+}
+
+// S is column-major with leading dimension NMAX + 1: S(r, c) = S[c * LS + r].
+template <typename T, int NMAX>
+struct Lds {
+  static constexpr int LS = NMAX + 1;
+  static constexpr int TPR = DIAG_THREADS / NMAX;  // threads per row in sweeps
+};
+
+// Load the lower form of the n x n block at A: S(r, c) = A(r, c) (lower) or
+// conj(A(c, r)) (upper), r >= c.  Memory is read along its contiguous index.
+template <typename T, int NMAX>
+__device__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
+  constexpr int LS = Lds<T, NMAX>::LS;
+  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+    const int a = idx % n, b = idx / n;  // a: contiguous index in memory
+    if (lower) {
+      if (a >= b) S[b * LS + a] = A[(long)b * lda + a];
+    } else {
+      // A(a, b) with a <= b is upper; it is conj of lower-form (b, a)
+      if (a <= b) S[a * LS + b] = conjx(A[(long)b * lda + a]);
+    }
+  }
+}
+
+// Store the lower form back into the caller's triangle (diagonal included
+// unless skip_diag).
+template <typename T, int NMAX>
+__device__ void store_lower_form(const T *S, bool lower, int n, T *A, long lda, bool skip_diag) {
+  constexpr int LS = Lds<T, NMAX>::LS;
+  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+    const int a = idx % n, b = idx / n;
+    if (skip_diag && a == b) continue;
+    if (lower) {
+      if (a >= b) A[(long)b * lda + a] = S[b * LS + a];
+    } else {
+      if (a <= b) A[(long)b * lda + a] = conjx(S[a * LS + b]);
+    }
+  }
+}
+
+// W(r, c) = S(r, c) for r >= c, zero above; unit -> ones on the diagonal.
+// With upper, W is the conjugate transpose instead (upper triangular).
+template <typename T, int NMAX>
+__device__ void store_dense(const T *S, bool upper, int n, bool unit, T *W, long ldw) {
+  constexpr int LS = Lds<T, NMAX>::LS;
+  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+    const int r = idx % n, c = idx / n;
+    T v = from_real<T>(0.0);
+    if (r == c) v = unit ? from_real<T>(1.0) : (upper ? conjx(S[c * LS + r]) : S[c * LS + r]);
+    else if (!upper && r > c) v = S[c * LS + r];
+    else if (upper && r < c) v = conjx(S[r * LS + c]);
+    W[(long)c * ldw + r] = v;
+  }
+}
+
+// In-place Cholesky of the lower form: right-looking, one column per step.
+// Returns the 0-based index of the first non-positive (or NaN) pivot, or -1.
+template <typename T, int NMAX>
+__device__ int lds_potrf(T *S, int n) {
+  constexpr int LS = Lds<T, NMAX>::LS;
+  const int tid = threadIdx.x;
+  for (int k = 0; k < n; k++) {
+    const double d = realx(S[k * LS + k]);
+    if (!(d > 0.0)) return k;  // every thread read the same value: uniform
+    const double sd = sqrt(d);
+    for (int i = k + 1 + tid; i < n; i += DIAG_THREADS) S[k * LS + i] = S[k * LS + i] / sd;
+    __syncthreads();
+    if (tid == 0) S[k * LS + k] = from_real<T>(sd);
+    const int len = n - k - 1;
+    for (int idx = tid; idx < len * len; idx += DIAG_THREADS) {
+      const int rr = idx % len, cc = idx / len;
+      if (rr >= cc) {
+        const int r = k + 1 + rr, c = k + 1 + cc;
+        S[c * LS + r] = S[c * LS + r] - S[k * LS + r] * conjx(S[k * LS + c]);
+      }
+    }
+    __syncthreads();
+  }
+  return -1;
+}
+
+// In-place inverse of the lower form, columns from last to first:
+// column j <- -inv(L(j,j)) * X(j+1:, j+1:) * L(j+1:, j) with X the already
+// inverted trailing block (the column sweep of LAPACK xTRTI2).
+template <typename T, int NMAX>
+__device__ void lds_trtri(T *S, int n, bool unit) {
+  constexpr int LS = Lds<T, NMAX>::LS, TPR = Lds<T, NMAX>::TPR;
+  const int tid = threadIdx.x, row = tid / TPR, part = tid % TPR;
+  for (int j = n - 1; j >= 0; j--) {
+    const T djinv = unit ? from_real<T>(1.0) : from_real<T>(1.0) / S[j * LS + j];
+    const int i = j + 1 + row;
+    T sum = from_real<T>(0.0);
+    if (i < n) {
+      for (int k = j + 1 + part; k <= i; k += TPR) {
+        const T x = (k == i) ? (unit ? from_real<T>(1.0) : S[i * LS + i]) : S[k * LS + i];
+        sum = sum + x * S[j * LS + k];
+      }
+    }
+    sum = group_sum<TPR>(sum);
+    __syncthreads();
+    if (i < n && part == 0) S[j * LS + i] = -(djinv * sum);
+    if (tid == 0 && !unit) S[j * LS + j] = djinv;
+    __syncthreads();
+  }
+}
+
+// In-place R = L^H L of the lower form (lower part), columns left to right:
+// R(r, c) = sum_{k >= r} conj(L(k, r)) L(k, c).
+template <typename T, int NMAX>
+__device__ void lds_lauum(T *S, int n) {
+  constexpr int LS = Lds<T, NMAX>::LS, TPR = Lds<T, NMAX>::TPR;
+  const int tid = threadIdx.x, row = tid / TPR, part = tid % TPR;
+  for (int c = 0; c < n; c++) {
+    const int r = c + row;
+    T sum = from_real<T>(0.0);
+    if (r < n)
+      for (int k = r + part; k < n; k += TPR) sum = sum + conjx(S[r * LS + k]) * S[c * LS + k];
+    sum = group_sum<TPR>(sum);
+    __syncthreads();
+    if (r < n && part == 0) S[c * LS + r] = sum;
+    __syncthreads();
+  }
+}
+
+template <typename T, int NMAX>
+__global__ void __launch_bounds__(DIAG_THREADS)
+diag_potrf_kernel(int lower, int n, T *A, long lda, T *W, long ldw, int *info, long info_off) {
+  if (*info != 0) return;
+  __shared__ T S[NMAX * (NMAX + 1)];
+  load_lower_form<T, NMAX>(S, lower, n, A, lda);
+  __syncthreads();
+  const int bad = lds_potrf<T, NMAX>(S, n);
+  if (bad >= 0) {
+    if (threadIdx.x == 0) *info = (int)(info_off + bad + 1);
+    return;
+  }
+  __syncthreads();
+  store_lower_form<T, NMAX>(S, lower, n, A, lda, false);
+  lds_trtri<T, NMAX>(S, n, false);
+  store_dense<T, NMAX>(S, false, n, false, W, ldw);
+}
+
+template <typename T, int NMAX>
+__global__ void __launch_bounds__(DIAG_THREADS)
+diag_trtri_kernel(int lower, int unit, int n, const T *A, long lda, T *B, long ldb, T *W, long ldw,
+                  const int *abort_flag) {
+  if (abort_flag != nullptr && *abort_flag != 0) return;
+  __shared__ T S[NMAX * (NMAX + 1)];
+  load_lower_form<T, NMAX>(S, lower, n, A, lda);
+  __syncthreads();
+  lds_trtri<T, NMAX>(S, n, unit);
+  store_lower_form<T, NMAX>(S, lower, n, B, ldb, unit != 0);
+  if (W != nullptr) store_dense<T, NMAX>(S, false, n, unit != 0, W, ldw);
+}
+
+template <typename T, int NMAX>
+__global__ void __launch_bounds__(DIAG_THREADS) diag_lauum_kernel(int lower, int n, T *A, long lda) {
+  __shared__ T S[NMAX * (NMAX + 1)];
+  load_lower_form<T, NMAX>(S, lower, n, A, lda);
+  __syncthreads();
+  lds_lauum<T, NMAX>(S, n);
+  store_lower_form<T, NMAX>(S, lower, n, A, lda, false);
+}
+
+// Block b inverts the nb x nb diagonal block of op(A) starting at b * nb.
+template <typename T, int NMAX>
+__global__ void __launch_bounds__(DIAG_THREADS)
+trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const T *A, long lda, T *W) {
+  constexpr int LS = Lds<T, NMAX>::LS;
+  __shared__ T S[NMAX * (NMAX + 1)];
+  const long off = (long)blockIdx.x * nb;
+  const int n = (int)((m - off) < nb ? (m - off) : nb);
+  // lower form: S(r, c) = op(A)(r, c) when op(A) is lower-like, else
+  // conj(op(A)(c, r)); op(A)(i, j) = A(i, j), A(j, i) or conj(A(j, i)).
+  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+    const int a = idx % n, b = idx / n;  // memory element A(off + a, off + b)
+    const T v = A[(long)(off + b) * lda + off + a];
+    // as an element of op(A): (a, b) for N, (b, a) for T/C
+    int r = (op == OP_N) ? a : b, c = (op == OP_N) ? b : a;
+    T e = (op == OP_C) ? conjx(v) : v;
+    if (!lowerlike) { const int t = r; r = c; c = t; e = conjx(e); }
+    if (r >= c) S[c * LS + r] = e;
+  }
+  __syncthreads();
+  lds_trtri<T, NMAX>(S, n, unit);
+  store_dense<T, NMAX>(S, !lowerlike, n, unit != 0, W + (long)blockIdx.x * nb * nb, nb);
+}
+
+// One workgroup scans the diagonal; info = first (upper) or last (lower)
+// 1-based index of an exact zero, 0 when none.
+template <typename T>
+__global__ void __launch_bounds__(1024) zero_scan_kernel(int lower, long n, const T *A, long lda, int *info) {
+  __shared__ int best[1024];
+  int b = lower ? 0 : INT_MAX;
+  for (long i = threadIdx.x; i < n; i += 1024) {
+    if (is_zero(A[i * lda + i])) {
+      const int v = (int)(i + 1);
+      b = lower ? (v > b ? v : b) : (v < b ? v : b);
+    }
+  }
+  best[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const int o = best[threadIdx.x + s], c = best[threadIdx.x];
+      best[threadIdx.x] = lower ? (o > c ? o : c) : (o < c ? o : c);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *info = (best[0] == INT_MAX) ? 0 : best[0];
+}
+
+// ---- compensated (double-double) summation of logs ----
+struct dd {
+  double hi, lo;
+};
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b.hi - bb) + a.lo + b.lo;
+  dd r;
+  r.hi = s + e;
+  r.lo = e - (r.hi - s);
+  return r;
+}
+__device__ __forceinline__ dd dd_shfl_xor(dd v, int m) {
+  dd r;
+  r.hi = __shfl_xor(v.hi, m, 64);
+  r.lo = __shfl_xor(v.lo, m, 64);
+  return r;
+}
+__device__ dd block_sum(dd v) {
+  __shared__ double part[2 * 16];
+  for (int off = 32; off > 0; off >>= 1) v = dd_add(v, dd_shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) { part[2 * wave] = v.hi; part[2 * wave + 1] = v.lo; }
+  __syncthreads();
+  dd t = {0.0, 0.0};
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nw; w++) t = dd_add(t, dd{part[2 * w], part[2 * w + 1]});
+  return t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) logdet_partial_kernel(const T *x, long incx, long n, double *partials) {
+  dd acc = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    acc = dd_add(acc, dd{log(realx(x[i * incx])), 0.0});
+  const dd t = block_sum(acc);
+  if (threadIdx.x == 0) { partials[2 * blockIdx.x] = t.hi; partials[2 * blockIdx.x + 1] = t.lo; }
+}
+
+__global__ void __launch_bounds__(256) logdet_final_kernel(const double *partials, int nblocks, double *out) {
+  dd acc = {0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    acc = dd_add(acc, dd{partials[2 * b], partials[2 * b + 1]});
+  const dd t = block_sum(acc);
+  if (threadIdx.x == 0) out[0] = 2.0 * (t.hi + t.lo);
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t diag_potrf(hipStream_t s, char uplo, int n, T *A, long lda, T *W, long ldw, int *info,
+                      long info_off) {
+  constexpr int NMAX = diag_max<T>();
+  if (n <= 0) return hipSuccess;
+  if (n > NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((diag_potrf_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
+                     uplo == 'L' ? 1 : 0, n, A, lda, W, ldw, info, info_off);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t diag_trtri(hipStream_t s, char uplo, char diag, int n, const T *A, long lda, T *B,
+                      long ldb, T *W, long ldw, const int *abort_flag) {
+  constexpr int NMAX = diag_max<T>();
+  if (n <= 0) return hipSuccess;
+  if (n > NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((diag_trtri_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
+                     uplo == 'L' ? 1 : 0, diag == 'U' ? 1 : 0, n, A, lda, B, ldb, W, ldw,
+                     abort_flag);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda) {
+  constexpr int NMAX = diag_max<T>();
+  if (n <= 0) return hipSuccess;
+  if (n > NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
+                     uplo == 'L' ? 1 : 0, n, A, lda);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t trsm_diag_inverses(hipStream_t s, char uplo, char trans, char diag, long m, int nb,
+                              const T *A, long lda, T *W) {
+  constexpr int NMAX = diag_max<T>();
+  if (m <= 0) return hipSuccess;
+  if (nb > NMAX) return hipErrorInvalidValue;
+  const int op = is_complex<T>::value ? op_code(trans) : (trans == 'N' ? OP_N : OP_T);
+  const int lowerlike = ((uplo == 'L') == (trans == 'N')) ? 1 : 0;
+  const unsigned nblk = (unsigned)((m + nb - 1) / nb);
+  hipLaunchKernelGGL((trsm_inv_kernel<T, NMAX>), dim3(nblk), dim3(DIAG_THREADS), 0, s, lowerlike,
+                     op, diag == 'U' ? 1 : 0, m, nb, A, lda, W);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t diag_zero_scan(hipStream_t s, char uplo, long n, const T *A, long lda, int *info) {
+  hipLaunchKernelGGL((zero_scan_kernel<T>), dim3(1), dim3(1024), 0, s, uplo == 'L' ? 1 : 0, n, A,
+                     lda, info);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials, double *out) {
+  long blocks = (n + 255) / 256;
+  if (blocks > kLogdetMaxBlocks) blocks = kLogdetMaxBlocks;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((logdet_partial_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, x, incx,
+                     n, partials);
+  HC_TRY(hipGetLastError());
+  hipLaunchKernelGGL(logdet_final_kernel, dim3(1), dim3(256), 0, s, partials, (int)blocks, out);
+  return hipGetLastError();
+}
+
+#define HC_INST(T)                                                                            \
+  template hipError_t diag_potrf<T>(hipStream_t, char, int, T *, long, T *, long, int *, long); \
+  template hipError_t diag_trtri<T>(hipStream_t, char, char, int, const T *, long, T *, long,   \
+                                    T *, long, const int *);                                    \
+  template hipError_t diag_lauum<T>(hipStream_t, char, int, T *, long);                         \
+  template hipError_t trsm_diag_inverses<T>(hipStream_t, char, char, char, long, int,           \
+                                            const T *, long, T *);                              \
+  template hipError_t diag_zero_scan<T>(hipStream_t, char, long, const T *, long, int *);       \
+  template hipError_t logdet<T>(hipStream_t, const T *, long, long, double *, double *);
+HC_INST(double)
+HC_INST(zd)
+#undef HC_INST
+
+}  // namespace hc

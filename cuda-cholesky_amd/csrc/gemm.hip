@@ -1,0 +1,354 @@
+// Tile GEMM on CDNA4 matrix cores, double and double complex.
+//
+// D <- alpha * op(A) * op(B) + beta * C, column-major, op in {N, T, C}.  One
+// kernel serves every BLAS-3 shape on the Cholesky path:
+//   * GEMM                (reference blas/dgemm.cu:198-324, zgemm.cu:53-330)
+//   * SYRK / HERK         tri_out + tri_grid (reference blas/dsyrk.cu:261-455)
+//   * TRMM                tri_a / tri_b masks (reference blas/dtrmm.cu:941-1727)
+//   * TRSM block steps    with inverted diagonal blocks (blas_ops.cpp)
+//
+// Work decomposition: a 256-thread workgroup (4 waves in a 2 x 2 grid) owns an
+// MT x NT tile of D and walks K in steps of 16.  Each step stages op(A)
+// (MT x 16) and op(B) (16 x NT) into LDS, transposed so that both are stored
+// k-major with the M / N index contiguous; the next step's global loads are
+// issued into registers before the current step's matrix-core work so the
+// HBM/L2 latency overlaps with compute.  Each wave multiplies its
+// (MT/2) x (NT/2) sub-tile with v_mfma_f64_16x16x4_f64; the op(B) fragment is
+// fed as the instruction's first operand and the op(A) fragment as its second,
+// so the 16 x 16 result block comes out with the row index of D across lanes
+// 0..15 and the epilogue writes 128-byte column segments (coalesced).
+// Complex products use four real MFMAs (re*re - im*im, re*im + im*re) on
+// separate real / imaginary LDS planes.
+#include "internal.h"
+
+namespace hc {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int KT = 16;         // K step
+constexpr int NTHREADS = 256;  // 4 waves
+constexpr int PAD = 2;         // doubles of padding per LDS row
+
+template <typename T>
+struct KParams {
+  int m, n, k;
+  T alpha, beta;
+  const T *A;
+  long lda;
+  const T *B;
+  long ldb;
+  const T *C;
+  long ldc;
+  T *D;
+  long ldd;
+  int tri_out, tri_grid, real_diag, tri_a, tri_b, unit;
+  const int *abort_flag;
+};
+
+// Element (r, c) of op(X).
+template <typename T, int OP>
+__device__ __forceinline__ T fetch(const T *X, long ld, int r, int c) {
+  if (OP == OP_N) return X[(long)c * ld + r];
+  const T v = X[(long)r * ld + c];
+  return OP == OP_C ? conjx(v) : v;
+}
+
+// Element (r, c) of a triangular operand: zero outside the triangle (selected,
+// not multiplied, so garbage there never propagates), one on a unit diagonal.
+template <typename T>
+__device__ __forceinline__ T tri_mask(T v, int tri, int unit, int r, int c) {
+  if ((tri == TRI_LOWER && c > r) || (tri == TRI_UPPER && c < r)) return from_real<T>(0.0);
+  if (unit && r == c) return from_real<T>(1.0);
+  return v;
+}
+
+__device__ __forceinline__ void lds_put(double *s, int plane_stride, int idx, double v) {
+  (void)plane_stride;
+  s[idx] = v;
+}
+__device__ __forceinline__ void lds_put(double *s, int plane_stride, int idx, zd v) {
+  s[idx] = v.x;
+  s[plane_stride + idx] = v.y;
+}
+
+template <typename T, int TA, int TB, int MT, int NT>
+__global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
+  if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+
+  constexpr bool CPLX = is_complex<T>::value;
+  constexpr int P = CPLX ? 2 : 1;
+  constexpr int LSA = MT + PAD, LSB = NT + PAD;
+  constexpr int PSA = KT * LSA, PSB = KT * LSB;  // plane strides
+  constexpr int EA = MT * KT / NTHREADS, EB = NT * KT / NTHREADS;
+  constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
+  static_assert(EA >= 1 && EB >= 1 && BI >= 1 && BJ >= 1, "tile too small");
+
+  __shared__ double sA[P * PSA];
+  __shared__ double sB[P * PSB];
+
+  // ---- tile coordinates ----
+  int bm, bn;
+  if (p.tri_grid) {
+    const int id = blockIdx.x;
+    int r = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= id) r++;
+    while (r * (r + 1) / 2 > id) r--;
+    const int c = id - r * (r + 1) / 2;
+    if (p.tri_out == TRI_LOWER) { bm = r; bn = c; } else { bm = c; bn = r; }
+  } else {
+    bm = blockIdx.x;
+    bn = blockIdx.y;
+  }
+  const int m0 = bm * MT, n0 = bn * NT;
+
+  // ---- K range: a triangular operand bounds it ----
+  int kbeg = 0, kend = p.k;
+  if (p.tri_a == TRI_LOWER) kend = min(kend, m0 + MT);
+  else if (p.tri_a == TRI_UPPER) kbeg = m0;
+  if (p.tri_b == TRI_LOWER) kbeg = max(kbeg, n0);
+  else if (p.tri_b == TRI_UPPER) kend = min(kend, n0 + NT);
+  kbeg &= ~(KT - 1);
+  if (is_zero(p.alpha)) kend = kbeg;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+
+  v4d acc[P][BI][BJ];
+#pragma unroll
+  for (int q = 0; q < P; q++)
+#pragma unroll
+    for (int i = 0; i < BI; i++)
+#pragma unroll
+      for (int j = 0; j < BJ; j++) acc[q][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  T ra[EA], rb[EB];
+
+  // element e of the per-thread load pattern -> (index in M or N, index in K)
+  auto a_coord = [&](int q, int &mi, int &ki) {
+    const int e = tid + q * NTHREADS;
+    if (TA == OP_N) { mi = e % MT; ki = e / MT; }   // M contiguous in memory
+    else            { mi = e / KT; ki = e % KT; }   // K contiguous in memory
+  };
+  auto b_coord = [&](int q, int &nj, int &ki) {
+    const int e = tid + q * NTHREADS;
+    if (TB == OP_N) { nj = e / KT; ki = e % KT; }   // K contiguous
+    else            { nj = e % NT; ki = e / NT; }   // N contiguous
+  };
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < EA; q++) {
+      int mi, ki;
+      a_coord(q, mi, ki);
+      const int gi = m0 + mi, gk = k0 + ki;
+      T v = from_real<T>(0.0);
+      if (gi < p.m && gk < kend) {
+        v = fetch<T, TA>(p.A, p.lda, gi, gk);
+        if (p.tri_a) v = tri_mask(v, p.tri_a, p.unit, gi, gk);
+      }
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      int nj, ki;
+      b_coord(q, nj, ki);
+      const int gj = n0 + nj, gk = k0 + ki;
+      T v = from_real<T>(0.0);
+      if (gj < p.n && gk < kend) {
+        v = fetch<T, TB>(p.B, p.ldb, gk, gj);
+        if (p.tri_b) v = tri_mask(v, p.tri_b, p.unit, gk, gj);
+      }
+      rb[q] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < EA; q++) {
+      int mi, ki;
+      a_coord(q, mi, ki);
+      lds_put(sA, PSA, ki * LSA + mi, ra[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      int nj, ki;
+      b_coord(q, nj, ki);
+      lds_put(sB, PSB, ki * LSB + nj, rb[q]);
+    }
+  };
+
+  if (kbeg < kend) {
+    load(kbeg);
+    store();
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += KT) {
+      const bool more = k0 + KT < kend;
+      if (more) load(k0 + KT);
+#pragma unroll
+      for (int ks = 0; ks < KT; ks += 4) {
+        const int kr = ks + lg;
+        double ar[BI], br[BJ];
+#pragma unroll
+        for (int i = 0; i < BI; i++) ar[i] = sA[kr * LSA + wm0 + 16 * i + lc];
+#pragma unroll
+        for (int j = 0; j < BJ; j++) br[j] = sB[kr * LSB + wn0 + 16 * j + lc];
+        if constexpr (!CPLX) {
+#pragma unroll
+          for (int i = 0; i < BI; i++)
+#pragma unroll
+            for (int j = 0; j < BJ; j++)
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ar[i], acc[0][i][j], 0, 0, 0);
+        } else {
+          double ai[BI], bi[BJ];
+#pragma unroll
+          for (int i = 0; i < BI; i++) ai[i] = sA[PSA + kr * LSA + wm0 + 16 * i + lc];
+#pragma unroll
+          for (int j = 0; j < BJ; j++) bi[j] = sB[PSB + kr * LSB + wn0 + 16 * j + lc];
+#pragma unroll
+          for (int i = 0; i < BI; i++)
+#pragma unroll
+            for (int j = 0; j < BJ; j++) {
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ar[i], acc[0][i][j], 0, 0, 0);
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bi[j], ai[i], acc[0][i][j], 0, 0, 0);
+              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[j], ar[i], acc[1][i][j], 0, 0, 0);
+              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ai[i], acc[1][i][j], 0, 0, 0);
+            }
+        }
+      }
+      __syncthreads();
+      if (more) {
+        store();
+        __syncthreads();
+      }
+    }
+  }
+
+  // ---- epilogue: lane lc holds row m, register r holds column 4*lg + r ----
+  const bool use_beta = !is_zero(p.beta);
+#pragma unroll
+  for (int i = 0; i < BI; i++) {
+    const int gm = m0 + wm0 + 16 * i + lc;
+    if (gm >= p.m) continue;
+#pragma unroll
+    for (int j = 0; j < BJ; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gn = n0 + wn0 + 16 * j + 4 * lg + r;
+        if (gn >= p.n) continue;
+        if (p.tri_out == TRI_LOWER && gm < gn) continue;
+        if (p.tri_out == TRI_UPPER && gm > gn) continue;
+        T v;
+        if constexpr (CPLX) v = make_zd(acc[0][i][j][r], acc[1][i][j][r]);
+        else v = acc[0][i][j][r];
+        v = p.alpha * v;
+        if (use_beta) v = v + p.beta * p.C[(long)gn * p.ldc + gm];
+        if constexpr (CPLX) {
+          if (p.real_diag && gm == gn) v.y = 0.0;
+        }
+        p.D[(long)gn * p.ldd + gm] = v;
+      }
+    }
+  }
+}
+
+template <typename T, int TA, int TB, int MT, int NT>
+hipError_t launch(hipStream_t s, const KParams<T> &p) {
+  dim3 grid;
+  if (p.tri_grid) {
+    const long t = (p.m + MT - 1) / MT;
+    grid = dim3((unsigned)(t * (t + 1) / 2));
+  } else {
+    grid = dim3((unsigned)((p.m + MT - 1) / MT), (unsigned)((p.n + NT - 1) / NT));
+  }
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, NT>), grid, dim3(NTHREADS), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, int TA, int TB>
+hipError_t dispatch_tile(hipStream_t s, const KParams<T> &p, int mt, int nt) {
+  constexpr bool CPLX = is_complex<T>::value;
+  constexpr int BIG = CPLX ? 64 : 128, SMALL = CPLX ? 32 : 64;
+  if (mt == BIG && nt == BIG) return launch<T, TA, TB, BIG, BIG>(s, p);
+  if (mt == SMALL && nt == SMALL) return launch<T, TA, TB, SMALL, SMALL>(s, p);
+  if (mt == SMALL && nt == BIG) return launch<T, TA, TB, SMALL, BIG>(s, p);
+  return launch<T, TA, TB, BIG, SMALL>(s, p);
+}
+
+template <typename T>
+hipError_t dispatch_ops(hipStream_t s, int ta, int tb, const KParams<T> &p, int mt, int nt) {
+  if constexpr (!is_complex<T>::value) {
+    // for real data C and T coincide
+    const bool a = ta != OP_N, b = tb != OP_N;
+    if (!a && !b) return dispatch_tile<T, OP_N, OP_N>(s, p, mt, nt);
+    if (!a && b) return dispatch_tile<T, OP_N, OP_T>(s, p, mt, nt);
+    if (a && !b) return dispatch_tile<T, OP_T, OP_N>(s, p, mt, nt);
+    return dispatch_tile<T, OP_T, OP_T>(s, p, mt, nt);
+  } else {
+#define HC_OPS_CASE(X, Y) \
+    if (ta == X && tb == Y) return dispatch_tile<T, X, Y>(s, p, mt, nt);
+    HC_OPS_CASE(OP_N, OP_N) HC_OPS_CASE(OP_N, OP_T) HC_OPS_CASE(OP_N, OP_C)
+    HC_OPS_CASE(OP_T, OP_N) HC_OPS_CASE(OP_T, OP_T) HC_OPS_CASE(OP_T, OP_C)
+    HC_OPS_CASE(OP_C, OP_N) HC_OPS_CASE(OP_C, OP_T) HC_OPS_CASE(OP_C, OP_C)
+#undef HC_OPS_CASE
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+__global__ void copy2d_kernel(int m, int n, const T *S, long lds, T *D, long ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < m && j < n) D[(long)j * ldd + i] = S[(long)j * lds + i];
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
+                long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
+                const GemmOpts &o) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  KParams<T> p;
+  p.m = (int)m; p.n = (int)n; p.k = (int)(k > 0 ? k : 0);
+  p.alpha = alpha; p.beta = beta;
+  p.A = A; p.lda = lda; p.B = B; p.ldb = ldb; p.C = C; p.ldc = ldc; p.D = D; p.ldd = ldd;
+  p.tri_out = o.tri_out; p.tri_grid = o.tri_grid ? 1 : 0; p.real_diag = o.real_diag ? 1 : 0;
+  p.tri_a = o.tri_a; p.tri_b = o.tri_b; p.unit = o.unit ? 1 : 0;
+  p.abort_flag = o.abort_flag;
+
+  // Tile choice: the large tile when it still yields about one workgroup per
+  // CU (256 CUs), otherwise the small one; in-place callers may demand a tile
+  // that spans a whole panel in M or N.
+  constexpr bool CPLX = is_complex<T>::value;
+  constexpr int BIG = CPLX ? 64 : 128, SMALL = CPLX ? 32 : 64;
+  const long tb_m = (m + BIG - 1) / BIG, tb_n = (n + BIG - 1) / BIG;
+  const long tiles_big = o.tri_grid ? tb_m * (tb_m + 1) / 2 : tb_m * tb_n;
+  int mt = tiles_big >= 240 ? BIG : SMALL, nt = mt;
+  if (o.min_mt > mt) mt = BIG;
+  if (o.min_nt > nt) nt = BIG;
+  if (o.min_mt > BIG || o.min_nt > BIG) return hipErrorInvalidValue;
+  if (o.tri_grid) {
+    if (m != n) return hipErrorInvalidValue;
+    mt = nt = (mt > nt ? mt : nt);
+  }
+  return dispatch_ops<T>(s, ta, tb, p, mt, nt);
+}
+
+template <typename T>
+hipError_t copy2d(hipStream_t s, long m, long n, const T *S, long lds, T *D, long ldd) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  dim3 grid((unsigned)((m + 255) / 256), (unsigned)n);
+  hipLaunchKernelGGL((copy2d_kernel<T>), grid, dim3(256), 0, s, (int)m, (int)n, S, lds, D, ldd);
+  return hipGetLastError();
+}
+
+template hipError_t gemm<double>(hipStream_t, int, int, long, long, long, double, const double *,
+                                 long, const double *, long, double, const double *, long,
+                                 double *, long, const GemmOpts &);
+template hipError_t gemm<zd>(hipStream_t, int, int, long, long, long, zd, const zd *, long,
+                             const zd *, long, zd, const zd *, long, zd *, long,
+                             const GemmOpts &);
+template hipError_t copy2d<double>(hipStream_t, long, long, const double *, long, double *, long);
+template hipError_t copy2d<zd>(hipStream_t, long, long, const zd *, long, zd *, long);
+
+}  // namespace hc

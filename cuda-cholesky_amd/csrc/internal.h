@@ -1,0 +1,201 @@
+// Internal declarations of the MI355X Cholesky library (not installed).
+//
+// Device scalar types, the tile-GEMM parameter block shared by every BLAS-3
+// operation, the host launchers, the handle structures and the error helpers
+// behind the public C ABI in include/blas.h and include/lapack.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+// hip_runtime.h maps CUDA_SUCCESS onto hipSuccess by macro; the public ABI
+// defines CUDA_SUCCESS as its own enumerator.
+#undef CUDA_SUCCESS
+
+#include "blas.h"
+#include "lapack.h"
+#include "error.h"
+
+namespace hc {
+
+// ---------------------------------------------------------------------------
+// scalar types
+// ---------------------------------------------------------------------------
+
+// Double complex, layout-compatible with C99 `double complex`.
+struct __attribute__((aligned(16))) zd {
+  double x, y;
+};
+
+__host__ __device__ inline zd make_zd(double re, double im) { zd r; r.x = re; r.y = im; return r; }
+__host__ __device__ inline double conjx(double a) { return a; }
+__host__ __device__ inline zd conjx(zd a) { return make_zd(a.x, -a.y); }
+__host__ __device__ inline double realx(double a) { return a; }
+__host__ __device__ inline double realx(zd a) { return a.x; }
+__host__ __device__ inline double imagx(double) { return 0.0; }
+__host__ __device__ inline double imagx(zd a) { return a.y; }
+__host__ __device__ inline zd operator+(zd a, zd b) { return make_zd(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline zd operator-(zd a, zd b) { return make_zd(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline zd operator-(zd a) { return make_zd(-a.x, -a.y); }
+__host__ __device__ inline zd operator*(zd a, zd b) {
+  return make_zd(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline zd operator*(double a, zd b) { return make_zd(a * b.x, a * b.y); }
+__host__ __device__ inline zd operator/(zd a, double b) { return make_zd(a.x / b, a.y / b); }
+// Complex division by the scaled method (Smith, "Algorithm 116", CACM 1962).
+__host__ __device__ inline zd operator/(zd a, zd b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, d = b.x + b.y * r;
+    return make_zd((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  }
+  const double r = b.x / b.y, d = b.y + b.x * r;
+  return make_zd((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+__host__ __device__ inline bool is_zero(double a) { return a == 0.0; }
+__host__ __device__ inline bool is_zero(zd a) { return a.x == 0.0 && a.y == 0.0; }
+__host__ __device__ inline bool is_one(double a) { return a == 1.0; }
+__host__ __device__ inline bool is_one(zd a) { return a.x == 1.0 && a.y == 0.0; }
+
+template <typename T> __host__ __device__ inline T from_real(double v);
+template <> __host__ __device__ inline double from_real<double>(double v) { return v; }
+template <> __host__ __device__ inline zd from_real<zd>(double v) { return make_zd(v, 0.0); }
+
+template <typename T> struct is_complex { static constexpr bool value = false; };
+template <> struct is_complex<zd> { static constexpr bool value = true; };
+
+// ---------------------------------------------------------------------------
+// operand codes
+// ---------------------------------------------------------------------------
+enum Op { OP_N = 0, OP_T = 1, OP_C = 2 };
+enum Tri { TRI_NONE = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
+
+inline int op_code(char t) { return t == 'N' ? OP_N : (t == 'T' ? OP_T : OP_C); }
+
+// ---------------------------------------------------------------------------
+// tile GEMM: D <- alpha * op(A) * op(B) + beta * C
+// ---------------------------------------------------------------------------
+// Options turn the one kernel into GEMM, SYRK/HERK (tri_out + tri_grid),
+// TRMM (tri_a / tri_b masks) and the inverse-based TRSM step.
+struct GemmOpts {
+  int tri_out = TRI_NONE;   // write only row >= col (lower) or row <= col (upper)
+  bool tri_grid = false;    // launch only tiles meeting the tri_out triangle (m == n)
+  bool real_diag = false;   // HERK: force the diagonal of D to be real
+  int tri_a = TRI_NONE;     // op(A) is triangular (lower- or upper-like): TRMM left
+  int tri_b = TRI_NONE;     // op(B) is triangular: TRMM right
+  bool unit = false;        // the triangular operand has an implicit unit diagonal
+  int min_mt = 0;           // the tile must span at least this many rows of D
+  int min_nt = 0;           // ... or columns of D (in-place safety)
+  const int *abort_flag = nullptr;  // skip all work when *abort_flag != 0
+};
+
+template <typename T>
+hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
+                long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
+                const GemmOpts &o = GemmOpts());
+
+// 2-D copy D <- S (m x n)
+template <typename T>
+hipError_t copy2d(hipStream_t s, long m, long n, const T *S, long lds, T *D, long ldd);
+
+// ---------------------------------------------------------------------------
+// level-3 building blocks (blas_ops.cpp)
+// ---------------------------------------------------------------------------
+// Workspace supplier: a stream-ordered device allocation of `bytes`.
+struct Workspace {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Out-of-place triangular multiply X = alpha op(A) B (side L) or alpha B op(A).
+template <typename T>
+hipError_t trmm(hipStream_t s, char side, char uplo, char trans, char diag, long m, long n,
+                T alpha, const T *A, long lda, const T *B, long ldb, T *X, long ldx);
+
+// In-place triangular solve.  `ws` must hold trsm_ws_elems(side, m, n) elements.
+long trsm_ws_elems(char side, long m, long n);
+template <typename T>
+hipError_t trsm(hipStream_t s, char side, char uplo, char trans, char diag, long m, long n,
+                T alpha, const T *A, long lda, T *B, long ldb, T *ws);
+
+// SYRK (real) / HERK (complex, real alpha/beta): one triangle of C.
+template <typename T>
+hipError_t herk(hipStream_t s, char uplo, char trans, long n, long k, double alpha,
+                const T *A, long lda, double beta, T *C, long ldc,
+                const int *abort_flag = nullptr);
+
+// ---------------------------------------------------------------------------
+// diagonal-block kernels (diag.hip); the block is held in LDS
+// ---------------------------------------------------------------------------
+template <typename T> constexpr int diag_max() { return is_complex<T>::value ? 64 : 128; }
+
+// Factor the n x n diagonal block in place (n <= diag_max<T>()), write
+// W = inv(L) densely (ld ldw) where L is the lower form of the factor
+// (L = U^H for upper), and on a non-positive pivot set *info = info_off + k + 1.
+template <typename T>
+hipError_t diag_potrf(hipStream_t s, char uplo, int n, T *A, long lda, T *W, long ldw, int *info,
+                      long info_off);
+// Invert the triangular block in place (B = inv(A), B may equal A) and, when
+// W != nullptr, also write inv of its lower form densely into W.
+template <typename T>
+hipError_t diag_trtri(hipStream_t s, char uplo, char diag, int n, const T *A, long lda, T *B,
+                      long ldb, T *W, long ldw, const int *abort_flag);
+// In-place U U^H (upper) or L^H L (lower) of a diagonal block.
+template <typename T>
+hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda);
+// Batched inverses of the nb x nb diagonal blocks of op(A) (order m) into W,
+// block b at W + b * nb * nb with leading dimension nb.
+template <typename T>
+hipError_t trsm_diag_inverses(hipStream_t s, char uplo, char trans, char diag, long m, int nb,
+                              const T *A, long lda, T *W);
+// Scan the diagonal for exact zeros; *info = 1-based index of the first
+// (upper) or last (lower) zero, 0 when none.
+template <typename T>
+hipError_t diag_zero_scan(hipStream_t s, char uplo, long n, const T *A, long lda, int *info);
+// 2 * sum(log(Re x[i*incx])) into *out (device), compensated summation.
+template <typename T>
+hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials, double *out);
+constexpr int kLogdetMaxBlocks = 1024;
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+CUresult to_cu(hipError_t e);
+void report(const char *call, const char *func, const char *file, int line, CUresult err);
+
+}  // namespace hc
+
+#define HC_CHECK(call)                                                        \
+  do {                                                                        \
+    hipError_t hc_e_ = (call);                                                \
+    if (hc_e_ != hipSuccess) {                                                \
+      CUresult hc_r_ = hc::to_cu(hc_e_);                                      \
+      hc::report(#call, __func__, __FILE__, __LINE__, hc_r_);                 \
+      return hc_r_;                                                           \
+    }                                                                         \
+  } while (0)
+
+// Same, inside functions that return hipError_t.
+#define HC_TRY(call)                                                          \
+  do {                                                                        \
+    hipError_t hc_e_ = (call);                                                \
+    if (hc_e_ != hipSuccess) return hc_e_;                                    \
+  } while (0)
+
+// Public handle state.
+struct __cublashandle_st {
+  int device;
+};
+
+struct __culapackhandle_st {
+  CUBLAShandle blas;
+  int device;
+  hipStream_t stream;     // main stream (trailing updates)
+  hipStream_t side;       // look-ahead stream (diagonal block and panel)
+  hipEvent_t ev_panel;    // panel ready
+  hipEvent_t ev_update;   // next panel column updated
+  void *ws;               // device workspace, grown on demand
+  size_t ws_bytes;
+  int *dinfo;             // device info word
+  int *hinfo;             // pinned host copy
+};

@@ -1,0 +1,351 @@
+// Public LAPACK entry points (include/lapack.h): xPOTRF, xTRTRI, xTRTI22,
+// xLAUUM, xPOTRI and xLOGDET for double and double complex.
+//
+// Every step runs on the GPU.  The reference copies each diagonal block to the
+// host, factors it there and copies it back (lapack/dpotrf.c:263-275); here a
+// single-workgroup LDS kernel factors the block and also returns its inverse,
+// so the panel solve becomes a matrix-core GEMM and the loop never waits for
+// the host.  `info` lives in device memory; kernels issued after a failed
+// pivot see it and do nothing, and one copy at the end returns it.
+#include "api_util.h"
+
+#include <algorithm>
+
+using hc::bad_arg;
+using hc::dptr;
+using hc::zd;
+
+namespace {
+
+// Handle workspace of at least `bytes`, grown (never shrunk) on demand.
+hipError_t workspace(CULAPACKhandle h, size_t bytes, void **out) {
+  if (bytes > h->ws_bytes) {
+    if (h->ws != nullptr) {
+      HC_TRY(hipStreamSynchronize(h->stream));
+      HC_TRY(hipStreamSynchronize(h->side));
+      HC_TRY(hipFree(h->ws));
+      h->ws = nullptr;
+      h->ws_bytes = 0;
+    }
+    bytes = (bytes + 0xFFFFF) & ~size_t(0xFFFFF);  // 1 MiB granules
+    HC_TRY(hipMalloc(&h->ws, bytes));
+    h->ws_bytes = bytes;
+  }
+  *out = h->ws;
+  return hipSuccess;
+}
+
+template <typename T> constexpr int conj_op() { return hc::is_complex<T>::value ? hc::OP_C : hc::OP_T; }
+
+// Outer panel width of the right-looking factorisation: the trailing update
+// has inner dimension NB, i.e. NB / 8 flops per byte of the trailing matrix
+// it streams (double), enough to stay matrix-core bound.
+template <typename T> constexpr long potrf_nb() { return hc::is_complex<T>::value ? 128 : 256; }
+
+// Right-looking blocked Cholesky (the reference's loop is left-looking,
+// lapack/dpotrf.c:248-286; both produce the same factor).  Each outer panel
+// of width NB is factored in IB-column slices: diagonal block (LDS kernel,
+// factor + inverse), panel solve as a GEMM with the inverse, and the
+// remaining panel columns updated; then one SYRK/HERK updates the trailing
+// matrix with the whole panel.
+template <typename T>
+hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W) {
+  constexpr int IB = hc::diag_max<T>();
+  constexpr long NB = potrf_nb<T>();
+  hipStream_t s = h->stream;
+  const bool lower = (uplo == 'L');
+  const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0), zero = hc::from_real<T>(0.0);
+  auto at = [&](long r, long c) { return A + c * lda + r; };
+  for (long j = 0; j < n; j += NB) {
+    const long jb = std::min(NB, n - j);
+    for (long jj = j; jj < j + jb; jj += IB) {
+      const int ib = (int)std::min<long>(IB, j + jb - jj);
+      HC_TRY(hc::diag_potrf<T>(s, uplo, ib, at(jj, jj), lda, W, IB, h->dinfo, jj));
+      const long r0 = jj + ib, rest = n - r0;
+      hc::GemmOpts o;
+      o.abort_flag = h->dinfo;
+      if (rest > 0) {
+        if (lower) {  // L21 <- A21 inv(L11)^H, in place: a tile spans all ib columns
+          o.min_nt = ib;
+          HC_TRY(hc::gemm<T>(s, hc::OP_N, conj_op<T>(), rest, ib, ib, one, at(r0, jj), lda, W, IB, zero,
+                             nullptr, 0, at(r0, jj), lda, o));
+        } else {      // U12 <- inv(U11^H) A12, in place: a tile spans all ib rows
+          o.min_mt = ib;
+          HC_TRY(hc::gemm<T>(s, hc::OP_N, hc::OP_N, ib, rest, ib, one, W, IB, at(jj, r0), lda, zero,
+                             nullptr, 0, at(jj, r0), lda, o));
+        }
+      }
+      const long rem = j + jb - r0;  // panel columns still to update
+      if (rem > 0) {
+        hc::GemmOpts u;
+        u.abort_flag = h->dinfo;
+        if (lower) {
+          u.tri_out = hc::TRI_LOWER;
+          HC_TRY(hc::gemm<T>(s, hc::OP_N, conj_op<T>(), rest, rem, ib, mone, at(r0, jj), lda,
+                             at(r0, jj), lda, one, at(r0, r0), lda, at(r0, r0), lda, u));
+        } else {
+          u.tri_out = hc::TRI_UPPER;
+          HC_TRY(hc::gemm<T>(s, conj_op<T>(), hc::OP_N, rem, rest, ib, mone, at(jj, r0), lda,
+                             at(jj, r0), lda, one, at(r0, r0), lda, at(r0, r0), lda, u));
+        }
+      }
+    }
+    const long t = n - j - jb;
+    if (t > 0) {
+      if (lower)
+        HC_TRY(hc::herk<T>(s, 'L', 'N', t, jb, -1.0, at(j + jb, j), lda, 1.0, at(j + jb, j + jb), lda,
+                           h->dinfo));
+      else
+        HC_TRY(hc::herk<T>(s, 'U', 'C', t, jb, -1.0, at(j, j + jb), lda, 1.0, at(j + jb, j + jb), lda,
+                           h->dinfo));
+    }
+  }
+  return hipSuccess;
+}
+
+// Recursive triangular inverse: inv([A11 0; A21 A22]) =
+// [X11 0; -X22 A21 X11  X22] (lower; upper is the mirror).  The reference's
+// loop (lapack/dtrtri.c:360-458) walks nb-column blocks; halving instead keeps
+// the two TRMMs per level large.  X holds an n2 x n1 intermediate.
+template <typename T>
+hipError_t trtri_rec(hipStream_t s, char uplo, char diag, long n, T *A, long lda, T *X) {
+  constexpr int IB = hc::diag_max<T>();
+  if (n <= IB) return hc::diag_trtri<T>(s, uplo, diag, (int)n, A, lda, A, lda, nullptr, 0, nullptr);
+  const long n1 = ((n / 2 + IB - 1) / IB) * IB, n2 = n - n1;
+  T *A11 = A, *A22 = A + n1 * lda + n1;
+  HC_TRY(trtri_rec(s, uplo, diag, n1, A11, lda, X));
+  HC_TRY(trtri_rec(s, uplo, diag, n2, A22, lda, X));
+  const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0);
+  if (uplo == 'L') {
+    T *A21 = A + n1;
+    HC_TRY(hc::trmm<T>(s, 'L', 'L', 'N', diag, n2, n1, mone, A22, lda, A21, lda, X, n2));
+    return hc::trmm<T>(s, 'R', 'L', 'N', diag, n2, n1, one, A11, lda, X, n2, A21, lda);
+  }
+  T *A12 = A + n1 * lda;
+  HC_TRY(hc::trmm<T>(s, 'L', 'U', 'N', diag, n1, n2, mone, A11, lda, A12, lda, X, n1));
+  return hc::trmm<T>(s, 'R', 'U', 'N', diag, n1, n2, one, A22, lda, X, n1, A12, lda);
+}
+
+// Recursive triangular square: lower L^H L =
+// [L11^H L11 + L21^H L21, .; L22^H L21, L22^H L22]; upper U U^H mirrors it.
+// The reference's loop is lapack/dlauum.c:188-281.
+template <typename T>
+hipError_t lauum_rec(hipStream_t s, char uplo, long n, T *A, long lda, T *X) {
+  constexpr int IB = hc::diag_max<T>();
+  if (n <= IB) return hc::diag_lauum<T>(s, uplo, (int)n, A, lda);
+  const long n1 = ((n / 2 + IB - 1) / IB) * IB, n2 = n - n1;
+  T *A11 = A, *A22 = A + n1 * lda + n1;
+  const T one = hc::from_real<T>(1.0);
+  HC_TRY(lauum_rec(s, uplo, n1, A11, lda, X));
+  if (uplo == 'L') {
+    T *A21 = A + n1;
+    HC_TRY(hc::herk<T>(s, 'L', 'C', n1, n2, 1.0, A21, lda, 1.0, A11, lda));
+    HC_TRY(hc::trmm<T>(s, 'L', 'L', 'C', 'N', n2, n1, one, A22, lda, A21, lda, X, n2));
+    HC_TRY(hc::copy2d<T>(s, n2, n1, X, n2, A21, lda));
+  } else {
+    T *A12 = A + n1 * lda;
+    HC_TRY(hc::herk<T>(s, 'U', 'N', n1, n2, 1.0, A12, lda, 1.0, A11, lda));
+    HC_TRY(hc::trmm<T>(s, 'R', 'U', 'C', 'N', n1, n2, one, A22, lda, A12, lda, X, n1));
+    HC_TRY(hc::copy2d<T>(s, n1, n2, X, n1, A12, lda));
+  }
+  return lauum_rec(s, uplo, n2, A22, lda, X);
+}
+
+template <typename T>
+size_t rec_ws_bytes(long n) {
+  constexpr int IB = hc::diag_max<T>();
+  if (n <= IB) return 0;
+  const long n1 = ((n / 2 + IB - 1) / IB) * IB;
+  return (size_t)n1 * (size_t)(n - n1) * sizeof(T);
+}
+
+// Copies the device info word back and waits for the handle's streams.
+CUresult finish(CULAPACKhandle h, long *info) {
+  HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HC_CHECK(hipStreamSynchronize(h->stream));
+  *info = (long)h->hinfo[0];
+  return CUDA_SUCCESS;
+}
+
+template <typename T>
+CUresult potrf_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A,
+                   size_t lda, long *info) {
+  // reference lapack/dpotrf.c:173-182
+  if (info == nullptr) return bad_arg(name, 6);
+  *info = 0;
+  if (h == nullptr) { *info = -1; return bad_arg(name, 1); }
+  if (!hc::valid_uplo(uplo)) { *info = -2; return bad_arg(name, 2); }
+  if (lda < n) { *info = -4; return bad_arg(name, 4); }
+  if (n == 0) return CUDA_SUCCESS;
+  hc::DeviceGuard g(h->device);
+  HC_CHECK(g.status());
+  void *ws = nullptr;
+  HC_CHECK(workspace(h, (size_t)hc::diag_max<T>() * hc::diag_max<T>() * sizeof(T), &ws));
+  HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), h->stream));
+  HC_CHECK(potrf_device<T>(h, (char)uplo, (long)n, dptr<T>(A), (long)lda, static_cast<T *>(ws)));
+  return finish(h, info);
+}
+
+template <typename T>
+CUresult trtri_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n,
+                   CUdeviceptr A, size_t lda, long *info) {
+  // reference lapack/dtrtri.c:331-340 (info = -5 for lda)
+  if (info == nullptr) return bad_arg(name, 7);
+  *info = 0;
+  if (h == nullptr) { *info = -1; return bad_arg(name, 1); }
+  if (!hc::valid_uplo(uplo)) { *info = -2; return bad_arg(name, 2); }
+  if (!hc::valid_diag(diag)) { *info = -3; return bad_arg(name, 3); }
+  if (lda < n) { *info = -5; return bad_arg(name, 5); }
+  if (n == 0) return CUDA_SUCCESS;
+  hc::DeviceGuard g(h->device);
+  HC_CHECK(g.status());
+  T *Ap = dptr<T>(A);
+  if (diag == CBlasNonUnit) {
+    HC_CHECK(hc::diag_zero_scan<T>(h->stream, (char)uplo, (long)n, Ap, (long)lda, h->dinfo));
+    CUresult r = finish(h, info);
+    if (r != CUDA_SUCCESS || *info != 0) return r;
+  }
+  void *ws = nullptr;
+  HC_CHECK(workspace(h, rec_ws_bytes<T>((long)n) + 256, &ws));
+  HC_CHECK(trtri_rec<T>(h->stream, (char)uplo, (char)diag, (long)n, Ap, (long)lda, static_cast<T *>(ws)));
+  HC_CHECK(hipStreamSynchronize(h->stream));
+  return CUDA_SUCCESS;
+}
+
+template <typename T>
+CUresult lauum_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A,
+                   size_t lda, long *info) {
+  // reference lapack/dlauum.c:159-168
+  if (info == nullptr) return bad_arg(name, 6);
+  *info = 0;
+  if (h == nullptr) { *info = -1; return bad_arg(name, 1); }
+  if (!hc::valid_uplo(uplo)) { *info = -2; return bad_arg(name, 2); }
+  if (lda < n) { *info = -4; return bad_arg(name, 4); }
+  if (n == 0) return CUDA_SUCCESS;
+  hc::DeviceGuard g(h->device);
+  HC_CHECK(g.status());
+  void *ws = nullptr;
+  HC_CHECK(workspace(h, rec_ws_bytes<T>((long)n) + 256, &ws));
+  HC_CHECK(lauum_rec<T>(h->stream, (char)uplo, (long)n, dptr<T>(A), (long)lda, static_cast<T *>(ws)));
+  HC_CHECK(hipStreamSynchronize(h->stream));
+  return CUDA_SUCCESS;
+}
+
+template <typename T>
+CUresult potri_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A,
+                   size_t lda, long *info) {
+  // reference lapack/dpotri.c:25-46: TRTRI (non-unit) then LAUUM
+  if (info == nullptr) return bad_arg(name, 6);
+  *info = 0;
+  if (h == nullptr) { *info = -1; return bad_arg(name, 1); }
+  if (!hc::valid_uplo(uplo)) { *info = -2; return bad_arg(name, 2); }
+  if (lda < n) { *info = -4; return bad_arg(name, 4); }
+  if (n == 0) return CUDA_SUCCESS;
+  CUresult r = trtri_api<T>(name, h, uplo, CBlasNonUnit, n, A, lda, info);
+  if (r != CUDA_SUCCESS || *info != 0) return r;
+  return lauum_api<T>(name, h, uplo, n, A, lda, info);
+}
+
+template <typename T>
+CUresult trti22_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n,
+                    CUdeviceptr A, size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr info,
+                    CUstream stream) {
+  // reference lapack/dtrtri.c:301-324 (there n <= 32; here one LDS block)
+  if (h == nullptr) return bad_arg(name, 1);
+  if (!hc::valid_uplo(uplo)) return bad_arg(name, 2);
+  if (!hc::valid_diag(diag)) return bad_arg(name, 3);
+  if (n > (size_t)hc::diag_max<T>()) return CUDA_ERROR_INVALID_VALUE;
+  if (lda < n) return bad_arg(name, 6);
+  if (ldb < n) return bad_arg(name, 8);
+  hc::DeviceGuard g(h->device);
+  HC_CHECK(g.status());
+  int *dinfo = dptr<int>(info);
+  if (n == 0) {
+    if (dinfo != nullptr) HC_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int), stream));
+    return CUDA_SUCCESS;
+  }
+  if (dinfo != nullptr) {
+    if (diag == CBlasNonUnit)
+      HC_CHECK(hc::diag_zero_scan<T>(stream, (char)uplo, (long)n, dptr<const T>(A), (long)lda, dinfo));
+    else
+      HC_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int), stream));
+  }
+  HC_CHECK(hc::diag_trtri<T>(stream, (char)uplo, (char)diag, (int)n, dptr<const T>(A), (long)lda,
+                             dptr<T>(B), (long)ldb, nullptr, 0, dinfo));
+  return CUDA_SUCCESS;
+}
+
+template <typename T>
+CUresult logdet_api(const char *name, CULAPACKhandle h, CUdeviceptr x, size_t incx, size_t n,
+                    double *result, CUstream stream) {
+  // reference lapack/dlogdet.c:37-74
+  if (result == nullptr) return bad_arg(name, 5);
+  if (n == 0) { *result = 0.0; return CUDA_SUCCESS; }
+  if (h == nullptr) return bad_arg(name, 1);
+  if (incx == 0) return bad_arg(name, 3);
+  hc::DeviceGuard g(h->device);
+  HC_CHECK(g.status());
+  // Partial sums and the result sit behind the info word in the handle's
+  // small device buffer area: a separate allocation would be per call.
+  void *ws = nullptr;
+  HC_CHECK(workspace(h, (2 * hc::kLogdetMaxBlocks + 8) * sizeof(double), &ws));
+  double *partials = static_cast<double *>(ws);
+  double *out = partials + 2 * hc::kLogdetMaxBlocks;
+  // The handle workspace is ordered on the handle stream; join the caller's.
+  HC_CHECK(hipStreamSynchronize(h->stream));
+  HC_CHECK(hc::logdet<T>(stream, dptr<const T>(x), (long)incx, (long)n, partials, out));
+  double *hres = reinterpret_cast<double *>(h->hinfo) + 1;
+  HC_CHECK(hipMemcpyAsync(hres, out, sizeof(double), hipMemcpyDeviceToHost, stream));
+  HC_CHECK(hipStreamSynchronize(stream));
+  *result = *hres;
+  return CUDA_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+CUresult cuDpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return potrf_api<double>("cuDpotrf", h, uplo, n, A, lda, info);
+}
+CUresult cuZpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return potrf_api<zd>("cuZpotrf", h, uplo, n, A, lda, info);
+}
+CUresult cuDtrtri(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A,
+                  size_t lda, long *info) {
+  return trtri_api<double>("cuDtrtri", h, uplo, diag, n, A, lda, info);
+}
+CUresult cuZtrtri(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A,
+                  size_t lda, long *info) {
+  return trtri_api<zd>("cuZtrtri", h, uplo, diag, n, A, lda, info);
+}
+CUresult cuDtrti22(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A,
+                   size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr info, CUstream stream) {
+  return trti22_api<double>("cuDtrti22", h, uplo, diag, n, A, lda, B, ldb, info, stream);
+}
+CUresult cuZtrti22(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A,
+                   size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr info, CUstream stream) {
+  return trti22_api<zd>("cuZtrti22", h, uplo, diag, n, A, lda, B, ldb, info, stream);
+}
+CUresult cuDlauum(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return lauum_api<double>("cuDlauum", h, uplo, n, A, lda, info);
+}
+CUresult cuZlauum(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return lauum_api<zd>("cuZlauum", h, uplo, n, A, lda, info);
+}
+CUresult cuDpotri(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return potri_api<double>("cuDpotri", h, uplo, n, A, lda, info);
+}
+CUresult cuZpotri(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return potri_api<zd>("cuZpotri", h, uplo, n, A, lda, info);
+}
+CUresult cuDlogdet(CULAPACKhandle h, CUdeviceptr x, size_t incx, size_t n, double *result,
+                   CUstream stream) {
+  return logdet_api<double>("cuDlogdet", h, x, incx, n, result, stream);
+}
+CUresult cuZlogdet(CULAPACKhandle h, CUdeviceptr x, size_t incx, size_t n, double *result,
+                   CUstream stream) {
+  return logdet_api<zd>("cuZlogdet", h, x, incx, n, result, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+/*
+ * GPU BLAS-3 entry points of the MI355X Cholesky library.
+ *
+ * Drop-in for the device-pointer ("cu") half of the reference's
+ * include/blas.h, double and double-complex precision: same names, argument
+ * order, enum values and error behaviour.  Each function replaces the
+ * reference function cited beside it.  Matrices are column-major; every
+ * CUdeviceptr is a device byte address.  All calls are asynchronous on the
+ * given stream.
+ *
+ * The host-memory CPU routines of the reference header (dgemm, dsyrk, ...)
+ * are not part of this library: the GPU path no longer factors diagonal
+ * blocks on the host, so it has no CPU BLAS inside.
+ */
+#ifndef BLAS_H
+#define BLAS_H
+
+#include <stddef.h>
+#include <stdbool.h>
+#include "cu_abi.h"
+
+#ifdef __cplusplus
+typedef _Complex double cuDoubleComplex_t;
+extern "C" {
+#else
+#include <complex.h>
+typedef double complex cuDoubleComplex_t;
+#endif
+
+/** BLAS enums, castable to char (reference include/blas.h:24-27). */
+typedef enum { CBlasNoTrans = 'N', CBlasTrans = 'T', CBlasConjTrans = 'C' } CBlasTranspose;
+typedef enum { CBlasLower = 'L', CBlasUpper = 'U' } CBlasUplo;
+typedef enum { CBlasLeft = 'L', CBlasRight = 'R' } CBlasSide;
+typedef enum { CBlasNonUnit = 'N', CBlasUnit = 'U' } CBlasDiag;
+
+/**
+ * Illegal-argument hook (reference include/blas.h:37-43, blas/xerbla.c:5-9).
+ * Called with the function name and the 1-based index of the bad argument.
+ * The default prints one line to stderr; NULL silences it.
+ */
+typedef void (*xerbla_t)(const char *, long);
+extern xerbla_t xerbla;
+
+/** Handle: binds a device and its kernels (reference include/blas.h:153-155). */
+typedef struct __cublashandle_st * CUBLAShandle;
+CUresult cuBLASCreate(CUBLAShandle *);
+CUresult cuBLASDestroy(CUBLAShandle);
+
+/* C <- alpha * op(A) * op(A)^T + beta * C, one triangle
+ * (replaces cuDsyrk, reference blas/dsyrk.c:163-209). */
+CUresult cuDsyrk(CUBLAShandle, CBlasUplo, CBlasTranspose,
+                 size_t, size_t,
+                 double, CUdeviceptr, size_t,
+                 double, CUdeviceptr, size_t, CUstream);
+
+/* C <- alpha * op(A) * op(A)^H + beta * C, one triangle, real alpha and beta;
+ * trans must be N or C (replaces cuZherk, reference blas/zherk.c:190). */
+CUresult cuZherk(CUBLAShandle, CBlasUplo, CBlasTranspose,
+                 size_t, size_t,
+                 double, CUdeviceptr, size_t,
+                 double, CUdeviceptr, size_t, CUstream);
+
+/* In-place matrix multiply (reference include/blas.h:183-201). */
+#define cuDgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream) \
+        cuDgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc, stream)
+#define cuZgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream) \
+        cuZgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc, stream)
+
+/* D <- alpha * op(A) * op(B) + beta * C (D may equal C)
+ * (replaces cuDgemm2, reference blas/dgemm.c:148-200). */
+CUresult cuDgemm2(CUBLAShandle, CBlasTranspose, CBlasTranspose,
+                  size_t, size_t, size_t,
+                  double, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  double, CUdeviceptr, size_t, CUdeviceptr, size_t, CUstream);
+/* (replaces cuZgemm2, reference blas/zgemm.c:226) */
+CUresult cuZgemm2(CUBLAShandle, CBlasTranspose, CBlasTranspose,
+                  size_t, size_t, size_t,
+                  cuDoubleComplex_t, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  cuDoubleComplex_t, CUdeviceptr, size_t, CUdeviceptr, size_t, CUstream);
+
+/* Out-of-place triangular multiply X <- alpha * op(A) * B or alpha * B * op(A)
+ * (replaces cuDtrmm2, reference blas/dtrmm.c:374-426). */
+CUresult cuDtrmm2(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                  size_t, size_t,
+                  double, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  CUdeviceptr, size_t, CUstream);
+/* (replaces cuZtrmm2, reference blas/ztrmm.c:422) */
+CUresult cuZtrmm2(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                  size_t, size_t,
+                  cuDoubleComplex_t, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  CUdeviceptr, size_t, CUstream);
+
+/* In-place triangular solve B <- alpha * op(A)^-1 * B or alpha * B * op(A)^-1
+ * (replaces cuDtrsm, reference blas/dtrsm.c:218-265). */
+CUresult cuDtrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                 size_t, size_t,
+                 double, CUdeviceptr, size_t,
+                 CUdeviceptr, size_t, CUstream);
+/* (replaces cuZtrsm, reference blas/ztrsm.c:248) */
+CUresult cuZtrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                 size_t, size_t,
+                 cuDoubleComplex_t, CUdeviceptr, size_t,
+                 CUdeviceptr, size_t, CUstream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,82 @@
+/*
+ * GPU LAPACK entry points of the MI355X Cholesky library.
+ *
+ * Drop-in for the device-pointer ("cu") half of the reference's
+ * include/lapack.h, double and double-complex precision: same names, argument
+ * order, info conventions and return codes.  Matrices are column-major and
+ * device resident (CUdeviceptr = device byte address).  Like the reference,
+ * the factorisation and inverse routines are synchronous: `info` is valid on
+ * return.  Positive info is a numerical failure reported with CUDA_SUCCESS;
+ * negative info is an illegal argument reported through xerbla with
+ * CUDA_ERROR_INVALID_VALUE.
+ *
+ * Unlike the reference, no diagonal block travels to the host: every step runs
+ * on the GPU, so the host CPU routines of the reference header are not part of
+ * this library.
+ */
+#ifndef LAPACK_H
+#define LAPACK_H
+
+#include "blas.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/** Handle (reference include/lapack.h:63-65). */
+typedef struct __culapackhandle_st * CULAPACKhandle;
+CUresult cuLAPACKCreate(CULAPACKhandle *);
+CUresult cuLAPACKDestroy(CULAPACKhandle);
+
+/* Cholesky factorisation A = L L^T (lower) or U^T U (upper), in place
+ * (replaces cuDpotrf, reference lapack/dpotrf.c:172-296). */
+CUresult cuDpotrf(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* Hermitian Cholesky A = L L^H or U^H U
+ * (replaces cuZpotrf, reference lapack/zpotrf.c:174-298). */
+CUresult cuZpotrf(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+
+/* In-place triangular inverse; info > 0 names an exactly zero diagonal
+ * (replaces cuDtrtri, reference lapack/dtrtri.c:326-468). */
+CUresult cuDtrtri(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, long *);
+/* (replaces cuZtrtri, reference lapack/ztrtri.c:326) */
+CUresult cuZtrtri(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, long *);
+
+/* In-place macros over the out-of-place single-block inverse
+ * (reference include/lapack.h:85-88). */
+#define cuDtrti2(handle, uplo, diag, n, A, lda, info, stream) cuDtrti22(handle, uplo, diag, n, A, lda, A, lda, info, stream)
+#define cuZtrti2(handle, uplo, diag, n, A, lda, info, stream) cuZtrti22(handle, uplo, diag, n, A, lda, A, lda, info, stream)
+
+/* Single-block triangular inverse B <- A^-1 of order n <= 128 (double) or
+ * n <= 64 (complex), asynchronous on `stream`; the int at device address `info`
+ * receives 0 or the 1-based index of a zero diagonal
+ * (replaces cuDtrti22, reference lapack/dtrtri.c:301-324, which accepts
+ * n <= 32). */
+CUresult cuDtrti22(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, CUdeviceptr, size_t, CUdeviceptr, CUstream);
+/* (replaces cuZtrti22, reference lapack/ztrtri.c:301) */
+CUresult cuZtrti22(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, CUdeviceptr, size_t, CUdeviceptr, CUstream);
+
+/* Triangular square U U^T (upper) or L^T L (lower), in place
+ * (replaces cuDlauum, reference lapack/dlauum.c:154-292). */
+CUresult cuDlauum(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* (replaces cuZlauum, reference lapack/zlauum.c:153) */
+CUresult cuZlauum(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+
+/* Inverse of an SPD matrix from its Cholesky factor: TRTRI then LAUUM
+ * (replaces cuDpotri, reference lapack/dpotri.c:25-46). */
+CUresult cuDpotri(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* (replaces cuZpotri, reference lapack/zpotri.c:25) */
+CUresult cuZpotri(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+
+/* 2 * sum(log(x[i * incx])), i < n; *result is valid on return
+ * (replaces cuDlogdet, reference lapack/dlogdet.c:37-74). */
+CUresult cuDlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, double *, CUstream);
+/* 2 * sum(log(Re x[i * incx])), computed in double precision
+ * (replaces cuZlogdet, reference lapack/zlogdet.c:38, which takes the logs in
+ * single precision). */
+CUresult cuZlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, double *, CUstream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

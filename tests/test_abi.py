@@ -1,0 +1,157 @@
+"""C-ABI boundary checks that need no GPU.
+
+* libcucholesky.so loads and exports every function and variable that
+  include/*.h declares;
+* the public headers compile as C99 and as C++ and keep the reference's enum
+  values;
+* argument errors are reported exactly as the reference reports them
+  (CUDA_ERROR_INVALID_VALUE, negative info, xerbla with the parameter index)
+  before any device work, so they run here.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared_functions():
+    names = set()
+    for h in ("blas.h", "lapack.h", "error.h"):
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
+        for m in re.finditer(r"^\s*(?:CUresult|const char \*)\s*(\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(cc):
+    L = cc.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    for name in sorted(names):
+        assert hasattr(L, name), f"{name} declared in include/ but not exported"
+        assert name in cc.SIGNATURES, f"{name} not bound by the Python mirror"
+    for var in ("xerbla", "errorHandler"):
+        assert ctypes.c_void_p.in_dll(L, var).value is not None
+
+
+def test_headers_compile_as_c_and_cpp(tmp_path):
+    src = tmp_path / "use.c"
+    src.write_text(
+        '#include "lapack.h"\n#include "error.h"\n'
+        "_Static_assert(CBlasLower == 'L' && CBlasUpper == 'U', \"uplo\");\n"
+        "_Static_assert(CBlasNoTrans == 'N' && CBlasTrans == 'T' && CBlasConjTrans == 'C', \"t\");\n"
+        "_Static_assert(CBlasLeft == 'L' && CBlasRight == 'R', \"side\");\n"
+        "_Static_assert(CBlasNonUnit == 'N' && CBlasUnit == 'U', \"diag\");\n"
+        "_Static_assert(CUDA_SUCCESS == 0 && CUDA_ERROR_INVALID_VALUE == 1 && "
+        "CUDA_ERROR_OUT_OF_MEMORY == 2, \"codes\");\n"
+        "_Static_assert(sizeof(CUdeviceptr) == 8, \"ptr\");\n"
+        "int f(CULAPACKhandle h, CUdeviceptr A) { long info;\n"
+        "  CUresult r = cuDpotrf(h, CBlasLower, 4, A, 4, &info);\n"
+        "  double complex one = 1.0;\n"
+        "  return (int)r + (int)cuZgemm(0, CBlasNoTrans, CBlasConjTrans, 1, 1, 1, one, A, 1, A, 1,"
+        " one, A, 1, (CUstream)0); }\n")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{INCLUDE}", str(src)],
+                   check=True)
+    cpp = tmp_path / "use.cpp"
+    cpp.write_text('#include "lapack.h"\n#include "error.h"\n'
+                   "int g(CULAPACKhandle h) { long i; return cuDpotri(h, CBlasUpper, 0, 0, 1, &i); }\n")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-fsyntax-only", f"-I{INCLUDE}", str(cpp)],
+                   check=True)
+
+
+FAKE = ctypes.c_void_p(0x1000)  # never dereferenced on the argument-error paths
+
+
+@pytest.mark.parametrize("fn,expect_info", [("cuDpotrf", -4), ("cuZpotrf", -4), ("cuDlauum", -4),
+                                            ("cuZlauum", -4), ("cuDpotri", -4), ("cuZpotri", -4)])
+def test_lapack_lda_error(cc, fn, expect_info):
+    """lda < n: info = -4, xerbla(4), CUDA_ERROR_INVALID_VALUE
+    (reference lapack/dpotrf.c:173-179)."""
+    with cc.Xerbla() as x:
+        st, info = getattr(cc, fn)(FAKE, cc.Lower, 10, 0, 9)
+    assert st == cc.CUDA_ERROR_INVALID_VALUE
+    assert info == expect_info
+    assert x.calls == [(fn, 4)]
+
+
+@pytest.mark.parametrize("fn", ["cuDtrtri", "cuZtrtri"])
+def test_trtri_lda_error(cc, fn):
+    """TRTRI reports lda as parameter 5 (reference lapack/dtrtri.c:331-337)."""
+    with cc.Xerbla() as x:
+        st, info = getattr(cc, fn)(FAKE, cc.Upper, cc.NonUnit, 10, 0, 3)
+    assert st == cc.CUDA_ERROR_INVALID_VALUE and info == -5 and x.calls == [(fn, 5)]
+
+
+@pytest.mark.parametrize("fn", ["cuDpotrf", "cuDpotri", "cuDlauum", "cuZpotrf"])
+def test_lapack_n_zero_quick_return(cc, fn):
+    st, info = getattr(cc, fn)(FAKE, cc.Lower, 0, 0, 1)
+    assert st == cc.CUDA_SUCCESS and info == 0
+
+
+def test_logdet_n_zero(cc):
+    st, r = cc.cuDlogdet(FAKE, 0, 1, 0)
+    assert st == cc.CUDA_SUCCESS and r == 0.0
+    st, r = cc.cuZlogdet(FAKE, 0, 1, 0)
+    assert st == cc.CUDA_SUCCESS and r == 0.0
+
+
+def test_gemm_argument_indices(cc):
+    """reference blas/dgemm.c:153-167: lda -> 8, ldb -> 10, ldc -> 13, ldd -> 15."""
+    cases = [((5, 4, 3, 4, 3, 5, 5), 8),     # m, n, k, lda, ldb, ldc, ldd ; NN lda < m
+             ((5, 4, 3, 5, 2, 5, 5), 10),
+             ((5, 4, 3, 5, 3, 4, 5), 13),
+             ((5, 4, 3, 5, 3, 5, 4), 15)]
+    for (m, n, k, lda, ldb, ldc, ldd), idx in cases:
+        with cc.Xerbla() as x:
+            st = cc.cuDgemm2(FAKE, cc.NoTrans, cc.NoTrans, m, n, k, 1.0, 0, lda, 0, ldb, 1.0, 0, ldc,
+                             0, ldd)
+        assert st == cc.CUDA_ERROR_INVALID_VALUE and x.calls == [("cuDgemm2", idx)]
+        with cc.Xerbla() as x:
+            st = cc.cuZgemm2(FAKE, cc.NoTrans, cc.NoTrans, m, n, k, 1.0, 0, lda, 0, ldb, 1.0, 0, ldc,
+                             0, ldd)
+        assert st == cc.CUDA_ERROR_INVALID_VALUE and x.calls == [("cuZgemm2", idx)]
+
+
+def test_gemm_quick_returns(cc):
+    # m == 0, n == 0, and (C == D with alpha == 0 and beta == 1) need no device
+    assert cc.cuDgemm2(FAKE, cc.NoTrans, cc.NoTrans, 0, 4, 3, 1.0, 0, 1, 0, 3, 1.0, 0, 1, 0, 1) == 0
+    assert cc.cuDgemm2(FAKE, cc.NoTrans, cc.NoTrans, 4, 0, 3, 1.0, 0, 4, 0, 3, 1.0, 0, 4, 0, 4) == 0
+    assert cc.cuDgemm2(FAKE, cc.NoTrans, cc.NoTrans, 4, 4, 3, 0.0, 0, 4, 0, 3, 1.0, 64, 4, 64, 4) == 0
+
+
+def test_syrk_herk_argument_indices(cc):
+    with cc.Xerbla() as x:
+        assert cc.cuDsyrk(FAKE, cc.Lower, cc.NoTrans, 5, 3, 1.0, 0, 4, 1.0, 0, 5) == 1
+        assert cc.cuDsyrk(FAKE, cc.Lower, cc.Trans, 5, 3, 1.0, 0, 3, 1.0, 0, 4) == 1
+        # ZHERK rejects trans = T (reference blas/zherk.c:197-198)
+        assert cc.cuZherk(FAKE, cc.Lower, cc.Trans, 5, 3, 1.0, 0, 5, 1.0, 0, 5) == 1
+    assert x.calls == [("cuDsyrk", 7), ("cuDsyrk", 10), ("cuZherk", 3)]
+
+
+def test_trsm_trmm_argument_indices(cc):
+    with cc.Xerbla() as x:
+        assert cc.cuDtrsm(FAKE, cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, 6, 4, 1.0, 0, 5, 0, 6) == 1
+        assert cc.cuDtrsm(FAKE, cc.Right, cc.Lower, cc.NoTrans, cc.NonUnit, 6, 4, 1.0, 0, 4, 0, 5) == 1
+        assert cc.cuDtrmm2(FAKE, cc.Left, cc.Upper, cc.Trans, cc.Unit, 6, 4, 1.0, 0, 6, 0, 6, 0, 5) == 1
+        assert cc.cuZtrsm(FAKE, cc.Right, cc.Upper, cc.ConjTrans, cc.Unit, 6, 4, 1.0, 0, 3, 0, 6) == 1
+    assert x.calls == [("cuDtrsm", 9), ("cuDtrsm", 11), ("cuDtrmm2", 13), ("cuZtrsm", 9)]
+
+
+def test_invalid_enum_rejected(cc):
+    with cc.Xerbla() as x:
+        st, info = cc.cuDpotrf(FAKE, ord("X"), 4, 0, 4)
+    assert st == 1 and info == -2 and x.calls == [("cuDpotrf", 2)]
+
+
+def test_error_strings(cc):
+    assert cc.error_string(0) == "No errors"
+    assert cc.error_string(1) == "Invalid value"
+    assert cc.error_string(2) == "Out of memory"
