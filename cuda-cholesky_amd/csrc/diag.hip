@@ -154,6 +154,9 @@ __device__ void lds_lauum(T *S, int n) {
     if (r < n)
       for (int k = r + part; k < n; k += TPR) sum = sum + conjx(S[r * LS + k]) * S[c * LS + k];
     sum = group_sum<TPR>(sum);
+    // the diagonal of L^H L is real; fused rounding of conj(a) a may leave an
+    // imaginary part of order eps |a|^2, which a Hermitian result must not carry
+    if (r == c) sum = from_real<T>(realx(sum));
     __syncthreads();
     if (r < n && part == 0) S[c * LS + r] = sum;
     __syncthreads();

@@ -16,7 +16,8 @@
 // (MT/2) x (NT/2) sub-tile with v_mfma_f64_16x16x4_f64; the op(B) fragment is
 // fed as the instruction's first operand and the op(A) fragment as its second,
 // so the 16 x 16 result block comes out with the row index of D across lanes
-// 0..15 and the epilogue writes 128-byte column segments (coalesced).
+// 0..15 (lane group g = lane / 16 and register r hold column g + 4 r) and the
+// epilogue writes 128-byte column segments (coalesced).
 // Complex products use four real MFMAs (re*re - im*im, re*im + im*re) on
 // separate real / imaginary LDS planes.
 #include "internal.h"
@@ -223,7 +224,9 @@ __global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
     }
   }
 
-  // ---- epilogue: lane lc holds row m, register r holds column 4*lg + r ----
+  // ---- epilogue: lane lc holds row m; register r of lane group lg holds
+  // column lg + 4 r (the 16x16x4 f64 result layout: block row lane/16 + 4 r,
+  // block column lane % 16, with rows of the block indexing N here) ----
   const bool use_beta = !is_zero(p.beta);
 #pragma unroll
   for (int i = 0; i < BI; i++) {
@@ -233,7 +236,7 @@ __global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
     for (int j = 0; j < BJ; j++) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gn = n0 + wn0 + 16 * j + 4 * lg + r;
+        const int gn = n0 + wn0 + 16 * j + lg + 4 * r;
         if (gn >= p.n) continue;
         if (p.tri_out == TRI_LOWER && gm < gn) continue;
         if (p.tri_out == TRI_UPPER && gm > gn) continue;

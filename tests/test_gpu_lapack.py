@@ -30,7 +30,8 @@ def tri(M, uplo):
 
 
 def rel(a, b):
-    return np.linalg.norm(a - b) / np.linalg.norm(b)
+    d = np.linalg.norm(a - b)
+    return 0.0 if d == 0.0 else d / np.linalg.norm(b)
 
 
 def gpu_call(gpu, fn, lapack, *args):
