@@ -1,7 +1,8 @@
 // Diagonal-block kernels and reductions, double and double complex.
 //
 // A diagonal block of order n <= 128 (double) or n <= 64 (complex) is held in
-// LDS by one workgroup of 512 threads and processed in "lower form": for an
+// LDS (and, for the factor and inverse, in registers) by one workgroup and
+// processed in "lower form": for an
 // upper-triangle caller the block is loaded conjugate-transposed, so
 // U^H U = L L^H, inv(U) = inv(L)^H and U U^H = L^H L reduce every case to a
 // lower-triangular kernel, and stored back conjugate-transposed.
@@ -19,7 +20,11 @@
 namespace hc {
 namespace {
 
-constexpr int DIAG_THREADS = 512;
+// Factor / inverse kernels: ten waves, one per 32 x 32 (double) or 16 x 16
+// (complex) region of the lower triangle of a 4 x 4 grid of regions.
+constexpr int DIAG_THREADS = 640;
+// L^H L kernel: 512 threads in groups of TPR per row.
+constexpr int LAUUM_THREADS = 512;
 
 template <typename T> __device__ __forceinline__ T shfl_xor_t(T v, int m);
 template <> __device__ __forceinline__ double shfl_xor_t<double>(double v, int m) {
@@ -41,15 +46,15 @@ __device__ __forceinline__ T group_sum(T v) {
 template <typename T, int NMAX>
 struct Lds {
   static constexpr int LS = NMAX + 1;
-  static constexpr int TPR = DIAG_THREADS / NMAX;  // threads per row in sweeps
+  static constexpr int TPR = LAUUM_THREADS / NMAX;  // threads per row in sweeps
 };
 
 // Load the lower form of the n x n block at A: S(r, c) = A(r, c) (lower) or
 // conj(A(c, r)) (upper), r >= c.  Memory is read along its contiguous index.
 template <typename T, int NMAX>
-__device__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
+__device__ __forceinline__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
   constexpr int LS = Lds<T, NMAX>::LS;
-  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
     const int a = idx % n, b = idx / n;  // a: contiguous index in memory
     if (lower) {
       if (a >= b) S[b * LS + a] = A[(long)b * lda + a];
@@ -63,9 +68,9 @@ __device__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
 // Store the lower form back into the caller's triangle (diagonal included
 // unless skip_diag).
 template <typename T, int NMAX>
-__device__ void store_lower_form(const T *S, bool lower, int n, T *A, long lda, bool skip_diag) {
+__device__ __forceinline__ void store_lower_form(const T *S, bool lower, int n, T *A, long lda, bool skip_diag) {
   constexpr int LS = Lds<T, NMAX>::LS;
-  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
     const int a = idx % n, b = idx / n;
     if (skip_diag && a == b) continue;
     if (lower) {
@@ -79,9 +84,9 @@ __device__ void store_lower_form(const T *S, bool lower, int n, T *A, long lda, 
 // W(r, c) = S(r, c) for r >= c, zero above; unit -> ones on the diagonal.
 // With upper, W is the conjugate transpose instead (upper triangular).
 template <typename T, int NMAX>
-__device__ void store_dense(const T *S, bool upper, int n, bool unit, T *W, long ldw) {
+__device__ __forceinline__ void store_dense(const T *S, bool upper, int n, bool unit, T *W, long ldw) {
   constexpr int LS = Lds<T, NMAX>::LS;
-  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
     const int r = idx % n, c = idx / n;
     T v = from_real<T>(0.0);
     if (r == c) v = unit ? from_real<T>(1.0) : (upper ? conjx(S[c * LS + r]) : S[c * LS + r]);
@@ -91,61 +96,10 @@ __device__ void store_dense(const T *S, bool upper, int n, bool unit, T *W, long
   }
 }
 
-// In-place Cholesky of the lower form: right-looking, one column per step.
-// Returns the 0-based index of the first non-positive (or NaN) pivot, or -1.
-template <typename T, int NMAX>
-__device__ int lds_potrf(T *S, int n) {
-  constexpr int LS = Lds<T, NMAX>::LS;
-  const int tid = threadIdx.x;
-  for (int k = 0; k < n; k++) {
-    const double d = realx(S[k * LS + k]);
-    if (!(d > 0.0)) return k;  // every thread read the same value: uniform
-    const double sd = sqrt(d);
-    for (int i = k + 1 + tid; i < n; i += DIAG_THREADS) S[k * LS + i] = S[k * LS + i] / sd;
-    __syncthreads();
-    if (tid == 0) S[k * LS + k] = from_real<T>(sd);
-    const int len = n - k - 1;
-    for (int idx = tid; idx < len * len; idx += DIAG_THREADS) {
-      const int rr = idx % len, cc = idx / len;
-      if (rr >= cc) {
-        const int r = k + 1 + rr, c = k + 1 + cc;
-        S[c * LS + r] = S[c * LS + r] - S[k * LS + r] * conjx(S[k * LS + c]);
-      }
-    }
-    __syncthreads();
-  }
-  return -1;
-}
-
-// In-place inverse of the lower form, columns from last to first:
-// column j <- -inv(L(j,j)) * X(j+1:, j+1:) * L(j+1:, j) with X the already
-// inverted trailing block (the column sweep of LAPACK xTRTI2).
-template <typename T, int NMAX>
-__device__ void lds_trtri(T *S, int n, bool unit) {
-  constexpr int LS = Lds<T, NMAX>::LS, TPR = Lds<T, NMAX>::TPR;
-  const int tid = threadIdx.x, row = tid / TPR, part = tid % TPR;
-  for (int j = n - 1; j >= 0; j--) {
-    const T djinv = unit ? from_real<T>(1.0) : from_real<T>(1.0) / S[j * LS + j];
-    const int i = j + 1 + row;
-    T sum = from_real<T>(0.0);
-    if (i < n) {
-      for (int k = j + 1 + part; k <= i; k += TPR) {
-        const T x = (k == i) ? (unit ? from_real<T>(1.0) : S[i * LS + i]) : S[k * LS + i];
-        sum = sum + x * S[j * LS + k];
-      }
-    }
-    sum = group_sum<TPR>(sum);
-    __syncthreads();
-    if (i < n && part == 0) S[j * LS + i] = -(djinv * sum);
-    if (tid == 0 && !unit) S[j * LS + j] = djinv;
-    __syncthreads();
-  }
-}
-
 // In-place R = L^H L of the lower form (lower part), columns left to right:
 // R(r, c) = sum_{k >= r} conj(L(k, r)) L(k, c).
 template <typename T, int NMAX>
-__device__ void lds_lauum(T *S, int n) {
+__device__ __forceinline__ void lds_lauum(T *S, int n) {
   constexpr int LS = Lds<T, NMAX>::LS, TPR = Lds<T, NMAX>::TPR;
   const int tid = threadIdx.x, row = tid / TPR, part = tid % TPR;
   for (int c = 0; c < n; c++) {
@@ -163,21 +117,215 @@ __device__ void lds_lauum(T *S, int n) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident factor and inverse.  Each of ten waves owns one region
+// of the lower triangle (a 4 x 4 grid of NMAX/4-square regions); each lane
+// owns an RB x CB block of its wave's region in registers.  A column step
+// needs one broadcast vector, published through a double-buffered LDS array,
+// and one barrier; a wave whose region the step cannot change skips it, so
+// the work per step follows the shrinking trailing triangle.
+// ---------------------------------------------------------------------------
+template <typename T, int NMAX>
+struct Reg {
+  static constexpr int RS = NMAX / 4;          // region side (one wave)
+  static constexpr int RB = RS / 8, CB = RS / 8;  // lane block: 8 x 8 lanes per region
+  // region (I, J), I >= J, of wave w in the order (0,0) (1,0) (1,1) (2,0) ...
+  __device__ static int reg_i() {
+    const int w = threadIdx.x >> 6;
+    return w >= 6 ? 3 : (w >= 3 ? 2 : (w >= 1 ? 1 : 0));
+  }
+  __device__ static int reg_j() {
+    const int i = reg_i();
+    return (threadIdx.x >> 6) - i * (i + 1) / 2;
+  }
+  __device__ static int r0() { return reg_i() * RS + ((threadIdx.x & 63) >> 3) * RB; }
+  __device__ static int c0() { return reg_j() * RS + (threadIdx.x & 7) * CB; }
+};
+template <typename T, int NMAX>
+using RegTile = T[NMAX / 32][NMAX / 32];
+
+// Lane-wise select written per component, so a complex value never becomes
+// an aggregate select (which the compiler lowers through private memory).
+__device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+__device__ __forceinline__ zd sel(bool c, zd a, zd b) { return make_zd(c ? a.x : b.x, c ? a.y : b.y); }
+
+// Column k of the register tile (rows R0 .. R0 + RB) into the broadcast
+// vector, by the threads whose columns include k.
+template <typename T, int NMAX>
+__device__ __forceinline__ void publish_column(const RegTile<T, NMAX> &a, T *colbuf, int k, int n) {
+  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
+  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
+  if (k < n && k >= C0 && k < C0 + CB) {
+    const int kc = k - C0;
+#pragma unroll
+    for (int ri = 0; ri < RB; ri++) {
+      T v = a[ri][0];
+#pragma unroll
+      for (int ci = 1; ci < CB; ci++) v = sel(ci == kc, a[ri][ci], v);
+      colbuf[(k & 1) * NMAX + R0 + ri] = v;
+    }
+  }
+}
+
+// Right-looking Cholesky of the lower form held in registers (a) with the
+// pivot column broadcast through colbuf[2][NMAX].  Returns the 0-based index
+// of the first non-positive or NaN pivot, or -1; on success the lower part of
+// `a` holds L.  Registers of the strict upper part are scratch: they take the
+// same update as the rest of their column and are never read into the lower
+// part (a lower element (r, c) only combines column-k values of rows r and
+// c, both > k) nor stored.  Conditions therefore depend on the row alone or
+// the column alone, and the step loop has no lane-divergent branch.
+template <typename T, int NMAX>
+__device__ __forceinline__ int reg_potrf(RegTile<T, NMAX> &a, T *colbuf, int n) {
+  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
+  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
+  publish_column<T, NMAX>(a, colbuf, 0, n);
+  for (int k = 0; k < n; k++) {
+    __syncthreads();
+    const T *cb = colbuf + (k & 1) * NMAX;
+    const double d = realx(cb[k]);
+    if (!(d > 0.0)) return k;  // every thread read the same value: uniform
+    const double sd = sqrt(d), rs = 1.0 / sd;
+    if (Reg<T, NMAX>::reg_j() * Reg<T, NMAX>::RS + Reg<T, NMAX>::RS > k) {  // regions left of column k are final
+      T lr[RB], lv[CB], fin[RB];
+#pragma unroll
+      for (int ri = 0; ri < RB; ri++) {
+        lr[ri] = rs * cb[R0 + ri];
+        fin[ri] = sel(R0 + ri == k, from_real<T>(sd), lr[ri]);  // final column-k entry
+      }
+#pragma unroll
+      for (int ci = 0; ci < CB; ci++) lv[ci] = conjx(rs * cb[C0 + ci]);
+#pragma unroll
+      for (int ci = 0; ci < CB; ci++) {
+        const bool trailing = C0 + ci > k, pivot = C0 + ci == k;
+#pragma unroll
+        for (int ri = 0; ri < RB; ri++) {
+          const T upd = a[ri][ci] - lr[ri] * lv[ci];
+          a[ri][ci] = sel(trailing, upd, sel(pivot, fin[ri], a[ri][ci]));
+        }
+      }
+    }
+    publish_column<T, NMAX>(a, colbuf, k + 1, n);
+  }
+  return -1;
+}
+
+// X = inv(L) of the lower form L held in LDS (S, leading dimension NMAX + 1),
+// forward substitution by rows: row k of X is final once the updates of rows
+// 0..k-1 have been applied; it is published through xrow[2][NMAX] and the
+// rows below take its contribution.  dinv[k] = 1 / L(k, k) (unused for a unit
+// diagonal).  The strict upper part of X is exactly zero throughout (it starts
+// at zero and only ever gains products with published zeros), so neither the
+// finalisation nor the update needs a column condition.  On return x holds
+// the thread's block of X.
+template <typename T, int NMAX>
+__device__ __forceinline__ void reg_trtri(const T *S, RegTile<T, NMAX> &x, T *xrow, const T *dinv, int n, bool unit) {
+  constexpr int LS = NMAX + 1, RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
+  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
+#pragma unroll
+  for (int ri = 0; ri < RB; ri++)
+#pragma unroll
+    for (int ci = 0; ci < CB; ci++) x[ri][ci] = from_real<T>(0.0);
+  const T one = from_real<T>(1.0), zero = from_real<T>(0.0);
+  for (int k = 0; k < n; k++) {
+    if (k >= R0 && k < R0 + RB) {  // this thread holds part of row k: finalise it
+      const T dk = unit ? one : dinv[k];
+      const int kr = k - R0;
+      T v[CB];
+#pragma unroll
+      for (int ci = 0; ci < CB; ci++) {
+        const T delta = sel(C0 + ci == k, one, zero);
+        v[ci] = zero;
+#pragma unroll
+        for (int ri = 0; ri < RB; ri++) {
+          const T fin = (delta - x[ri][ci]) * dk;
+          x[ri][ci] = sel(ri == kr, fin, x[ri][ci]);
+          v[ci] = sel(ri == kr, fin, v[ci]);
+        }
+        xrow[(k & 1) * NMAX + C0 + ci] = v[ci];
+      }
+    }
+    __syncthreads();
+    if ((Reg<T, NMAX>::reg_i() + 1) * Reg<T, NMAX>::RS > k + 1 &&
+        Reg<T, NMAX>::reg_j() * Reg<T, NMAX>::RS <= k) {  // regions with rows below k, columns <= k
+      const T *xr = xrow + (k & 1) * NMAX;
+      T lk[RB], xk[CB];
+#pragma unroll
+      for (int ri = 0; ri < RB; ri++) lk[ri] = S[k * LS + R0 + ri];
+#pragma unroll
+      for (int ci = 0; ci < CB; ci++) xk[ci] = xr[C0 + ci];
+#pragma unroll
+      for (int ri = 0; ri < RB; ri++) {
+        const bool below = R0 + ri > k;
+#pragma unroll
+        for (int ci = 0; ci < CB; ci++) {
+          const T upd = x[ri][ci] + lk[ri] * xk[ci];
+          x[ri][ci] = sel(below, upd, x[ri][ci]);
+        }
+      }
+    }
+  }
+}
+
+// Registers <-> LDS copies of the thread's block (lower part, inside n x n).
+template <typename T, int NMAX>
+__device__ __forceinline__ void reg_to_lds(const RegTile<T, NMAX> &v, T *S, int n) {
+  constexpr int LS = NMAX + 1, RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
+  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
+#pragma unroll
+  for (int ri = 0; ri < RB; ri++)
+#pragma unroll
+    for (int ci = 0; ci < CB; ci++) {
+      const int r = R0 + ri, c = C0 + ci;
+      if (r < n && c < n && r >= c) S[c * LS + r] = v[ri][ci];
+    }
+}
+template <typename T, int NMAX>
+__device__ __forceinline__ void lds_to_reg(const T *S, RegTile<T, NMAX> &v, int n) {
+  constexpr int LS = NMAX + 1, RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
+  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
+#pragma unroll
+  for (int ri = 0; ri < RB; ri++)
+#pragma unroll
+    for (int ci = 0; ci < CB; ci++) {
+      const int r = R0 + ri, c = C0 + ci;
+      v[ri][ci] = (r < n && c < n && r >= c) ? S[c * LS + r] : from_real<T>(0.0);
+    }
+}
+
+// dinv[k] = 1 / S(k, k) for k < n.
+template <typename T, int NMAX>
+__device__ __forceinline__ void diag_reciprocals(const T *S, T *dinv, int n) {
+  constexpr int LS = NMAX + 1;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) dinv[k] = from_real<T>(1.0) / S[k * LS + k];
+}
+
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(DIAG_THREADS)
 diag_potrf_kernel(int lower, int n, T *A, long lda, T *W, long ldw, int *info, long info_off) {
   if (*info != 0) return;
+  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
   __shared__ T S[NMAX * (NMAX + 1)];
+  __shared__ T vec[2 * NMAX];
+  __shared__ T dinv[NMAX];
   load_lower_form<T, NMAX>(S, lower, n, A, lda);
   __syncthreads();
-  const int bad = lds_potrf<T, NMAX>(S, n);
+  T a[RB][CB];
+  lds_to_reg<T, NMAX>(S, a, n);
+  const int bad = reg_potrf<T, NMAX>(a, vec, n);
   if (bad >= 0) {
     if (threadIdx.x == 0) *info = (int)(info_off + bad + 1);
     return;
   }
+  reg_to_lds<T, NMAX>(a, S, n);
   __syncthreads();
   store_lower_form<T, NMAX>(S, lower, n, A, lda, false);
-  lds_trtri<T, NMAX>(S, n, false);
+  diag_reciprocals<T, NMAX>(S, dinv, n);
+  __syncthreads();
+  reg_trtri<T, NMAX>(S, a, vec, dinv, n, false);
+  __syncthreads();
+  reg_to_lds<T, NMAX>(a, S, n);
+  __syncthreads();
   store_dense<T, NMAX>(S, false, n, false, W, ldw);
 }
 
@@ -186,16 +334,25 @@ __global__ void __launch_bounds__(DIAG_THREADS)
 diag_trtri_kernel(int lower, int unit, int n, const T *A, long lda, T *B, long ldb, T *W, long ldw,
                   const int *abort_flag) {
   if (abort_flag != nullptr && *abort_flag != 0) return;
+  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
   __shared__ T S[NMAX * (NMAX + 1)];
+  __shared__ T vec[2 * NMAX];
+  __shared__ T dinv[NMAX];
   load_lower_form<T, NMAX>(S, lower, n, A, lda);
   __syncthreads();
-  lds_trtri<T, NMAX>(S, n, unit);
+  if (!unit) diag_reciprocals<T, NMAX>(S, dinv, n);
+  __syncthreads();
+  T x[RB][CB];
+  reg_trtri<T, NMAX>(S, x, vec, dinv, n, unit != 0);
+  __syncthreads();
+  reg_to_lds<T, NMAX>(x, S, n);
+  __syncthreads();
   store_lower_form<T, NMAX>(S, lower, n, B, ldb, unit != 0);
   if (W != nullptr) store_dense<T, NMAX>(S, false, n, unit != 0, W, ldw);
 }
 
 template <typename T, int NMAX>
-__global__ void __launch_bounds__(DIAG_THREADS) diag_lauum_kernel(int lower, int n, T *A, long lda) {
+__global__ void __launch_bounds__(LAUUM_THREADS) diag_lauum_kernel(int lower, int n, T *A, long lda) {
   __shared__ T S[NMAX * (NMAX + 1)];
   load_lower_form<T, NMAX>(S, lower, n, A, lda);
   __syncthreads();
@@ -209,11 +366,13 @@ __global__ void __launch_bounds__(DIAG_THREADS)
 trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const T *A, long lda, T *W) {
   constexpr int LS = Lds<T, NMAX>::LS;
   __shared__ T S[NMAX * (NMAX + 1)];
+  __shared__ T vec[2 * NMAX];
+  __shared__ T dinv[NMAX];
   const long off = (long)blockIdx.x * nb;
   const int n = (int)((m - off) < nb ? (m - off) : nb);
   // lower form: S(r, c) = op(A)(r, c) when op(A) is lower-like, else
   // conj(op(A)(c, r)); op(A)(i, j) = A(i, j), A(j, i) or conj(A(j, i)).
-  for (int idx = threadIdx.x; idx < n * n; idx += DIAG_THREADS) {
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
     const int a = idx % n, b = idx / n;  // memory element A(off + a, off + b)
     const T v = A[(long)(off + b) * lda + off + a];
     // as an element of op(A): (a, b) for N, (b, a) for T/C
@@ -223,7 +382,13 @@ trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const T *A, lon
     if (r >= c) S[c * LS + r] = e;
   }
   __syncthreads();
-  lds_trtri<T, NMAX>(S, n, unit);
+  if (!unit) diag_reciprocals<T, NMAX>(S, dinv, n);
+  __syncthreads();
+  RegTile<T, NMAX> x;
+  reg_trtri<T, NMAX>(S, x, vec, dinv, n, unit != 0);
+  __syncthreads();
+  reg_to_lds<T, NMAX>(x, S, n);
+  __syncthreads();
   store_dense<T, NMAX>(S, !lowerlike, n, unit != 0, W + (long)blockIdx.x * nb * nb, nb);
 }
 
@@ -329,7 +494,7 @@ hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda) {
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
+  hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3(1), dim3(LAUUM_THREADS), 0, s,
                      uplo == 'L' ? 1 : 0, n, A, lda);
   return hipGetLastError();
 }

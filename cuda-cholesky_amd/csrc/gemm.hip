@@ -73,20 +73,36 @@ __device__ __forceinline__ void lds_put(double *s, int plane_stride, int idx, zd
   s[plane_stride + idx] = v.y;
 }
 
+// Per-thread global-load pattern for one operand tile of R x KT elements
+// (R = MT for op(A), NT for op(B)).  With the R index contiguous in memory
+// (op N for A, op T/C for B) a thread keeps one R index and walks K; with K
+// contiguous (op T/C for A, op N for B) it keeps one K index and walks R.
+template <int R, bool R_CONTIG>
+struct LoadPattern {
+  static constexpr int E = R * KT / NTHREADS;           // elements per thread
+  static constexpr int RSTEP = R_CONTIG ? 0 : NTHREADS / KT;
+  static constexpr int KSTEP = R_CONTIG ? NTHREADS / R : 0;
+  __device__ static int r0(int tid) { return R_CONTIG ? tid % R : tid / KT; }
+  __device__ static int k0(int tid) { return R_CONTIG ? tid / R : tid % KT; }
+};
+
 template <typename T, int TA, int TB, int MT, int NT>
-__global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
+__global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
 
   constexpr bool CPLX = is_complex<T>::value;
   constexpr int P = CPLX ? 2 : 1;
   constexpr int LSA = MT + PAD, LSB = NT + PAD;
-  constexpr int PSA = KT * LSA, PSB = KT * LSB;  // plane strides
-  constexpr int EA = MT * KT / NTHREADS, EB = NT * KT / NTHREADS;
+  constexpr int PSA = KT * LSA, PSB = KT * LSB;  // plane strides (doubles)
+  constexpr int BUFA = P * PSA, BUFB = P * PSB;  // one LDS buffer
   constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
+  using PA = LoadPattern<MT, TA == OP_N>;
+  using PB = LoadPattern<NT, TB != OP_N>;
+  constexpr int EA = PA::E, EB = PB::E;
   static_assert(EA >= 1 && EB >= 1 && BI >= 1 && BJ >= 1, "tile too small");
 
-  __shared__ double sA[P * PSA];
-  __shared__ double sB[P * PSB];
+  __shared__ double sA[2 * BUFA];
+  __shared__ double sB[2 * BUFB];
 
   // ---- tile coordinates ----
   int bm, bn;
@@ -124,75 +140,76 @@ __global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
 #pragma unroll
       for (int j = 0; j < BJ; j++) acc[q][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
-  T ra[EA], rb[EB];
-
-  // element e of the per-thread load pattern -> (index in M or N, index in K)
-  auto a_coord = [&](int q, int &mi, int &ki) {
-    const int e = tid + q * NTHREADS;
-    if (TA == OP_N) { mi = e % MT; ki = e / MT; }   // M contiguous in memory
-    else            { mi = e / KT; ki = e % KT; }   // K contiguous in memory
-  };
-  auto b_coord = [&](int q, int &nj, int &ki) {
-    const int e = tid + q * NTHREADS;
-    if (TB == OP_N) { nj = e / KT; ki = e % KT; }   // K contiguous
-    else            { nj = e % NT; ki = e / NT; }   // N contiguous
-  };
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < EA; q++) {
-      int mi, ki;
-      a_coord(q, mi, ki);
-      const int gi = m0 + mi, gk = k0 + ki;
-      T v = from_real<T>(0.0);
-      if (gi < p.m && gk < kend) {
-        v = fetch<T, TA>(p.A, p.lda, gi, gk);
-        if (p.tri_a) v = tri_mask(v, p.tri_a, p.unit, gi, gk);
-      }
-      ra[q] = v;
-    }
-#pragma unroll
-    for (int q = 0; q < EB; q++) {
-      int nj, ki;
-      b_coord(q, nj, ki);
-      const int gj = n0 + nj, gk = k0 + ki;
-      T v = from_real<T>(0.0);
-      if (gj < p.n && gk < kend) {
-        v = fetch<T, TB>(p.B, p.ldb, gk, gj);
-        if (p.tri_b) v = tri_mask(v, p.tri_b, p.unit, gk, gj);
-      }
-      rb[q] = v;
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int q = 0; q < EA; q++) {
-      int mi, ki;
-      a_coord(q, mi, ki);
-      lds_put(sA, PSA, ki * LSA + mi, ra[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < EB; q++) {
-      int nj, ki;
-      b_coord(q, nj, ki);
-      lds_put(sB, PSB, ki * LSB + nj, rb[q]);
-    }
-  };
-
   if (kbeg < kend) {
-    load(kbeg);
-    store();
+    // Per-thread source pointers at k = kbeg, and their per-element /
+    // per-K-step strides; every address is formed by additions only.
+    const int ar0 = m0 + PA::r0(tid), ak0 = kbeg + PA::k0(tid);
+    const int br0 = n0 + PB::r0(tid), bk0 = kbeg + PB::k0(tid);
+    const T *pa = (TA == OP_N) ? p.A + (long)ak0 * p.lda + ar0 : p.A + (long)ar0 * p.lda + ak0;
+    const T *pb = (TB == OP_N) ? p.B + (long)br0 * p.ldb + bk0 : p.B + (long)bk0 * p.ldb + br0;
+    // element q sits PA::RSTEP rows / PA::KSTEP k further on
+    const long a_qstride = (TA == OP_N) ? (long)PA::KSTEP * p.lda : (long)PA::RSTEP * p.lda;
+    const long b_qstride = (TB == OP_N) ? (long)PB::RSTEP * p.ldb : (long)PB::KSTEP * p.ldb;
+    const long a_kstride = (TA == OP_N) ? (long)KT * p.lda : (long)KT;
+    const long b_kstride = (TB == OP_N) ? (long)KT : (long)KT * p.ldb;
+    // LDS offsets of the thread's elements (k-major, R contiguous)
+    const int sa0 = PA::k0(tid) * LSA + PA::r0(tid);
+    const int sb0 = PB::k0(tid) * LSB + PB::r0(tid);
+    constexpr int SA_Q = PA::KSTEP * LSA + PA::RSTEP;
+    constexpr int SB_Q = PB::KSTEP * LSB + PB::RSTEP;
+
+    T ra[EA], rb[EB];
+    // Out-of-range elements read a valid address (the operand's origin) and
+    // are replaced by zero, so the loads carry no branches.
+    auto load = [&](int kt_off, const T *qa, const T *qb) {
+#pragma unroll
+      for (int q = 0; q < EA; q++) {
+        const int gi = ar0 + q * PA::RSTEP, gk = ak0 + kt_off + q * PA::KSTEP;
+        const bool ok = gi < p.m && gk < kend;
+        T v = *(ok ? qa + q * a_qstride : p.A);
+        v = ok ? v : from_real<T>(0.0);
+        if (TA == OP_C) v = conjx(v);
+        if (p.tri_a) v = tri_mask(v, p.tri_a, p.unit, gi, gk);
+        ra[q] = v;
+      }
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        const int gj = br0 + q * PB::RSTEP, gk = bk0 + kt_off + q * PB::KSTEP;
+        const bool ok = gj < p.n && gk < kend;
+        T v = *(ok ? qb + q * b_qstride : p.B);
+        v = ok ? v : from_real<T>(0.0);
+        if (TB == OP_C) v = conjx(v);
+        if (p.tri_b) v = tri_mask(v, p.tri_b, p.unit, gk, gj);
+        rb[q] = v;
+      }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < EA; q++) lds_put(sA + buf * BUFA, PSA, sa0 + q * SA_Q, ra[q]);
+#pragma unroll
+      for (int q = 0; q < EB; q++) lds_put(sB + buf * BUFB, PSB, sb0 + q * SB_Q, rb[q]);
+    };
+
+    const int ntiles = (kend - kbeg + KT - 1) / KT;
+    load(0, pa, pb);
+    store(0);
     __syncthreads();
-    for (int k0 = kbeg; k0 < kend; k0 += KT) {
-      const bool more = k0 + KT < kend;
-      if (more) load(k0 + KT);
+    for (int kt = 0; kt < ntiles; kt++) {
+      const bool more = kt + 1 < ntiles;
+      if (more) {
+        pa += a_kstride;
+        pb += b_kstride;
+        load((kt + 1) * KT, pa, pb);
+      }
+      const double *cA = sA + (kt & 1) * BUFA, *cB = sB + (kt & 1) * BUFB;
 #pragma unroll
       for (int ks = 0; ks < KT; ks += 4) {
         const int kr = ks + lg;
         double ar[BI], br[BJ];
 #pragma unroll
-        for (int i = 0; i < BI; i++) ar[i] = sA[kr * LSA + wm0 + 16 * i + lc];
+        for (int i = 0; i < BI; i++) ar[i] = cA[kr * LSA + wm0 + 16 * i + lc];
 #pragma unroll
-        for (int j = 0; j < BJ; j++) br[j] = sB[kr * LSB + wn0 + 16 * j + lc];
+        for (int j = 0; j < BJ; j++) br[j] = cB[kr * LSB + wn0 + 16 * j + lc];
         if constexpr (!CPLX) {
 #pragma unroll
           for (int i = 0; i < BI; i++)
@@ -202,9 +219,9 @@ __global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
         } else {
           double ai[BI], bi[BJ];
 #pragma unroll
-          for (int i = 0; i < BI; i++) ai[i] = sA[PSA + kr * LSA + wm0 + 16 * i + lc];
+          for (int i = 0; i < BI; i++) ai[i] = cA[PSA + kr * LSA + wm0 + 16 * i + lc];
 #pragma unroll
-          for (int j = 0; j < BJ; j++) bi[j] = sB[PSB + kr * LSB + wn0 + 16 * j + lc];
+          for (int j = 0; j < BJ; j++) bi[j] = cB[PSB + kr * LSB + wn0 + 16 * j + lc];
 #pragma unroll
           for (int i = 0; i < BI; i++)
 #pragma unroll
@@ -216,11 +233,10 @@ __global__ void __launch_bounds__(NTHREADS) gemm_kernel(KParams<T> p) {
             }
         }
       }
+      // The other buffer was last read in the previous step, which every
+      // thread finished before the previous barrier.
+      if (more) store((kt + 1) & 1);
       __syncthreads();
-      if (more) {
-        store();
-        __syncthreads();
-      }
     }
   }
 

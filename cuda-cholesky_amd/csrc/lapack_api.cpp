@@ -42,63 +42,95 @@ template <typename T> constexpr int conj_op() { return hc::is_complex<T>::value 
 // it streams (double), enough to stay matrix-core bound.
 template <typename T> constexpr long potrf_nb() { return hc::is_complex<T>::value ? 128 : 256; }
 
-// Right-looking blocked Cholesky (the reference's loop is left-looking,
-// lapack/dpotrf.c:248-286; both produce the same factor).  Each outer panel
-// of width NB is factored in IB-column slices: diagonal block (LDS kernel,
-// factor + inverse), panel solve as a GEMM with the inverse, and the
-// remaining panel columns updated; then one SYRK/HERK updates the trailing
-// matrix with the whole panel.
+// Panel factorisation of columns [j, j + jb) (lower; rows [j, j + jb) for
+// upper), all rows below, in IB-column slices: diagonal block (LDS kernel,
+// factor + inverse), panel solve as a GEMM with the inverse, and the rest of
+// the panel updated with the slice.
 template <typename T>
-hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W) {
+hipError_t factor_panel(CULAPACKhandle h, hipStream_t s, bool lower, long n, T *A, long lda, T *W,
+                        long j, long jb) {
   constexpr int IB = hc::diag_max<T>();
-  constexpr long NB = potrf_nb<T>();
-  hipStream_t s = h->stream;
-  const bool lower = (uplo == 'L');
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0), zero = hc::from_real<T>(0.0);
   auto at = [&](long r, long c) { return A + c * lda + r; };
+  const char uplo = lower ? 'L' : 'U';
+  for (long jj = j; jj < j + jb; jj += IB) {
+    const int ib = (int)std::min<long>(IB, j + jb - jj);
+    HC_TRY(hc::diag_potrf<T>(s, uplo, ib, at(jj, jj), lda, W, IB, h->dinfo, jj));
+    const long r0 = jj + ib, rest = n - r0;
+    hc::GemmOpts o;
+    o.abort_flag = h->dinfo;
+    if (rest > 0) {
+      if (lower) {  // L21 <- A21 inv(L11)^H, in place: a tile spans all ib columns
+        o.min_nt = ib;
+        HC_TRY(hc::gemm<T>(s, hc::OP_N, conj_op<T>(), rest, ib, ib, one, at(r0, jj), lda, W, IB, zero,
+                           nullptr, 0, at(r0, jj), lda, o));
+      } else {      // U12 <- inv(U11^H) A12, in place: a tile spans all ib rows
+        o.min_mt = ib;
+        HC_TRY(hc::gemm<T>(s, hc::OP_N, hc::OP_N, ib, rest, ib, one, W, IB, at(jj, r0), lda, zero,
+                           nullptr, 0, at(jj, r0), lda, o));
+      }
+    }
+    const long rem = j + jb - r0;  // panel columns still to update
+    if (rem > 0) {
+      hc::GemmOpts u;
+      u.abort_flag = h->dinfo;
+      if (lower) {
+        u.tri_out = hc::TRI_LOWER;
+        HC_TRY(hc::gemm<T>(s, hc::OP_N, conj_op<T>(), rest, rem, ib, mone, at(r0, jj), lda,
+                           at(r0, jj), lda, one, at(r0, r0), lda, at(r0, r0), lda, u));
+      } else {
+        u.tri_out = hc::TRI_UPPER;
+        HC_TRY(hc::gemm<T>(s, conj_op<T>(), hc::OP_N, rem, rest, ib, mone, at(jj, r0), lda,
+                           at(jj, r0), lda, one, at(r0, r0), lda, at(r0, r0), lda, u));
+      }
+    }
+  }
+  return hipSuccess;
+}
+
+// Right-looking blocked Cholesky with look-ahead (the reference's loop is
+// left-looking, lapack/dpotrf.c:248-286; both produce the same factor).
+// With panel j factored, the main stream first applies it to the next
+// panel's columns only, signals, and then runs the large trailing SYRK/HERK
+// on everything beyond; meanwhile the look-ahead stream factors the next
+// panel.  The two streams touch disjoint column ranges, and each panel is
+// joined back into the main stream before its own trailing update.
+template <typename T>
+hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W) {
+  constexpr long NB = potrf_nb<T>();
+  hipStream_t s0 = h->stream, s1 = h->side;
+  const bool lower = (uplo == 'L');
+  const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0);
+  auto at = [&](long r, long c) { return A + c * lda + r; };
+  HC_TRY(factor_panel<T>(h, s0, lower, n, A, lda, W, 0, std::min(NB, n)));
   for (long j = 0; j < n; j += NB) {
     const long jb = std::min(NB, n - j);
-    for (long jj = j; jj < j + jb; jj += IB) {
-      const int ib = (int)std::min<long>(IB, j + jb - jj);
-      HC_TRY(hc::diag_potrf<T>(s, uplo, ib, at(jj, jj), lda, W, IB, h->dinfo, jj));
-      const long r0 = jj + ib, rest = n - r0;
-      hc::GemmOpts o;
-      o.abort_flag = h->dinfo;
-      if (rest > 0) {
-        if (lower) {  // L21 <- A21 inv(L11)^H, in place: a tile spans all ib columns
-          o.min_nt = ib;
-          HC_TRY(hc::gemm<T>(s, hc::OP_N, conj_op<T>(), rest, ib, ib, one, at(r0, jj), lda, W, IB, zero,
-                             nullptr, 0, at(r0, jj), lda, o));
-        } else {      // U12 <- inv(U11^H) A12, in place: a tile spans all ib rows
-          o.min_mt = ib;
-          HC_TRY(hc::gemm<T>(s, hc::OP_N, hc::OP_N, ib, rest, ib, one, W, IB, at(jj, r0), lda, zero,
-                             nullptr, 0, at(jj, r0), lda, o));
-        }
-      }
-      const long rem = j + jb - r0;  // panel columns still to update
-      if (rem > 0) {
-        hc::GemmOpts u;
-        u.abort_flag = h->dinfo;
-        if (lower) {
-          u.tri_out = hc::TRI_LOWER;
-          HC_TRY(hc::gemm<T>(s, hc::OP_N, conj_op<T>(), rest, rem, ib, mone, at(r0, jj), lda,
-                             at(r0, jj), lda, one, at(r0, r0), lda, at(r0, r0), lda, u));
-        } else {
-          u.tri_out = hc::TRI_UPPER;
-          HC_TRY(hc::gemm<T>(s, conj_op<T>(), hc::OP_N, rem, rest, ib, mone, at(jj, r0), lda,
-                             at(jj, r0), lda, one, at(r0, r0), lda, at(r0, r0), lda, u));
-        }
-      }
+    const long jn = j + jb;                 // next panel
+    if (jn >= n) break;
+    const long jbn = std::min(NB, n - jn);
+    hc::GemmOpts u;
+    u.abort_flag = h->dinfo;
+    if (lower) {  // next panel columns: A[jn:, jn:jn+jbn] -= P[jn:] P[jn:jn+jbn]^H
+      u.tri_out = hc::TRI_LOWER;
+      HC_TRY(hc::gemm<T>(s0, hc::OP_N, conj_op<T>(), n - jn, jbn, jb, mone, at(jn, j), lda, at(jn, j),
+                         lda, one, at(jn, jn), lda, at(jn, jn), lda, u));
+    } else {
+      u.tri_out = hc::TRI_UPPER;
+      HC_TRY(hc::gemm<T>(s0, conj_op<T>(), hc::OP_N, jbn, n - jn, jb, mone, at(j, jn), lda, at(j, jn),
+                         lda, one, at(jn, jn), lda, at(jn, jn), lda, u));
     }
-    const long t = n - j - jb;
+    HC_TRY(hipEventRecord(h->ev_update, s0));
+    HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
+    HC_TRY(factor_panel<T>(h, s1, lower, n, A, lda, W, jn, jbn));
+    HC_TRY(hipEventRecord(h->ev_panel, s1));
+    const long t0 = jn + jbn, t = n - t0;   // trailing matrix beyond the next panel
     if (t > 0) {
       if (lower)
-        HC_TRY(hc::herk<T>(s, 'L', 'N', t, jb, -1.0, at(j + jb, j), lda, 1.0, at(j + jb, j + jb), lda,
-                           h->dinfo));
+        HC_TRY(hc::herk<T>(s0, 'L', 'N', t, jb, -1.0, at(t0, j), lda, 1.0, at(t0, t0), lda, h->dinfo));
       else
-        HC_TRY(hc::herk<T>(s, 'U', 'C', t, jb, -1.0, at(j, j + jb), lda, 1.0, at(j + jb, j + jb), lda,
-                           h->dinfo));
+        HC_TRY(hc::herk<T>(s0, 'U', 'C', t, jb, -1.0, at(j, t0), lda, 1.0, at(t0, t0), lda, h->dinfo));
     }
+    HC_TRY(hipStreamWaitEvent(s0, h->ev_panel, 0));
   }
   return hipSuccess;
 }
