@@ -149,67 +149,6 @@ using RegTile = T[NMAX / 32][NMAX / 32];
 __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
 __device__ __forceinline__ zd sel(bool c, zd a, zd b) { return make_zd(c ? a.x : b.x, c ? a.y : b.y); }
 
-// Column k of the register tile (rows R0 .. R0 + RB) into the broadcast
-// vector, by the threads whose columns include k.
-template <typename T, int NMAX>
-__device__ __forceinline__ void publish_column(const RegTile<T, NMAX> &a, T *colbuf, int k, int n) {
-  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
-  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
-  if (k < n && k >= C0 && k < C0 + CB) {
-    const int kc = k - C0;
-#pragma unroll
-    for (int ri = 0; ri < RB; ri++) {
-      T v = a[ri][0];
-#pragma unroll
-      for (int ci = 1; ci < CB; ci++) v = sel(ci == kc, a[ri][ci], v);
-      colbuf[(k & 1) * NMAX + R0 + ri] = v;
-    }
-  }
-}
-
-// Right-looking Cholesky of the lower form held in registers (a) with the
-// pivot column broadcast through colbuf[2][NMAX].  Returns the 0-based index
-// of the first non-positive or NaN pivot, or -1; on success the lower part of
-// `a` holds L.  Registers of the strict upper part are scratch: they take the
-// same update as the rest of their column and are never read into the lower
-// part (a lower element (r, c) only combines column-k values of rows r and
-// c, both > k) nor stored.  Conditions therefore depend on the row alone or
-// the column alone, and the step loop has no lane-divergent branch.
-template <typename T, int NMAX>
-__device__ __forceinline__ int reg_potrf(RegTile<T, NMAX> &a, T *colbuf, int n) {
-  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
-  const int R0 = Reg<T, NMAX>::r0(), C0 = Reg<T, NMAX>::c0();
-  publish_column<T, NMAX>(a, colbuf, 0, n);
-  for (int k = 0; k < n; k++) {
-    __syncthreads();
-    const T *cb = colbuf + (k & 1) * NMAX;
-    const double d = realx(cb[k]);
-    if (!(d > 0.0)) return k;  // every thread read the same value: uniform
-    const double sd = sqrt(d), rs = 1.0 / sd;
-    if (Reg<T, NMAX>::reg_j() * Reg<T, NMAX>::RS + Reg<T, NMAX>::RS > k) {  // regions left of column k are final
-      T lr[RB], lv[CB], fin[RB];
-#pragma unroll
-      for (int ri = 0; ri < RB; ri++) {
-        lr[ri] = rs * cb[R0 + ri];
-        fin[ri] = sel(R0 + ri == k, from_real<T>(sd), lr[ri]);  // final column-k entry
-      }
-#pragma unroll
-      for (int ci = 0; ci < CB; ci++) lv[ci] = conjx(rs * cb[C0 + ci]);
-#pragma unroll
-      for (int ci = 0; ci < CB; ci++) {
-        const bool trailing = C0 + ci > k, pivot = C0 + ci == k;
-#pragma unroll
-        for (int ri = 0; ri < RB; ri++) {
-          const T upd = a[ri][ci] - lr[ri] * lv[ci];
-          a[ri][ci] = sel(trailing, upd, sel(pivot, fin[ri], a[ri][ci]));
-        }
-      }
-    }
-    publish_column<T, NMAX>(a, colbuf, k + 1, n);
-  }
-  return -1;
-}
-
 // X = inv(L) of the lower form L held in LDS (S, leading dimension NMAX + 1),
 // forward substitution by rows: row k of X is final once the updates of rows
 // 0..k-1 have been applied; it is published through xrow[2][NMAX] and the
@@ -300,32 +239,238 @@ __device__ __forceinline__ void diag_reciprocals(const T *S, T *dinv, int n) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) dinv[k] = from_real<T>(1.0) / S[k * LS + k];
 }
 
+// ---------------------------------------------------------------------------
+// Blocked factor + inverse of one diagonal block (diag_potrf).
+//
+// The block is split into 16-column leaves.  Leaf p is factored and inverted
+// by one wave entirely in registers (lane r + 16 q holds row r, columns
+// q + 4 j) with cross-lane shuffles, so its 16 sequential pivots need no
+// workgroup barrier.  Everything else is 16 x 16 x 16 tile products on the
+// matrix cores spread over the eight waves: the panel below the leaf
+// (L_Ip = A_Ip inv(L_pp)^H), the trailing update (A_IJ -= L_Ip L_Jp^H), and
+// afterwards the in-place blocked inverse, column blocks from last to first
+// (X_Ip = -(sum_J X_IJ L_Jp) inv(L_pp), as in LAPACK xTRTRI).  Three
+// barriers per leaf for the factor and two for the inverse.
+// ---------------------------------------------------------------------------
+constexpr int LEAF = 16;
+constexpr int BLK_THREADS = 512;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// A 16 x 16 tile accumulator in the 16x16x4 f64 result layout: lane l,
+// register r holds element (l / 16 + 4 r, l % 16).
+template <typename T> struct TileAcc;
+template <> struct TileAcc<double> { v4d re; };
+template <> struct TileAcc<zd> { v4d re, im; };
+
+__device__ __forceinline__ void acc_zero(TileAcc<double> &c) { c.re = (v4d){0.0, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ void acc_zero(TileAcc<zd> &c) {
+  c.re = (v4d){0.0, 0.0, 0.0, 0.0};
+  c.im = c.re;
+}
+
+// Tile at C (column-major, leading dimension ls) <-> accumulator.
+__device__ __forceinline__ void acc_load(TileAcc<double> &c, const double *C, int ls) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; r++) c.re[r] = C[li * ls + lk + 4 * r];
+}
+__device__ __forceinline__ void acc_load(TileAcc<zd> &c, const zd *C, int ls) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const zd v = C[li * ls + lk + 4 * r];
+    c.re[r] = v.x;
+    c.im[r] = v.y;
+  }
+}
+__device__ __forceinline__ void acc_store(const TileAcc<double> &c, double *C, int ls) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; r++) C[li * ls + lk + 4 * r] = c.re[r];
+}
+__device__ __forceinline__ void acc_store(const TileAcc<zd> &c, zd *C, int ls) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; r++) C[li * ls + lk + 4 * r] = make_zd(c.re[r], c.im[r]);
+}
+
+// acc += sx * X op(Y), 16-deep, X and Y tiles in LDS; op(Y) = Y^H when YH.
+// MFMA operands: X fragment lane l = X(l % 16, k0 + l / 16), op(Y) fragment
+// lane l = op(Y)(k0 + l / 16, l % 16).
+template <bool YH>
+__device__ __forceinline__ void tile_mma(TileAcc<double> &c, const double *X, int ldx, const double *Y,
+                                         int ldy, double sx) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < LEAF; k0 += 4) {
+    const double a = sx * X[(k0 + lk) * ldx + li];
+    const double b = YH ? Y[(k0 + lk) * ldy + li] : Y[li * ldy + k0 + lk];
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c.re, 0, 0, 0);
+  }
+}
+template <bool YH>
+__device__ __forceinline__ void tile_mma(TileAcc<zd> &c, const zd *X, int ldx, const zd *Y, int ldy,
+                                         double sx) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < LEAF; k0 += 4) {
+    const zd a = sx * X[(k0 + lk) * ldx + li];
+    const zd b = YH ? conjx(Y[(k0 + lk) * ldy + li]) : Y[li * ldy + k0 + lk];
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ double lane_shfl(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ zd lane_shfl(zd v, int src) {
+  return make_zd(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// Orders this wave's LDS writes before its following LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// One wave factors the 16 x 16 leaf at S (leading dimension ls) in place and
+// writes its inverse (lower, zeros above) to Li (leading dimension LEAF + 1).
+// Returns the first non-positive / NaN pivot (0-based) or -1.
+template <typename T>
+__device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, T *Li) {
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  T a[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) a[j] = S[(q + 4 * j) * ls + r];
+  int bad = -1;
+#pragma unroll
+  for (int k = 0; k < LEAF; k++) {
+    const int jk = k >> 2, qk = k & 3;  // column k: lanes (*, qk), register jk
+    const T ak = a[jk];  // column k before this step's update
+    const double d = realx(lane_shfl(ak, k + 16 * qk));
+    if (!(d > 0.0)) {  // the same value in every lane: uniform
+      bad = k;
+      break;
+    }
+    const double sd = sqrt(d), rs = 1.0 / sd;
+    const T lrk = rs * lane_shfl(ak, r + 16 * qk);
+    const T fin = sel(r == k, from_real<T>(sd), lrk);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int c = q + 4 * j;
+      const T lck = conjx(rs * lane_shfl(ak, c + 16 * qk));
+      const T upd = a[j] - lrk * lck;
+      a[j] = sel(c > k, upd, sel(c == k, fin, a[j]));
+    }
+  }
+  if (bad >= 0) return bad;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int c = q + 4 * j;
+    if (r >= c) S[c * ls + r] = a[j];
+  }
+  // inverse by forward substitution over rows; the strict upper part of x
+  // stays exactly zero (it only ever gains products with zeros)
+  T x[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) x[j] = from_real<T>(0.0);
+#pragma unroll
+  for (int k = 0; k < LEAF; k++) {
+    const int jk = k >> 2, qk = k & 3;
+    const T dk = from_real<T>(1.0) / lane_shfl(a[jk], k + 16 * qk);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int c = q + 4 * j;
+      const T fin = (sel(c == k, from_real<T>(1.0), from_real<T>(0.0)) - x[j]) * dk;
+      x[j] = sel(r == k, fin, x[j]);
+    }
+    const T lrk = lane_shfl(a[jk], r + 16 * qk);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const T xkc = lane_shfl(x[j], k + 16 * q);
+      x[j] = sel(r > k, x[j] + lrk * xkc, x[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) Li[(q + 4 * j) * (LEAF + 1) + r] = x[j];
+  return -1;
+}
+
 template <typename T, int NMAX>
-__global__ void __launch_bounds__(DIAG_THREADS)
+__global__ void __launch_bounds__(BLK_THREADS)
 diag_potrf_kernel(int lower, int n, T *A, long lda, T *W, long ldw, int *info, long info_off) {
   if (*info != 0) return;
-  constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
-  __shared__ T S[NMAX * (NMAX + 1)];
-  __shared__ T vec[2 * NMAX];
-  __shared__ T dinv[NMAX];
+  constexpr int LS = NMAX + 1, NPMAX = NMAX / LEAF, LL = LEAF + 1, NW = BLK_THREADS / 64;
+  __shared__ T S[NMAX * LS];
+  __shared__ T Li[NPMAX * LEAF * LL];
+  __shared__ int flag;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int np = (n + LEAF - 1) / LEAF, npad = np * LEAF;
+  auto tile = [&](int I, int J) { return S + (J * LEAF) * LS + I * LEAF; };
   load_lower_form<T, NMAX>(S, lower, n, A, lda);
-  __syncthreads();
-  T a[RB][CB];
-  lds_to_reg<T, NMAX>(S, a, n);
-  const int bad = reg_potrf<T, NMAX>(a, vec, n);
-  if (bad >= 0) {
-    if (threadIdx.x == 0) *info = (int)(info_off + bad + 1);
-    return;
+  // pad to a whole number of leaves with the identity
+  for (int idx = tid; idx < npad * npad; idx += BLK_THREADS) {
+    const int r = idx % npad, c = idx / npad;
+    if (r >= c && (r >= n || c >= n)) S[c * LS + r] = from_real<T>(r == c ? 1.0 : 0.0);
   }
-  reg_to_lds<T, NMAX>(a, S, n);
+  if (tid == 0) flag = -1;
   __syncthreads();
+  for (int p = 0; p < np; p++) {
+    if (wave == 0) {
+      const int bad = leaf_factor_inverse<T>(tile(p, p), LS, Li + p * LEAF * LL);
+      if (bad >= 0 && (tid & 63) == 0) flag = p * LEAF + bad;
+    }
+    __syncthreads();
+    if (flag >= 0) {  // padding never fails, so flag < n
+      if (tid == 0) *info = (int)(info_off + flag + 1);
+      return;
+    }
+    for (int I = p + 1 + wave; I < np; I += NW) {  // L_Ip = A_Ip inv(L_pp)^H
+      TileAcc<T> acc;
+      acc_zero(acc);
+      tile_mma<true>(acc, tile(I, p), LS, Li + p * LEAF * LL, LL, 1.0);
+      acc_store(acc, tile(I, p), LS);
+    }
+    __syncthreads();
+    int t = 0;
+    for (int J = p + 1; J < np; J++)
+      for (int I = J; I < np; I++, t++)
+        if (t % NW == wave) {  // A_IJ -= L_Ip L_Jp^H
+          TileAcc<T> acc;
+          acc_load(acc, tile(I, J), LS);
+          tile_mma<true>(acc, tile(I, p), LS, tile(J, p), LS, -1.0);
+          acc_store(acc, tile(I, J), LS);
+        }
+    __syncthreads();
+  }
   store_lower_form<T, NMAX>(S, lower, n, A, lda, false);
-  diag_reciprocals<T, NMAX>(S, dinv, n);
   __syncthreads();
-  reg_trtri<T, NMAX>(S, a, vec, dinv, n, false);
-  __syncthreads();
-  reg_to_lds<T, NMAX>(a, S, n);
-  __syncthreads();
+  for (int p = np - 1; p >= 0; p--) {
+    // column block p below the diagonal: B_I <- sum_{J = p+1..I} X_IJ B_J,
+    // then B_I <- -B_I inv(L_pp); X_JJ for J > p is already in place
+    const int I = p + 1 + wave;  // at most one tile per wave (np <= NW + 1)
+    TileAcc<T> acc;
+    acc_zero(acc);
+    if (I < np)
+      for (int J = p + 1; J <= I; J++) tile_mma<false>(acc, tile(I, J), LS, tile(J, p), LS, 1.0);
+    __syncthreads();
+    if (I < np) {
+      acc_store(acc, tile(I, p), LS);
+      wave_lds_fence();
+      TileAcc<T> acc2;
+      acc_zero(acc2);
+      tile_mma<false>(acc2, tile(I, p), LS, Li + p * LEAF * LL, LL, -1.0);
+      wave_lds_fence();
+      acc_store(acc2, tile(I, p), LS);
+    }
+    if (wave == NW - 1) {  // X_pp = inv(L_pp), dense lower with zeros above
+      const T *src = Li + p * LEAF * LL;
+      for (int e = tid & 63; e < LEAF * LEAF; e += 64) {
+        const int r = e % LEAF, c = e / LEAF;
+        tile(p, p)[c * LS + r] = src[c * LL + r];
+      }
+    }
+    __syncthreads();
+  }
   store_dense<T, NMAX>(S, false, n, false, W, ldw);
 }
 
@@ -472,7 +617,7 @@ hipError_t diag_potrf(hipStream_t s, char uplo, int n, T *A, long lda, T *W, lon
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_potrf_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
+  hipLaunchKernelGGL((diag_potrf_kernel<T, NMAX>), dim3(1), dim3(BLK_THREADS), 0, s,
                      uplo == 'L' ? 1 : 0, n, A, lda, W, ldw, info, info_off);
   return hipGetLastError();
 }
