@@ -2,18 +2,20 @@
 """Benchmark: blocked Cholesky factorisation on one MI355X.
 
 A step is one cuDpotrf (lower, FP64) of an n = 16384 SPD matrix
-A = G G^T / n + 0.1 I (G i.i.d. N(0,1), seeded), preceded by the
-device-to-device restore of A from a pristine copy already in HBM (the
-factorisation is in place).  `value` is POTRF TFLOP/s counted as n^3/3 over
-the wall time of the K timed steps.
+A = G G^T / n + 0.1 I (G i.i.d. N(0,1), seeded).  The factorisation is in
+place, so every step (warm-up and timed) gets its own copy of A, made in HBM
+before the timed region.  `value` is POTRF TFLOP/s counted as n^3/3 over the
+wall time of the K timed steps.
 
 The JSON line also carries
-  * roofline: the dominant kernel -- the trailing-matrix SYRK of the
-    factorisation -- replayed through cuDsyrk at the exact shapes the
-    factorisation launches it with (n_t = n - j - 256, k = 256), each launch
-    bracketed by HIP events on its own stream; achieved = sum of algorithmic
-    SYRK flops n_t (n_t + 1) k / sum of launch durations, against the FP64
-    matrix-core peak;
+  * roofline: the dominant kernel -- the trailing-matrix SYRK launched by the
+    factorisation on the library's main stream -- timed live during the timed
+    steps with HIP event pairs recorded on that stream around each launch
+    (cuLAPACKSetTiming); achieved = sum of algorithmic SYRK flops
+    n_t (n_t + 1) k over the sum of launch durations, against the FP64
+    matrix-core peak; traffic = HBM bytes per launch measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters (read from the committed summary under
+    profiles/ when present);
   * cpu_baseline: the reference's CPU algorithm (the oracle's restatement,
     OpenMP) timed on this host on a bounded sample, in the same unit;
   * extra: dpotri at the same n, zpotrf / zpotri (upper, n = 8192) and
@@ -32,7 +34,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "POTRF TFLOP/s (n³/3): FP64 vs BF16/FP32+IR at FP64 accuracy; batched mats/s"
 FP64_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 EPS = 2.220446049250313e-16
-NB_OUTER = 256     # outer panel width of the library's dpotrf (trailing SYRK inner dimension)
 
 
 def package():
@@ -77,45 +78,172 @@ def backward_error(torch, A, F, uplo):
     return (torch.linalg.norm(At - R) / torch.linalg.norm(At)).item()
 
 
-def roofline_syrk(torch, pkg, A, n, lda):
-    """Replay the factorisation's trailing SYRKs and time each launch."""
-    blas = pkg.BLASHandle()
-    s = torch.cuda.Stream()
-    base = pkg.ptr(A)
-    starts, ends, flops = [], [], []
-    with torch.cuda.stream(s):
-        for j in range(0, n, NB_OUTER):
-            t = n - j - NB_OUTER
-            if t <= 0:
-                break
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            P = base + ((j + NB_OUTER) + j * lda) * 8
-            C = base + ((j + NB_OUTER) + (j + NB_OUTER) * lda) * 8
-            e0.record(s)
-            st = pkg.cuDsyrk(blas, pkg.Lower, pkg.NoTrans, t, NB_OUTER, -1e-30, P, lda, 1.0, C, lda,
-                             s.cuda_stream)
-            e1.record(s)
-            assert st == 0, st
-            starts.append(e0)
-            ends.append(e1)
-            flops.append(t * (t + 1) * NB_OUTER)
-    s.synchronize()
-    blas.close()
-    times = [a.elapsed_time(b) * 1e-3 for a, b in zip(starts, ends)]
-    total_t, total_f = sum(times), sum(flops)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "syrk_trailing_update_dpotrf_n16384_pmc.txt")
+
+
+def measured_traffic():
+    """HBM bytes per trailing-SYRK launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def roofline_live(records):
+    """Roofline entry from the (ms, flops) event records of the trailing SYRKs."""
+    total_t = sum(ms for ms, _ in records) * 1e-3
+    total_f = sum(fl for _, fl in records)
     achieved = total_f / total_t / 1e12
     return {
-        "kernel": "gemm_kernel<double, N, T, 128, 128> (trailing SYRK, lower, triangle grid)",
+        "kernel": "gemm_kernel<double, N, T, 128, 128, tri-grid> (trailing SYRK, lower)",
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": FP64_PEAK,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK, 4),
-        "traffic": None,
-        "launches": len(times),
-        "avg_launch_us": round(total_t / len(times) * 1e6, 2),
-        "flops_per_launch_avg": total_f / len(times),
+        "traffic": measured_traffic(),
+        "launches": len(records),
+        "avg_launch_us": round(total_t / len(records) * 1e6, 2),
+        "flops_per_launch_avg": total_f / len(records),
     }
+
+
+def mixed_precision(torch, pkg, h, n, nrhs, alphas):
+    """SPD solve A X = B (n x n, nrhs right-hand sides) three ways: FP64
+    cuDpotrf + cuDpotrs, and the mixed-precision cuDsposvx with FP32 and BF16
+    trailing-update products (FP64 diagonal blocks, FP64 residuals, iterative
+    refinement to LAPACK dsposv's stopping test).  Wall time of each whole
+    solve, refinement steps, and the normwise backward error
+    max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)."""
+    out = {"n": n, "nrhs": nrhs}
+    for alpha in alphas:
+        A = gpu_spd(torch, n, alpha, seed=3)
+        g = torch.Generator(device="cuda").manual_seed(4)
+        B = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)  # column-major n x nrhs
+        anorm = A.abs().sum(dim=1).max().item()
+
+        def berr(X):
+            R = B - X @ A          # rows: (b - A x)^T, A symmetric
+            num = R.abs().max(dim=1).values
+            den = anorm * X.abs().max(dim=1).values + B.abs().max(dim=1).values
+            return (num / den).max().item()
+
+        rec = {}
+        F = A.clone()
+        X = B.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, info = pkg.cuDpotrf(h, pkg.Lower, n, pkg.ptr(F), n)
+        assert st == 0 and info == 0, (st, info)
+        st, info = pkg.cuDpotrs(h, pkg.Lower, n, nrhs, pkg.ptr(F), n, pkg.ptr(X), n)
+        assert st == 0 and info == 0, (st, info)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        rec["fp64_ms"] = round(t64 * 1e3, 2)
+        rec["fp64_backward_error"] = berr(X)
+        del F
+        for factor in ("S", "B"):
+            X = torch.zeros_like(B)
+            for rep in range(2):       # the first call also grows the workspace
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st, it, used, info = pkg.cuDsposvx(h, pkg.Lower, n, nrhs, pkg.ptr(A), n, pkg.ptr(B), n,
+                                                   pkg.ptr(X), n, factor)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                assert st == 0 and info == 0, (st, info)
+                if it < 0:             # A now holds its FP64 factor: regenerate
+                    A = gpu_spd(torch, n, alpha, seed=3)
+            key = "fp32" if factor == "S" else "bf16"
+            rec[key] = {"ms": round(dt * 1e3, 2), "iter": it, "used": used,
+                        "backward_error": berr(X), "speedup_vs_fp64": round(t64 / dt, 2)}
+        out[f"alpha={alpha:g}"] = rec
+        log(f"[bench] mixed n={n} alpha={alpha:g}: {rec}")
+        del A, B, X
+        torch.cuda.empty_cache()
+    return out
+
+
+def batched(torch, pkg, h, count, nmin, nmax, reps, loop_sample):
+    """Batched POTRF + logdet of `count` SPD matrices, n_i ~ U[nmin, nmax],
+    A_i = G_i G_i^T / n_i + 0.5 I, packed contiguously (column-major, ld n_i),
+    in FP64 and FP32 storage: one graph launch per batch (cuPotrfBatchedRun),
+    timed with HIP events on the launch stream over `reps` runs (each on a
+    fresh copy of the batch, made outside the timed launch).  Compared with
+    a per-matrix loop of cuDpotrf + cuDlogdet (FP64) over the first
+    `loop_sample` matrices."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    sizes = rng.integers(nmin, nmax + 1, size=count)
+    offs = np.zeros(count, dtype=np.int64)
+    offs[1:] = np.cumsum(sizes.astype(np.int64) ** 2)[:-1]
+    total = int((sizes.astype(np.int64) ** 2).sum())
+    A0 = torch.empty(total, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(6)
+    for n in np.unique(sizes):
+        idx = np.nonzero(sizes == n)[0]
+        G = torch.randn(len(idx), n, n, generator=g, device="cuda", dtype=torch.float64)
+        M = torch.bmm(G, G.transpose(1, 2).contiguous()) / n
+        M += 0.5 * torch.eye(n, device="cuda", dtype=torch.float64)
+        for j, i in enumerate(idx):
+            A0[offs[i]:offs[i] + n * n] = M[j].reshape(-1)   # symmetric: order irrelevant
+        del G, M
+    flops = float((sizes.astype(np.float64) ** 3).sum() / 3)
+    tri_bytes = float((sizes.astype(np.float64) * (sizes + 1) / 2).sum())
+    out = {"count": count, "n_range": [nmin, nmax], "flops_n3_3": flops}
+    s = torch.cuda.Stream()
+    for prec, dtype, esz in (("D", torch.float64, 8), ("S", torch.float32, 4)):
+        src = A0.to(dtype)
+        work = torch.empty_like(src)
+        logd = torch.empty(count, dtype=torch.float64, device="cuda")
+        info = torch.empty(count, dtype=torch.int32, device="cuda")
+        times = []
+        with pkg.BatchedPlan(h, pkg.Lower, prec, sizes.tolist(), offs.tolist()) as plan:
+            assert plan.status == 0
+            for r in range(reps + 1):
+                work.copy_(src)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                assert plan.run(pkg.ptr(work), pkg.ptr(logd), pkg.ptr(info), s.cuda_stream) == 0
+                e1.record(s)
+                s.synchronize()
+                if r > 0:            # the first run captures the graph
+                    times.append(e0.elapsed_time(e1) * 1e-3)
+        assert int(info.abs().sum().item()) == 0
+        t = sorted(times)[len(times) // 2]
+        byts = 2 * tri_bytes * esz            # read and write one triangle per matrix
+        out["fp64" if prec == "D" else "fp32"] = {
+            "ms": round(t * 1e3, 3), "matrices_per_s": round(count / t), "tflops": round(flops / t / 1e12, 3),
+            "hbm_gbs_algorithmic": round(byts / t / 1e9, 1),
+            "hbm_frac": round(byts / t / 8e12, 4)}
+        if prec == "D":
+            ref_log = logd.clone()
+        del src, work
+    # per-matrix FP64 loop: cuDpotrf + cuDlogdet on each matrix
+    k = min(loop_sample, count)
+    work = A0[: int(offs[k - 1] + sizes[k - 1] ** 2)].clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        n = int(sizes[i])
+        ptr = pkg.ptr(work) + int(offs[i]) * 8
+        st, inf = pkg.cuDpotrf(h, pkg.Lower, n, ptr, n)
+        assert st == 0 and inf == 0
+        st, ld = pkg.cuDlogdet(h, ptr, n + 1, n)
+        assert abs(ld - ref_log[i].item()) <= 1e-9 * max(1.0, abs(ld)), (i, ld, ref_log[i].item())
+    t_loop = time.perf_counter() - t0
+    out["per_matrix_loop_fp64"] = {"sample": k, "matrices_per_s": round(k / t_loop)}
+    out["speedup_fp64_vs_loop"] = round(out["fp64"]["matrices_per_s"] / (k / t_loop), 1)
+    log(f"[bench] batched: {out}")
+    del A0, work
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(torch, n_cpu):
@@ -153,6 +281,14 @@ def main():
     ap.add_argument("--n-cpu", type=int, default=12288, help="CPU baseline sample order")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip dpotri / zpotrf / zpotri")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision solve (n = 32768, 16 right-hand sides)")
+    ap.add_argument("--n-mixed", type=int, default=32768)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched factorisation (20000 matrices, n in [8, 256])")
+    ap.add_argument("--counter-calibration", action="store_true",
+                    help="run one elementwise copy of A (n^2 * 8 bytes read and written) before the "
+                         "steps, as the known-traffic reference kernel of a rocprofv3 --pmc run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
@@ -164,36 +300,44 @@ def main():
     pkg = package()
     pkg.lib()
     n, lda = args.n, args.n
-    log(f"[bench] generating A (n={n}) on the GPU")
+    log(f"[bench] generating A (n={n}) and {args.warmup + args.steps} copies in HBM")
     A0 = gpu_spd(torch, n, 0.1, seed=1)        # symmetric: storage order is irrelevant
-    A = torch.empty_like(A0)
+    copies = [A0.clone() for _ in range(args.warmup + args.steps)]
     torch.cuda.synchronize()
+    if args.counter_calibration:
+        torch.mul(A0, 1.0, out=copies[0])
+        torch.cuda.synchronize()
+        copies[0].copy_(A0)
+        torch.cuda.synchronize()
     h = pkg.LAPACKHandle()
 
-    def step():
-        A.copy_(A0)
-        torch.cuda.current_stream().synchronize()
+    def step(A):
         st, info = pkg.cuDpotrf(h, pkg.Lower, n, pkg.ptr(A), lda)
         if st != 0 or info != 0:
             raise RuntimeError(f"cuDpotrf failed: status {st} info {info}")
 
     for i in range(args.warmup):
-        step()
+        step(copies[i])
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    pkg.cuLAPACKSetTiming(h, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    for i in range(args.steps):
+        step(copies[args.warmup + i])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    st, records = pkg.cuLAPACKGetTiming(h)
+    pkg.cuLAPACKSetTiming(h, 0)
+    assert st == 0 and records, st
     ms = (t1 - t0) * 1e3 / args.steps
     tflops = n ** 3 / 3 / (ms * 1e-3) / 1e12
+    A = copies[-1]
+    del copies[:-1]
     bwd = backward_error(torch, A0, A, "L")
     log(f"[bench] dpotrf n={n}: {ms:.2f} ms/step, {tflops:.2f} TFLOP/s, backward error {bwd:.2e}")
-
-    log("[bench] roofline replay of the trailing SYRK")
-    roof = roofline_syrk(torch, pkg, A0.clone(), n, lda)
-    log(f"[bench] syrk achieved {roof['achieved']} TFLOP/s ({roof['frac'] * 100:.1f}% of FP64 peak)")
+    roof = roofline_live(records)
+    log(f"[bench] trailing syrk: {roof['launches']} launches, avg {roof['avg_launch_us']} us, "
+        f"{roof['achieved']} TFLOP/s ({roof['frac'] * 100:.1f}% of FP64 peak)")
 
     extra = {"dpotrf_backward_error": bwd, "dpotrf_backward_bound": 10 * n * EPS}
     if not args.no_extra:
@@ -233,6 +377,15 @@ def main():
         del Z, Z0, Zc
         log("[bench] complex done")
 
+    if not args.no_mixed:
+        del A
+        torch.cuda.empty_cache()
+        extra["mixed_precision_solve"] = mixed_precision(torch, pkg, h, args.n_mixed, 16,
+                                                         (1.0, 1e-3, 1e-6))
+
+    if not args.no_batched:
+        extra["batched_potrf_logdet"] = batched(torch, pkg, h, 20000, 8, 256, 5, 2000)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline n={args.n_cpu}")
@@ -254,7 +407,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"dpotrf lower FP64 n={n} lda={lda}, A = G G^T/n + 0.1 I (seeded)",
-                   "n": n, "uplo": "L", "step": "restore A (D2D copy) + cuDpotrf"},
+                   "n": n, "uplo": "L", "step": "cuDpotrf on a fresh HBM-resident copy of A"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -91,6 +91,25 @@ SIGNATURES = {
     "cuDlogdet": [_h, _ptr, _sz, _sz, ctypes.POINTER(ctypes.c_double), _stream],
     "cuZlogdet": [_h, _ptr, _sz, _sz, ctypes.POINTER(ctypes.c_double), _stream],
     "cuGetErrorString": [_res],
+    "cuSsyrk": [_h, _enum, _enum, _sz, _sz, ctypes.c_float, _ptr, _sz, ctypes.c_float, _ptr, _sz,
+                _stream],
+    "cuSgemm2": [_h, _enum, _enum, _sz, _sz, _sz, ctypes.c_float, _ptr, _sz, _ptr, _sz,
+                 ctypes.c_float, _ptr, _sz, _ptr, _sz, _stream],
+    "cuStrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_float, _ptr, _sz, _ptr, _sz,
+                _stream],
+    "cuSpotrf": [_h, _enum, _sz, _ptr, _sz, _longp],
+    # include/cucholesky.h (extensions)
+    "cuDpotrs": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _longp],
+    "cuDsposv": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, _longp, _longp],
+    "cuDsposvx": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, ctypes.c_char, _longp,
+                  ctypes.c_char_p, _longp],
+    "cuPotrfBatchedPlanCreate": [_h, _enum, ctypes.c_char, _sz, ctypes.POINTER(ctypes.c_size_t),
+                                 ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_h)],
+    "cuPotrfBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
+    "cuPotrfBatchedPlanDestroy": [_h],
+    "cuLAPACKSetTiming": [_h, ctypes.c_int],
+    "cuLAPACKGetTiming": [_h, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.c_long, _longp],
 }
 
 XERBLA_T = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_long)
@@ -183,6 +202,20 @@ def _hv(handle):
     return handle.h if hasattr(handle, "h") else handle
 
 
+def cuLAPACKSetTiming(handle, enable):
+    return lib().cuLAPACKSetTiming(_hv(handle), int(enable))
+
+
+def cuLAPACKGetTiming(handle, capacity=4096):
+    """(status, [(ms, flops), ...]) of the recorded trailing updates."""
+    ms = (ctypes.c_double * capacity)()
+    fl = (ctypes.c_double * capacity)()
+    count = ctypes.c_long(0)
+    st = lib().cuLAPACKGetTiming(_hv(handle), ms, fl, capacity, ctypes.byref(count))
+    n = min(count.value, capacity)
+    return st, [(ms[i], fl[i]) for i in range(n)]
+
+
 # ---- LAPACK (status, info) ----
 
 def _info_call(fn, *args):
@@ -193,6 +226,61 @@ def _info_call(fn, *args):
 
 def cuDpotrf(handle, uplo, n, A, lda):
     return _info_call(lib().cuDpotrf, _hv(handle), uplo, n, A, lda)
+
+
+def cuSpotrf(handle, uplo, n, A, lda):
+    return _info_call(lib().cuSpotrf, _hv(handle), uplo, n, A, lda)
+
+
+def cuDpotrs(handle, uplo, n, nrhs, A, lda, B, ldb):
+    return _info_call(lib().cuDpotrs, _hv(handle), uplo, n, nrhs, A, lda, B, ldb)
+
+
+def cuDsposv(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx):
+    """(status, iter, info)"""
+    it, info = ctypes.c_long(0), ctypes.c_long(0)
+    st = lib().cuDsposv(_hv(handle), uplo, n, nrhs, A, lda, B, ldb, X, ldx, ctypes.byref(it),
+                        ctypes.byref(info))
+    return st, it.value, info.value
+
+
+def cuDsposvx(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor="S"):
+    """(status, iter, used precision 'B' / 'S' / 'D', info)"""
+    it, info = ctypes.c_long(0), ctypes.c_long(0)
+    used = ctypes.create_string_buffer(1)
+    st = lib().cuDsposvx(_hv(handle), uplo, n, nrhs, A, lda, B, ldb, X, ldx,
+                         factor.encode() if isinstance(factor, str) else factor, ctypes.byref(it),
+                         used, ctypes.byref(info))
+    return st, it.value, used.raw[:1].decode(), info.value
+
+
+class BatchedPlan:
+    """cuPotrfBatchedPlanCreate / Run / Destroy for packed matrices of sizes n
+    at element offsets `offsets` (host sequences)."""
+
+    def __init__(self, handle, uplo, precision, n, offsets):
+        count = len(n)
+        nn = (ctypes.c_size_t * max(count, 1))(*n)
+        oo = (ctypes.c_size_t * max(count, 1))(*offsets)
+        self.h = _h()
+        self.status = lib().cuPotrfBatchedPlanCreate(_hv(handle), uplo,
+                                                     precision.encode() if isinstance(precision, str)
+                                                     else precision, count, nn, oo,
+                                                     ctypes.byref(self.h))
+
+    def run(self, A, logdet, info, stream=None):
+        return lib().cuPotrfBatchedRun(self.h, A, logdet, info, stream)
+
+    def close(self):
+        if self.h:
+            lib().cuPotrfBatchedPlanDestroy(self.h)
+            self.h = _h()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def cuZpotrf(handle, uplo, n, A, lda):
@@ -287,6 +375,20 @@ def cuZtrmm2(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ld
 
 def cuDtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
     return lib().cuDtrsm(_hv(handle), side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream)
+
+
+def cuSgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, D, ldd,
+             stream=None):
+    return lib().cuSgemm2(_hv(handle), transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                          D, ldd, stream)
+
+
+def cuSsyrk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream=None):
+    return lib().cuSsyrk(_hv(handle), uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream)
+
+
+def cuStrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
+    return lib().cuStrsm(_hv(handle), side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream)
 
 
 def cuZtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):

@@ -1,0 +1,155 @@
+// Batched Cholesky factorisation + log-determinant of many small SPD
+// matrices of uneven sizes, packed contiguously (matrix i: n_i x n_i,
+// column-major with leading dimension n_i, at element offset off_i).
+//
+// One persistent kernel: workgroup w factors matrices order[w], order[w + G],
+// ... (G = grid size; `order` lists the matrices by decreasing n, so the
+// round-robin deal is a longest-first schedule).  Each matrix is factored in
+// place, right-looking over 16-column leaves:
+//   * wave 0 factors leaf p in registers (cross-lane shuffles), writes L_pp,
+//     accumulates log of the pivots and inverts L_pp into LDS;
+//   * all waves form the panel L_Ip = A_Ip inv(L_pp)^T on the matrix cores
+//     (written to the matrix and kept in LDS);
+//   * all waves update the trailing lower tiles A_IJ -= L_Ip L_Jp^T with
+//     operands from LDS.
+// Storage is FP64 or FP32 (ST); arithmetic is FP64 throughout.  Only the
+// referenced triangle (uplo) is read or written; tiles reaching past n read
+// the identity and are not written back.
+#include "internal.h"
+#include "tile16.h"
+
+namespace hc {
+namespace {
+
+using t16::LEAF;
+using t16::TileAcc;
+
+constexpr int BT_THREADS = 256;                // 4 waves per matrix
+constexpr int BT_NW = BT_THREADS / 64;
+constexpr int BT_NPMAX = kBatchedMaxN / LEAF;  // leaves per matrix at most
+constexpr int LL = LEAF + 1;                   // LDS tile leading dimension
+
+// Lower-form view of one packed matrix: element (r, c), r >= c, is A(r, c)
+// for lower storage and A(c, r) for upper storage.
+template <typename ST, bool LOWER>
+struct View {
+  ST *M;
+  int n;
+  __device__ long off(int r, int c) const { return LOWER ? (long)c * n + r : (long)r * n + c; }
+  // value of lower-form (r, c): zero above the diagonal, identity beyond n
+  __device__ double get(int r, int c) const {
+    if (r < n && c < n) return r >= c ? (double)M[off(r, c)] : 0.0;
+    return r == c ? 1.0 : 0.0;
+  }
+  __device__ void put(int r, int c, double v) const {
+    if (r < n && c < n && r >= c) M[off(r, c)] = (ST)v;
+  }
+};
+
+template <typename ST, bool LOWER>
+__global__ void __launch_bounds__(BT_THREADS)
+potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *order, ST *A,
+                     double *logdet, int *info) {
+  __shared__ double Li[LEAF * LL];                 // inv(L_pp)
+  __shared__ double P[BT_NPMAX * LEAF * LL];       // panel tiles L_Ip
+  __shared__ int flag;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int idx = order[t];
+    const int n = nvec[idx];
+    const View<ST, LOWER> v{A + offs[idx], n};
+    const int np = (n + LEAF - 1) / LEAF;
+    double logsum = 0.0;
+    int bad = -1;
+    for (int p = 0; p < np; p++) {
+      const int g0 = p * LEAF;
+      if (wave == 0) {
+        const int r = li, q = lk;
+        double a[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) a[j] = v.get(g0 + r, g0 + q + 4 * j);
+        const int b = t16::leaf_factor<double>(a, n - g0, &logsum);
+        if (b < 0) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) v.put(g0 + r, g0 + q + 4 * j, a[j]);
+          double x[4];
+          t16::leaf_inverse<double>(a, x);
+#pragma unroll
+          for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
+        }
+        if (lane == 0) flag = b < 0 ? -1 : g0 + b;
+      }
+      __syncthreads();
+      bad = flag;
+      if (bad >= 0) break;
+      // panel: L_Ip = A_Ip inv(L_pp)^T, to the matrix and to LDS
+      for (int I = p + 1 + wave; I < np; I += BT_NW) {
+        TileAcc<double> acc;
+        t16::acc_zero(acc);
+#pragma unroll
+        for (int k0 = 0; k0 < LEAF; k0 += 4) {
+          const double x = v.get(I * LEAF + li, g0 + k0 + lk);
+          const double y = Li[(k0 + lk) * LL + li];
+          acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+        }
+        double *Pt = P + I * LEAF * LL;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = lk + 4 * r, col = li;
+          Pt[col * LL + row] = acc.re[r];
+          v.put(I * LEAF + row, g0 + col, acc.re[r]);
+        }
+      }
+      __syncthreads();
+      // trailing update of the lower tiles, operands from LDS
+      int tt = 0;
+      for (int J = p + 1; J < np; J++) {
+        for (int I = J; I < np; I++, tt++) {
+          if (tt % BT_NW != wave) continue;
+          TileAcc<double> acc;
+#pragma unroll
+          for (int r = 0; r < 4; r++) acc.re[r] = v.get(I * LEAF + lk + 4 * r, J * LEAF + li);
+          const double *PI = P + I * LEAF * LL, *PJ = P + J * LEAF * LL;
+#pragma unroll
+          for (int k0 = 0; k0 < LEAF; k0 += 4) {
+            const double x = -PI[(k0 + lk) * LL + li];
+            const double y = PJ[(k0 + lk) * LL + li];
+            acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++) v.put(I * LEAF + lk + 4 * r, J * LEAF + li, acc.re[r]);
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      info[idx] = bad >= 0 ? bad + 1 : 0;
+      if (logdet != nullptr) logdet[idx] = bad >= 0 ? __builtin_nan("") : logsum;
+    }
+    __syncthreads();  // flag and LDS are reused by the next matrix
+  }
+}
+
+}  // namespace
+
+template <typename ST>
+hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,
+                         const int *order, ST *A, double *logdet, int *info, int grid) {
+  if (count <= 0) return hipSuccess;
+  if (grid < 1) grid = 1;
+  if (uplo == 'L')
+    hipLaunchKernelGGL((potrf_batched_kernel<ST, true>), dim3((unsigned)grid), dim3(BT_THREADS), 0, s,
+                       count, nvec, offs, order, A, logdet, info);
+  else
+    hipLaunchKernelGGL((potrf_batched_kernel<ST, false>), dim3((unsigned)grid), dim3(BT_THREADS), 0,
+                       s, count, nvec, offs, order, A, logdet, info);
+  return hipGetLastError();
+}
+
+template hipError_t potrf_batched<double>(hipStream_t, char, int, const int *, const long *,
+                                          const int *, double *, double *, int *, int);
+template hipError_t potrf_batched<float>(hipStream_t, char, int, const int *, const long *,
+                                         const int *, float *, double *, int *, int);
+
+}  // namespace hc

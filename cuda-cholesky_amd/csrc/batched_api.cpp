@@ -1,0 +1,145 @@
+// Batched Cholesky + log-determinant of many small SPD matrices of uneven
+// sizes (no counterpart in the reference; its routines take one matrix per
+// call).  A plan fixes the sizes, offsets and schedule once; each run is one
+// persistent kernel, captured in a HIP graph on first use and replayed with a
+// single graph launch.
+#include "api_util.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <numeric>
+#include <vector>
+
+#include "cucholesky.h"
+
+using hc::bad_arg;
+
+struct __cubatchedplan_st {
+  int device;
+  char uplo;
+  char precision;  // 'D' (FP64 storage) or 'S' (FP32 storage)
+  int count;
+  int grid;
+  int *d_n;
+  long *d_off;
+  int *d_order;
+  // graph for the last (A, logdet, info) triple
+  hipStream_t capture;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  CUdeviceptr gA, gLog, gInfo;
+};
+
+namespace {
+
+hipError_t launch(const __cubatchedplan_st *p, hipStream_t s, CUdeviceptr A, CUdeviceptr logdet,
+                  CUdeviceptr info) {
+  double *ld = reinterpret_cast<double *>(logdet);
+  int *inf = reinterpret_cast<int *>(info);
+  if (p->precision == 'D')
+    return hc::potrf_batched<double>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
+                                     reinterpret_cast<double *>(A), ld, inf, p->grid);
+  return hc::potrf_batched<float>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
+                                  reinterpret_cast<float *>(A), ld, inf, p->grid);
+}
+
+void release_graph(__cubatchedplan_st *p) {
+  if (p->exec != nullptr) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph != nullptr) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precision, size_t count,
+                                  const size_t *n, const size_t *offsets, CUbatchedplan *plan) {
+  const char *name = "cuPotrfBatchedPlanCreate";
+  if (h == nullptr) return bad_arg(name, 1);
+  if (!hc::valid_uplo(uplo)) return bad_arg(name, 2);
+  if (precision != 'D' && precision != 'S') return bad_arg(name, 3);
+  if (count > (size_t)0x7fffffff) return bad_arg(name, 4);
+  if (count > 0 && n == nullptr) return bad_arg(name, 5);
+  if (count > 0 && offsets == nullptr) return bad_arg(name, 6);
+  if (plan == nullptr) return bad_arg(name, 7);
+  for (size_t i = 0; i < count; i++)
+    if (n[i] > (size_t)hc::kBatchedMaxN) return bad_arg(name, 5);
+  hc::DeviceGuard g(h->device);
+  HC_CHECK(g.status());
+  auto *p = static_cast<__cubatchedplan_st *>(std::calloc(1, sizeof(__cubatchedplan_st)));
+  if (p == nullptr) return CUDA_ERROR_OUT_OF_MEMORY;
+  p->device = h->device;
+  p->uplo = (char)uplo;
+  p->precision = precision;
+  p->count = (int)count;
+  // longest first: the round-robin deal over the persistent grid then
+  // balances uneven sizes (work grows as n^3)
+  std::vector<int> order(count), nv(count);
+  std::vector<long> off(count);
+  for (size_t i = 0; i < count; i++) {
+    nv[i] = (int)n[i];
+    off[i] = (long)offsets[i];
+  }
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nv[a] > nv[b]; });
+  // persistent grid: every CU filled to the kernel's occupancy
+  int cus = 0;
+  HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+  p->grid = std::max(1, std::min<int>((int)count, cus * hc::kBatchedBlocksPerCU));
+  if (count > 0) {
+    HC_CHECK(hipMalloc(&p->d_n, count * sizeof(int)));
+    HC_CHECK(hipMalloc(&p->d_off, count * sizeof(long)));
+    HC_CHECK(hipMalloc(&p->d_order, count * sizeof(int)));
+    HC_CHECK(hipMemcpy(p->d_n, nv.data(), count * sizeof(int), hipMemcpyHostToDevice));
+    HC_CHECK(hipMemcpy(p->d_off, off.data(), count * sizeof(long), hipMemcpyHostToDevice));
+    HC_CHECK(hipMemcpy(p->d_order, order.data(), count * sizeof(int), hipMemcpyHostToDevice));
+  }
+  HC_CHECK(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
+  *plan = p;
+  return CUDA_SUCCESS;
+}
+
+CUresult cuPotrfBatchedRun(CUbatchedplan p, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
+                           CUstream stream) {
+  const char *name = "cuPotrfBatchedRun";
+  if (p == nullptr) return bad_arg(name, 1);
+  if (p->count > 0 && A == 0) return bad_arg(name, 2);
+  if (p->count > 0 && info == 0) return bad_arg(name, 4);
+  if (p->count == 0) return CUDA_SUCCESS;
+  hc::DeviceGuard g(p->device);
+  HC_CHECK(g.status());
+  if (p->exec == nullptr || p->gA != A || p->gLog != logdet || p->gInfo != info) {
+    release_graph(p);
+    HC_CHECK(hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = launch(p, p->capture, A, logdet, info);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(p->capture, &graph);
+    HC_CHECK(e);
+    HC_CHECK(e2);
+    p->graph = graph;
+    HC_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    p->gA = A;
+    p->gLog = logdet;
+    p->gInfo = info;
+  }
+  HC_CHECK(hipGraphLaunch(p->exec, stream));
+  return CUDA_SUCCESS;
+}
+
+CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan p) {
+  if (p == nullptr) return CUDA_SUCCESS;
+  hc::DeviceGuard g(p->device);
+  HC_CHECK(g.status());
+  HC_CHECK(hipDeviceSynchronize());
+  release_graph(p);
+  if (p->capture != nullptr) HC_CHECK(hipStreamDestroy(p->capture));
+  if (p->d_n != nullptr) HC_CHECK(hipFree(p->d_n));
+  if (p->d_off != nullptr) HC_CHECK(hipFree(p->d_off));
+  if (p->d_order != nullptr) HC_CHECK(hipFree(p->d_order));
+  std::free(p);
+  return CUDA_SUCCESS;
+}
+
+}  // extern "C"

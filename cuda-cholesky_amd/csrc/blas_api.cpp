@@ -170,6 +170,29 @@ CUresult cuDtrsm(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTrans
                           stream);
 }
 
+// Single precision (reference include/blas.h:158,193,235): FP32 data and
+// accumulation on the FP32 matrix-core path of the mixed-precision solver.
+CUresult cuSgemm2(CUBLAShandle handle, CBlasTranspose transA, CBlasTranspose transB, size_t m,
+                  size_t n, size_t k, float alpha, CUdeviceptr A, size_t lda, CUdeviceptr B,
+                  size_t ldb, float beta, CUdeviceptr C, size_t ldc, CUdeviceptr D, size_t ldd,
+                  CUstream stream) {
+  return gemm_api<float>("cuSgemm2", handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta,
+                         C, ldc, D, ldd, stream);
+}
+
+CUresult cuSsyrk(CUBLAShandle handle, CBlasUplo uplo, CBlasTranspose trans, size_t n, size_t k,
+                 float alpha, CUdeviceptr A, size_t lda, float beta, CUdeviceptr C, size_t ldc,
+                 CUstream stream) {
+  return herk_api<float>("cuSsyrk", handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream);
+}
+
+CUresult cuStrsm(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
+                 CBlasDiag diag, size_t m, size_t n, float alpha, CUdeviceptr A, size_t lda,
+                 CUdeviceptr B, size_t ldb, CUstream stream) {
+  return trsm_api<float>("cuStrsm", handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb,
+                         stream);
+}
+
 CUresult cuZtrsm(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
                  CBlasDiag diag, size_t m, size_t n, cuDoubleComplex_t alpha, CUdeviceptr A,
                  size_t lda, CUdeviceptr B, size_t ldb, CUstream stream) {

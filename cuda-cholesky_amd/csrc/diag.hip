@@ -14,6 +14,7 @@
 //   trsm_diag_inverses  batched inverses of op(A)'s diagonal blocks
 //   logdet       2 sum log(Re x_i)  (reference lapack/dlogdet.cu:53-118)
 #include "internal.h"
+#include "tile16.h"
 
 #include <climits>
 
@@ -83,8 +84,8 @@ __device__ __forceinline__ void store_lower_form(const T *S, bool lower, int n, 
 
 // W(r, c) = S(r, c) for r >= c, zero above; unit -> ones on the diagonal.
 // With upper, W is the conjugate transpose instead (upper triangular).
-template <typename T, int NMAX>
-__device__ __forceinline__ void store_dense(const T *S, bool upper, int n, bool unit, T *W, long ldw) {
+template <typename T, int NMAX, typename ST = T>
+__device__ __forceinline__ void store_dense(const T *S, bool upper, int n, bool unit, ST *W, long ldw) {
   constexpr int LS = Lds<T, NMAX>::LS;
   for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
     const int r = idx % n, c = idx / n;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void store_dense(const T *S, bool upper, int n, bool 
     if (r == c) v = unit ? from_real<T>(1.0) : (upper ? conjx(S[c * LS + r]) : S[c * LS + r]);
     else if (!upper && r > c) v = S[c * LS + r];
     else if (upper && r < c) v = conjx(S[r * LS + c]);
-    W[(long)c * ldw + r] = v;
+    W[(long)c * ldw + r] = convert<ST>(v);
   }
 }
 
@@ -252,171 +253,82 @@ __device__ __forceinline__ void diag_reciprocals(const T *S, T *dinv, int n) {
 // (X_Ip = -(sum_J X_IJ L_Jp) inv(L_pp), as in LAPACK xTRTRI).  Three
 // barriers per leaf for the factor and two for the inverse.
 // ---------------------------------------------------------------------------
-constexpr int LEAF = 16;
+using t16::LEAF;
+using t16::TileAcc;
+using t16::acc_zero;
+using t16::acc_load;
+using t16::acc_store;
+using t16::tile_mma;
+using t16::wave_lds_fence;
+
 constexpr int BLK_THREADS = 512;
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-// A 16 x 16 tile accumulator in the 16x16x4 f64 result layout: lane l,
-// register r holds element (l / 16 + 4 r, l % 16).
-template <typename T> struct TileAcc;
-template <> struct TileAcc<double> { v4d re; };
-template <> struct TileAcc<zd> { v4d re, im; };
-
-__device__ __forceinline__ void acc_zero(TileAcc<double> &c) { c.re = (v4d){0.0, 0.0, 0.0, 0.0}; }
-__device__ __forceinline__ void acc_zero(TileAcc<zd> &c) {
-  c.re = (v4d){0.0, 0.0, 0.0, 0.0};
-  c.im = c.re;
-}
-
-// Tile at C (column-major, leading dimension ls) <-> accumulator.
-__device__ __forceinline__ void acc_load(TileAcc<double> &c, const double *C, int ls) {
-  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-#pragma unroll
-  for (int r = 0; r < 4; r++) c.re[r] = C[li * ls + lk + 4 * r];
-}
-__device__ __forceinline__ void acc_load(TileAcc<zd> &c, const zd *C, int ls) {
-  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const zd v = C[li * ls + lk + 4 * r];
-    c.re[r] = v.x;
-    c.im[r] = v.y;
-  }
-}
-__device__ __forceinline__ void acc_store(const TileAcc<double> &c, double *C, int ls) {
-  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-#pragma unroll
-  for (int r = 0; r < 4; r++) C[li * ls + lk + 4 * r] = c.re[r];
-}
-__device__ __forceinline__ void acc_store(const TileAcc<zd> &c, zd *C, int ls) {
-  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-#pragma unroll
-  for (int r = 0; r < 4; r++) C[li * ls + lk + 4 * r] = make_zd(c.re[r], c.im[r]);
-}
-
-// acc += sx * X op(Y), 16-deep, X and Y tiles in LDS; op(Y) = Y^H when YH.
-// MFMA operands: X fragment lane l = X(l % 16, k0 + l / 16), op(Y) fragment
-// lane l = op(Y)(k0 + l / 16, l % 16).
-template <bool YH>
-__device__ __forceinline__ void tile_mma(TileAcc<double> &c, const double *X, int ldx, const double *Y,
-                                         int ldy, double sx) {
-  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-#pragma unroll
-  for (int k0 = 0; k0 < LEAF; k0 += 4) {
-    const double a = sx * X[(k0 + lk) * ldx + li];
-    const double b = YH ? Y[(k0 + lk) * ldy + li] : Y[li * ldy + k0 + lk];
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c.re, 0, 0, 0);
-  }
-}
-template <bool YH>
-__device__ __forceinline__ void tile_mma(TileAcc<zd> &c, const zd *X, int ldx, const zd *Y, int ldy,
-                                         double sx) {
-  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-#pragma unroll
-  for (int k0 = 0; k0 < LEAF; k0 += 4) {
-    const zd a = sx * X[(k0 + lk) * ldx + li];
-    const zd b = YH ? conjx(Y[(k0 + lk) * ldy + li]) : Y[li * ldy + k0 + lk];
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
-  }
-}
-
-__device__ __forceinline__ double lane_shfl(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ zd lane_shfl(zd v, int src) {
-  return make_zd(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
-
-// Orders this wave's LDS writes before its following LDS reads by other lanes.
-__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
-
-// One wave factors the 16 x 16 leaf at S (leading dimension ls) in place and
-// writes its inverse (lower, zeros above) to Li (leading dimension LEAF + 1).
-// Returns the first non-positive / NaN pivot (0-based) or -1.
-template <typename T>
-__device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, T *Li) {
+// One wave factors the 16 x 16 leaf at S (leading dimension ls), writes the
+// factor straight to its place in the global matrix (rows and columns
+// g0 .. g0 + 15 of the n x n block at A, lower form: A(r, c) when lower,
+// conj into A(c, r) when upper; padding beyond n is not written) and
+// overwrites the leaf with its inverse (lower, zeros above).  Returns the
+// first non-positive / NaN pivot (0-based) or -1; on failure nothing is
+// written.
+template <typename T, typename ST>
+__device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int n, int g0, ST *A,
+                                                long lda) {
   const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
   T a[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) a[j] = S[(q + 4 * j) * ls + r];
-  int bad = -1;
-#pragma unroll
-  for (int k = 0; k < LEAF; k++) {
-    const int jk = k >> 2, qk = k & 3;  // column k: lanes (*, qk), register jk
-    const T ak = a[jk];  // column k before this step's update
-    const double d = realx(lane_shfl(ak, k + 16 * qk));
-    if (!(d > 0.0)) {  // the same value in every lane: uniform
-      bad = k;
-      break;
-    }
-    const double sd = sqrt(d), rs = 1.0 / sd;
-    const T lrk = rs * lane_shfl(ak, r + 16 * qk);
-    const T fin = sel(r == k, from_real<T>(sd), lrk);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int c = q + 4 * j;
-      const T lck = conjx(rs * lane_shfl(ak, c + 16 * qk));
-      const T upd = a[j] - lrk * lck;
-      a[j] = sel(c > k, upd, sel(c == k, fin, a[j]));
-    }
-  }
+  double unused = 0.0;
+  const int bad = t16::leaf_factor<T>(a, 0, &unused);
   if (bad >= 0) return bad;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int c = q + 4 * j;
-    if (r >= c) S[c * ls + r] = a[j];
+    const int c = q + 4 * j, gr = g0 + r, gc = g0 + c;
+    if (r >= c && gr < n) {
+      if (lower) A[(long)gc * lda + gr] = convert<ST>(a[j]);
+      else A[(long)gr * lda + gc] = convert<ST>(conjx(a[j]));
+    }
   }
-  // inverse by forward substitution over rows; the strict upper part of x
-  // stays exactly zero (it only ever gains products with zeros)
   T x[4];
+  t16::leaf_inverse<T>(a, x);
 #pragma unroll
-  for (int j = 0; j < 4; j++) x[j] = from_real<T>(0.0);
-#pragma unroll
-  for (int k = 0; k < LEAF; k++) {
-    const int jk = k >> 2, qk = k & 3;
-    const T dk = from_real<T>(1.0) / lane_shfl(a[jk], k + 16 * qk);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int c = q + 4 * j;
-      const T fin = (sel(c == k, from_real<T>(1.0), from_real<T>(0.0)) - x[j]) * dk;
-      x[j] = sel(r == k, fin, x[j]);
-    }
-    const T lrk = lane_shfl(a[jk], r + 16 * qk);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const T xkc = lane_shfl(x[j], k + 16 * q);
-      x[j] = sel(r > k, x[j] + lrk * xkc, x[j]);
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) Li[(q + 4 * j) * (LEAF + 1) + r] = x[j];
+  for (int j = 0; j < 4; j++) S[(q + 4 * j) * ls + r] = x[j];
   return -1;
 }
 
-template <typename T, int NMAX>
+// LDS holds only the lower-triangle tiles, packed: tile (I, J), I >= J, at
+// index I (I + 1) / 2 + J, each 16 x 16 column-major with leading dimension
+// 17 (78 KB for a 128 x 128 real block), so the kernel fits beside a
+// trailing-update workgroup on the same CU.  Once leaf p is factored its tile
+// holds inv(L_pp); its factor has already gone to global memory.
+// T is the compute type (the LDS copy), ST the storage type of A and W
+// (float for the mixed-precision factorisation, which computes the diagonal
+// blocks in double).
+template <typename T, typename ST, int NMAX>
 __global__ void __launch_bounds__(BLK_THREADS)
-diag_potrf_kernel(int lower, int n, T *A, long lda, T *W, long ldw, int *info, long info_off) {
+diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info, long info_off) {
   if (*info != 0) return;
-  constexpr int LS = NMAX + 1, NPMAX = NMAX / LEAF, LL = LEAF + 1, NW = BLK_THREADS / 64;
-  __shared__ T S[NMAX * LS];
-  __shared__ T Li[NPMAX * LEAF * LL];
+  constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL, NW = BLK_THREADS / 64;
+  __shared__ T S[NPMAX * (NPMAX + 1) / 2 * TS];
   __shared__ int flag;
   const int tid = threadIdx.x, wave = tid >> 6;
   const int np = (n + LEAF - 1) / LEAF, npad = np * LEAF;
-  auto tile = [&](int I, int J) { return S + (J * LEAF) * LS + I * LEAF; };
-  load_lower_form<T, NMAX>(S, lower, n, A, lda);
-  // pad to a whole number of leaves with the identity
+  auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
+  auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
+  // lower form, read along memory's contiguous index; identity padding
   for (int idx = tid; idx < npad * npad; idx += BLK_THREADS) {
-    const int r = idx % npad, c = idx / npad;
-    if (r >= c && (r >= n || c >= n)) S[c * LS + r] = from_real<T>(r == c ? 1.0 : 0.0);
+    const int i = idx % npad, k = idx / npad;
+    const int r = lower ? i : k, c = lower ? k : i;  // lower-form coordinates, r >= c wanted
+    if (r < c) continue;
+    T v;
+    if (r < n) v = lower ? convert<T>(A[(long)c * lda + r]) : conjx(convert<T>(A[(long)r * lda + c]));
+    else v = from_real<T>(r == c ? 1.0 : 0.0);
+    elem(r, c) = v;
   }
   if (tid == 0) flag = -1;
   __syncthreads();
   for (int p = 0; p < np; p++) {
     if (wave == 0) {
-      const int bad = leaf_factor_inverse<T>(tile(p, p), LS, Li + p * LEAF * LL);
+      const int bad = leaf_factor_inverse<T, ST>(tile(p, p), LL, lower, n, p * LEAF, A, lda);
       if (bad >= 0 && (tid & 63) == 0) flag = p * LEAF + bad;
     }
     __syncthreads();
@@ -427,8 +339,8 @@ diag_potrf_kernel(int lower, int n, T *A, long lda, T *W, long ldw, int *info, l
     for (int I = p + 1 + wave; I < np; I += NW) {  // L_Ip = A_Ip inv(L_pp)^H
       TileAcc<T> acc;
       acc_zero(acc);
-      tile_mma<true>(acc, tile(I, p), LS, Li + p * LEAF * LL, LL, 1.0);
-      acc_store(acc, tile(I, p), LS);
+      tile_mma<true>(acc, tile(I, p), LL, tile(p, p), LL, 1.0);
+      acc_store(acc, tile(I, p), LL);
     }
     __syncthreads();
     int t = 0;
@@ -436,42 +348,47 @@ diag_potrf_kernel(int lower, int n, T *A, long lda, T *W, long ldw, int *info, l
       for (int I = J; I < np; I++, t++)
         if (t % NW == wave) {  // A_IJ -= L_Ip L_Jp^H
           TileAcc<T> acc;
-          acc_load(acc, tile(I, J), LS);
-          tile_mma<true>(acc, tile(I, p), LS, tile(J, p), LS, -1.0);
-          acc_store(acc, tile(I, J), LS);
+          acc_load(acc, tile(I, J), LL);
+          tile_mma<true>(acc, tile(I, p), LL, tile(J, p), LL, -1.0);
+          acc_store(acc, tile(I, J), LL);
         }
     __syncthreads();
   }
-  store_lower_form<T, NMAX>(S, lower, n, A, lda, false);
+  // off-diagonal tiles of the factor (the diagonal ones are already out)
+  for (int idx = tid; idx < n * n; idx += BLK_THREADS) {
+    const int i = idx % n, k = idx / n;
+    const int r = lower ? i : k, c = lower ? k : i;
+    if (r / LEAF <= c / LEAF) continue;
+    const T v = elem(r, c);
+    if (lower) A[(long)c * lda + r] = convert<ST>(v);
+    else A[(long)r * lda + c] = convert<ST>(conjx(v));
+  }
   __syncthreads();
-  for (int p = np - 1; p >= 0; p--) {
-    // column block p below the diagonal: B_I <- sum_{J = p+1..I} X_IJ B_J,
-    // then B_I <- -B_I inv(L_pp); X_JJ for J > p is already in place
+  for (int p = np - 2; p >= 0; p--) {
+    // column block p below the diagonal: B_I <- sum_{J = p+1..I} X_IJ L_Jp,
+    // then X_Ip <- -B_I inv(L_pp); X_JJ for J > p is already in place
     const int I = p + 1 + wave;  // at most one tile per wave (np <= NW + 1)
     TileAcc<T> acc;
     acc_zero(acc);
     if (I < np)
-      for (int J = p + 1; J <= I; J++) tile_mma<false>(acc, tile(I, J), LS, tile(J, p), LS, 1.0);
+      for (int J = p + 1; J <= I; J++) tile_mma<false>(acc, tile(I, J), LL, tile(J, p), LL, 1.0);
     __syncthreads();
     if (I < np) {
-      acc_store(acc, tile(I, p), LS);
+      acc_store(acc, tile(I, p), LL);
       wave_lds_fence();
       TileAcc<T> acc2;
       acc_zero(acc2);
-      tile_mma<false>(acc2, tile(I, p), LS, Li + p * LEAF * LL, LL, -1.0);
+      tile_mma<false>(acc2, tile(I, p), LL, tile(p, p), LL, -1.0);
       wave_lds_fence();
-      acc_store(acc2, tile(I, p), LS);
-    }
-    if (wave == NW - 1) {  // X_pp = inv(L_pp), dense lower with zeros above
-      const T *src = Li + p * LEAF * LL;
-      for (int e = tid & 63; e < LEAF * LEAF; e += 64) {
-        const int r = e % LEAF, c = e / LEAF;
-        tile(p, p)[c * LS + r] = src[c * LL + r];
-      }
+      acc_store(acc2, tile(I, p), LL);
     }
     __syncthreads();
   }
-  store_dense<T, NMAX>(S, false, n, false, W, ldw);
+  // W = inv(L) dense, zeros above the diagonal
+  for (int idx = tid; idx < n * n; idx += BLK_THREADS) {
+    const int r = idx % n, c = idx / n;
+    W[(long)c * ldw + r] = convert<ST>(r >= c ? elem(r, c) : from_real<T>(0.0));
+  }
 }
 
 template <typename T, int NMAX>
@@ -506,9 +423,9 @@ __global__ void __launch_bounds__(LAUUM_THREADS) diag_lauum_kernel(int lower, in
 }
 
 // Block b inverts the nb x nb diagonal block of op(A) starting at b * nb.
-template <typename T, int NMAX>
+template <typename T, typename ST, int NMAX>
 __global__ void __launch_bounds__(DIAG_THREADS)
-trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const T *A, long lda, T *W) {
+trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const ST *A, long lda, ST *W) {
   constexpr int LS = Lds<T, NMAX>::LS;
   __shared__ T S[NMAX * (NMAX + 1)];
   __shared__ T vec[2 * NMAX];
@@ -519,7 +436,7 @@ trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const T *A, lon
   // conj(op(A)(c, r)); op(A)(i, j) = A(i, j), A(j, i) or conj(A(j, i)).
   for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
     const int a = idx % n, b = idx / n;  // memory element A(off + a, off + b)
-    const T v = A[(long)(off + b) * lda + off + a];
+    const T v = convert<T>(A[(long)(off + b) * lda + off + a]);
     // as an element of op(A): (a, b) for N, (b, a) for T/C
     int r = (op == OP_N) ? a : b, c = (op == OP_N) ? b : a;
     T e = (op == OP_C) ? conjx(v) : v;
@@ -534,7 +451,7 @@ trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const T *A, lon
   __syncthreads();
   reg_to_lds<T, NMAX>(x, S, n);
   __syncthreads();
-  store_dense<T, NMAX>(S, !lowerlike, n, unit != 0, W + (long)blockIdx.x * nb * nb, nb);
+  store_dense<T, NMAX, ST>(S, !lowerlike, n, unit != 0, W + (long)blockIdx.x * nb * nb, nb);
 }
 
 // One workgroup scans the diagonal; info = first (upper) or last (lower)
@@ -611,13 +528,18 @@ __global__ void __launch_bounds__(256) logdet_final_kernel(const double *partial
 
 }  // namespace
 
-template <typename T>
-hipError_t diag_potrf(hipStream_t s, char uplo, int n, T *A, long lda, T *W, long ldw, int *info,
+// Compute type of the diagonal-block kernels for a storage type.
+template <typename ST> struct Compute { using type = ST; };
+template <> struct Compute<float> { using type = double; };
+
+template <typename ST>
+hipError_t diag_potrf(hipStream_t s, char uplo, int n, ST *A, long lda, ST *W, long ldw, int *info,
                       long info_off) {
+  using T = typename Compute<ST>::type;
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_potrf_kernel<T, NMAX>), dim3(1), dim3(BLK_THREADS), 0, s,
+  hipLaunchKernelGGL((diag_potrf_kernel<T, ST, NMAX>), dim3(1), dim3(BLK_THREADS), 0, s,
                      uplo == 'L' ? 1 : 0, n, A, lda, W, ldw, info, info_off);
   return hipGetLastError();
 }
@@ -644,16 +566,17 @@ hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda) {
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename ST>
 hipError_t trsm_diag_inverses(hipStream_t s, char uplo, char trans, char diag, long m, int nb,
-                              const T *A, long lda, T *W) {
+                              const ST *A, long lda, ST *W) {
+  using T = typename Compute<ST>::type;
   constexpr int NMAX = diag_max<T>();
   if (m <= 0) return hipSuccess;
   if (nb > NMAX) return hipErrorInvalidValue;
   const int op = is_complex<T>::value ? op_code(trans) : (trans == 'N' ? OP_N : OP_T);
   const int lowerlike = ((uplo == 'L') == (trans == 'N')) ? 1 : 0;
   const unsigned nblk = (unsigned)((m + nb - 1) / nb);
-  hipLaunchKernelGGL((trsm_inv_kernel<T, NMAX>), dim3(nblk), dim3(DIAG_THREADS), 0, s, lowerlike,
+  hipLaunchKernelGGL((trsm_inv_kernel<T, ST, NMAX>), dim3(nblk), dim3(DIAG_THREADS), 0, s, lowerlike,
                      op, diag == 'U' ? 1 : 0, m, nb, A, lda, W);
   return hipGetLastError();
 }
@@ -689,5 +612,10 @@ hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials
 HC_INST(double)
 HC_INST(zd)
 #undef HC_INST
+// FP32 storage, FP64 computation (mixed-precision factorisation and solves)
+template hipError_t diag_potrf<float>(hipStream_t, char, int, float *, long, float *, long, int *,
+                                      long);
+template hipError_t trsm_diag_inverses<float>(hipStream_t, char, char, char, long, int,
+                                              const float *, long, float *);
 
 }  // namespace hc

@@ -56,11 +56,12 @@ __device__ __forceinline__ T fetch(const T *X, long ld, int r, int c) {
 }
 
 // Element (r, c) of a triangular operand: zero outside the triangle (selected,
-// not multiplied, so garbage there never propagates), one on a unit diagonal.
+// not multiplied, so garbage there never propagates), one on a unit diagonal
+// (unit == 1) or zero on an excluded one (unit == 2, a strict triangle).
 template <typename T>
 __device__ __forceinline__ T tri_mask(T v, int tri, int unit, int r, int c) {
   if ((tri == TRI_LOWER && c > r) || (tri == TRI_UPPER && c < r)) return from_real<T>(0.0);
-  if (unit && r == c) return from_real<T>(1.0);
+  if (unit && r == c) return from_real<T>(unit == 1 ? 1.0 : 0.0);
   return v;
 }
 
@@ -86,7 +87,19 @@ struct LoadPattern {
   __device__ static int k0(int tid) { return R_CONTIG ? tid / R : tid % KT; }
 };
 
-template <typename T, int TA, int TB, int MT, int NT>
+// TG: the grid enumerates only the tiles meeting the tri_out triangle (the
+// SYRK / HERK launch shape); otherwise it is the full (M / MT) x (N / NT) grid.
+constexpr int NXCD = 8;   // MI355X: 8 accelerator complex dies of 32 CUs
+constexpr int GROUP = 8;  // tile rows per traversal group
+
+// Logical index of workgroup w of nwg when workgroup w runs on XCD w % 8:
+// XCD x gets the contiguous logical range of its own workgroups.
+__device__ __forceinline__ int xcd_contiguous(int w, int nwg) {
+  const int x = w % NXCD, i = w / NXCD, base = nwg / NXCD, rem = nwg % NXCD;
+  return x < rem ? x * (base + 1) + i : rem * (base + 1) + (x - rem) * base + i;
+}
+
+template <typename T, int TA, int TB, int MT, int NT, bool TG>
 __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
 
@@ -95,7 +108,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   constexpr int LSA = MT + PAD, LSB = NT + PAD;
   constexpr int PSA = KT * LSA, PSB = KT * LSB;  // plane strides (doubles)
   constexpr int BUFA = P * PSA, BUFB = P * PSB;  // one LDS buffer
-  constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
+  // 2 x 2 waves; a 16-wide tile (skinny right-hand sides) uses 4 x 1
+  constexpr int WAVES_N = NT >= 32 ? 2 : 1, WAVES_M = 4 / WAVES_N;
+  constexpr int WM = MT / WAVES_M, WN = NT / WAVES_N, BI = WM / 16, BJ = WN / 16;
   using PA = LoadPattern<MT, TA == OP_N>;
   using PB = LoadPattern<NT, TB != OP_N>;
   constexpr int EA = PA::E, EB = PB::E;
@@ -105,17 +120,44 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   __shared__ double sB[2 * BUFB];
 
   // ---- tile coordinates ----
+  // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2):
+  // renumber so that every XCD owns a contiguous range of logical tiles, and
+  // walk tiles in groups of GROUP tile rows, column by column inside a group,
+  // so the tiles in flight on one XCD share their op(A) and op(B) panels.
+  const int nwg = TG ? (int)gridDim.x : (int)(gridDim.x * gridDim.y);
+  const int wid = TG ? (int)blockIdx.x : (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  const int logical = xcd_contiguous(wid, nwg);
   int bm, bn;
-  if (p.tri_grid) {
-    const int id = blockIdx.x;
+  if constexpr (TG) {
+    // row-major position of the logical tile in the lower triangle of tiles
+    const int id = logical;
     int r = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
     while ((r + 1) * (r + 2) / 2 <= id) r++;
     while (r * (r + 1) / 2 > id) r--;
-    const int c = id - r * (r + 1) / 2;
-    if (p.tri_out == TRI_LOWER) { bm = r; bn = c; } else { bm = c; bn = r; }
+    // rows first .. first + gs - 1 form one group (same tile set as row-major)
+    const int nt = (int)((sqrt(8.0 * (double)nwg + 1.0) - 1.0) * 0.5 + 0.5);  // tile rows
+    const int first = (r / GROUP) * GROUP, gs = min(GROUP, nt - first);
+    int o = id - first * (first + 1) / 2, tr, tc;
+    if (o < gs * first) {  // the rectangular part left of the group's diagonal
+      tc = o / gs;
+      tr = first + o % gs;
+    } else {                // the group's own triangle, column by column
+      o -= gs * first;
+      int cl = 0;
+      while (o >= gs - cl) {
+        o -= gs - cl;
+        cl++;
+      }
+      tc = first + cl;
+      tr = tc + o;
+    }
+    if (p.tri_out == TRI_LOWER) { bm = tr; bn = tc; } else { bm = tc; bn = tr; }
   } else {
-    bm = blockIdx.x;
-    bn = blockIdx.y;
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int per = GROUP * gy, first = (logical / per) * GROUP, gs = min(GROUP, gx - first);
+    const int o = logical % per;
+    bm = first + o % gs;
+    bn = o / gs;
   }
   const int m0 = bm * MT, n0 = bn * NT;
 
@@ -130,7 +172,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lc = lane & 15, lg = lane >> 4;
-  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
 
   v4d acc[P][BI][BJ];
 #pragma unroll
@@ -272,14 +314,16 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
 
 template <typename T, int TA, int TB, int MT, int NT>
 hipError_t launch(hipStream_t s, const KParams<T> &p) {
-  dim3 grid;
-  if (p.tri_grid) {
-    const long t = (p.m + MT - 1) / MT;
-    grid = dim3((unsigned)(t * (t + 1) / 2));
-  } else {
-    grid = dim3((unsigned)((p.m + MT - 1) / MT), (unsigned)((p.n + NT - 1) / NT));
-  }
-  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, NT>), grid, dim3(NTHREADS), 0, s, p);
+  const dim3 grid((unsigned)((p.m + MT - 1) / MT), (unsigned)((p.n + NT - 1) / NT));
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, NT, false>), grid, dim3(NTHREADS), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, int TA, int TB, int MT>
+hipError_t launch_tri(hipStream_t s, const KParams<T> &p) {
+  const long t = (p.m + MT - 1) / MT;
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, MT, true>), dim3((unsigned)(t * (t + 1) / 2)),
+                     dim3(NTHREADS), 0, s, p);
   return hipGetLastError();
 }
 
@@ -287,6 +331,17 @@ template <typename T, int TA, int TB>
 hipError_t dispatch_tile(hipStream_t s, const KParams<T> &p, int mt, int nt) {
   constexpr bool CPLX = is_complex<T>::value;
   constexpr int BIG = CPLX ? 64 : 128, SMALL = CPLX ? 32 : 64;
+  if (p.tri_grid) {
+    // SYRK / HERK: op(B) = op(A)^H, so only (N, T/C) and (T/C, N) occur
+    if constexpr ((TA == OP_N) != (TB == OP_N)) {
+      if (mt == BIG) return launch_tri<T, TA, TB, BIG>(s, p);
+      return launch_tri<T, TA, TB, SMALL>(s, p);
+    }
+    return hipErrorInvalidValue;
+  }
+  if constexpr (!CPLX && TB == OP_N) {
+    if (nt == 16) return launch<T, TA, TB, 128, 16>(s, p);  // few right-hand sides
+  }
   if (mt == BIG && nt == BIG) return launch<T, TA, TB, BIG, BIG>(s, p);
   if (mt == SMALL && nt == SMALL) return launch<T, TA, TB, SMALL, SMALL>(s, p);
   if (mt == SMALL && nt == BIG) return launch<T, TA, TB, SMALL, BIG>(s, p);
@@ -332,7 +387,7 @@ hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, 
   p.alpha = alpha; p.beta = beta;
   p.A = A; p.lda = lda; p.B = B; p.ldb = ldb; p.C = C; p.ldc = ldc; p.D = D; p.ldd = ldd;
   p.tri_out = o.tri_out; p.tri_grid = o.tri_grid ? 1 : 0; p.real_diag = o.real_diag ? 1 : 0;
-  p.tri_a = o.tri_a; p.tri_b = o.tri_b; p.unit = o.unit ? 1 : 0;
+  p.tri_a = o.tri_a; p.tri_b = o.tri_b; p.unit = o.unit ? 1 : (o.zero_diag ? 2 : 0);
   p.abort_flag = o.abort_flag;
 
   // Tile choice: the large tile when it still yields about one workgroup per
@@ -343,6 +398,12 @@ hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, 
   const long tb_m = (m + BIG - 1) / BIG, tb_n = (n + BIG - 1) / BIG;
   const long tiles_big = o.tri_grid ? tb_m * (tb_m + 1) / 2 : tb_m * tb_n;
   int mt = tiles_big >= 240 ? BIG : SMALL, nt = mt;
+  // real data with at most 16 columns and plenty of row tiles: 128 x 16 tiles
+  if (!CPLX && tb == OP_N && n <= 16 && o.tri_b == TRI_NONE && o.min_nt <= 16 && !o.tri_grid &&
+      (m + 127) / 128 >= 64) {
+    mt = 128;
+    nt = 16;
+  }
   if (o.min_mt > mt) mt = BIG;
   if (o.min_nt > nt) nt = BIG;
   if (o.min_mt > BIG || o.min_nt > BIG) return hipErrorInvalidValue;

@@ -1,0 +1,307 @@
+// Tile GEMM on CDNA4 matrix cores for FP32 storage (the mixed-precision
+// factorisation and its triangular solves).
+//
+// D <- alpha * op(A) * op(B) + beta * C, column-major FP32, op in {N, T}, FP32
+// accumulation.  The products run either in FP32 (v_mfma_f32_16x16x4_f32) or,
+// with the operands rounded to bfloat16 (round to nearest even) as they are
+// staged, in BF16 (v_mfma_f32_16x16x32_bf16).
+//
+// A 256-thread workgroup (4 waves, 2 x 2) owns an MT x NT tile of D and walks K
+// in steps of 32.  Both operands are staged in LDS row-major with K
+// contiguous (op(A) as MT rows, op(B) as NT columns), so every lane fetches
+// its MFMA fragment with one 16-byte LDS read: 4 consecutive k (FP32) or 8
+// (BF16).  For the FP32 instruction, lane group g supplies k = 4 g + t to the
+// t-th of four 4-deep products, a permutation of the 16 k values that A and B
+// share, so the sum is unchanged.  The op(B) fragment is the first operand, so
+// the 16 x 16 result comes out with D's row index across lanes 0..15 (lane
+// l, register r: row l % 16, column 4 (l / 16) + r) and stores write 64-byte
+// column segments.  Tiles are walked XCD-contiguously in groups of 8 tile rows
+// (see gemm.hip).
+#include "internal.h"
+
+namespace hc {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int KT = 32;
+constexpr int NTHREADS = 256;
+constexpr int NXCD = 8;
+constexpr int GROUP = 8;
+
+enum { MODE_F32 = 0, MODE_BF16 = 1 };
+
+template <int MODE> struct Stage;
+template <> struct Stage<MODE_F32> {
+  using type = float;
+  static constexpr int LK = KT + 4;  // row stride (elements): 144 bytes
+};
+template <> struct Stage<MODE_BF16> {
+  using type = __bf16;
+  static constexpr int LK = KT + 8;  // 80 bytes
+};
+
+struct LpParams {
+  int m, n, k;
+  float alpha, beta;
+  const float *A;
+  long lda;
+  const float *B;
+  long ldb;
+  const float *C;
+  long ldc;
+  float *D;
+  long ldd;
+  int tri_out;
+  const int *abort_flag;
+};
+
+__device__ __forceinline__ int xcd_contiguous(int w, int nwg) {
+  const int x = w % NXCD, i = w / NXCD, base = nwg / NXCD, rem = nwg % NXCD;
+  return x < rem ? x * (base + 1) + i : rem * (base + 1) + (x - rem) * base + i;
+}
+
+// Global -> register pattern for one operand tile of R x KT elements.
+// R_CONTIG: the R index is contiguous in memory (op N for A, op T for B): a
+// thread keeps one R index and KT * R / 256 consecutive k.  Otherwise k is
+// contiguous: 8 lanes cover 32 consecutive k (4 each) of one R index.
+template <int R, bool R_CONTIG>
+struct Pattern {
+  static constexpr int E = R * KT / NTHREADS;  // elements per thread
+  __device__ static int r0(int tid) { return R_CONTIG ? tid % R : tid / 8; }
+  __device__ static int k0(int tid) { return R_CONTIG ? (tid / R) * E : (tid % 8) * 4; }
+  // element q: (r0 + dr(q), k0 + dk(q))
+  __device__ static int dr(int q) { return R_CONTIG ? 0 : (q / 4) * (NTHREADS / 8); }
+  __device__ static int dk(int q) { return R_CONTIG ? q : q % 4; }
+};
+
+template <int MODE, int TA, int TB, int MT, int NT, bool TG>
+__global__ void __launch_bounds__(NTHREADS, 2) gemm_lp_kernel(LpParams p) {
+  if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+  using SE = typename Stage<MODE>::type;
+  constexpr int LK = Stage<MODE>::LK;
+  constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
+  using PA = Pattern<MT, TA == OP_N>;
+  using PB = Pattern<NT, TB != OP_N>;
+  constexpr int EA = PA::E, EB = PB::E;
+  static_assert(EA % 4 == 0 && EB % 4 == 0, "staging is in groups of 4");
+
+  __shared__ __attribute__((aligned(16))) SE sA[2][MT * LK];
+  __shared__ __attribute__((aligned(16))) SE sB[2][NT * LK];
+
+  // ---- tile coordinates (XCD-contiguous, grouped) ----
+  const int nwg = TG ? (int)gridDim.x : (int)(gridDim.x * gridDim.y);
+  const int wid = TG ? (int)blockIdx.x : (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  const int logical = xcd_contiguous(wid, nwg);
+  int bm, bn;
+  if constexpr (TG) {
+    const int id = logical;
+    int r = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= id) r++;
+    while (r * (r + 1) / 2 > id) r--;
+    const int nt = (int)((sqrt(8.0 * (double)nwg + 1.0) - 1.0) * 0.5 + 0.5);
+    const int first = (r / GROUP) * GROUP, gs = min(GROUP, nt - first);
+    int o = id - first * (first + 1) / 2, tr, tc;
+    if (o < gs * first) {
+      tc = o / gs;
+      tr = first + o % gs;
+    } else {
+      o -= gs * first;
+      int cl = 0;
+      while (o >= gs - cl) {
+        o -= gs - cl;
+        cl++;
+      }
+      tc = first + cl;
+      tr = tc + o;
+    }
+    if (p.tri_out == TRI_LOWER) { bm = tr; bn = tc; } else { bm = tc; bn = tr; }
+  } else {
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int per = GROUP * gy, first = (logical / per) * GROUP, gs = min(GROUP, gx - first);
+    const int o = logical % per;
+    bm = first + o % gs;
+    bn = o / gs;
+  }
+  const int m0 = bm * MT, n0 = bn * NT;
+  const int kend = p.alpha == 0.0f ? 0 : p.k;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+
+  v4f acc[BI][BJ];
+#pragma unroll
+  for (int i = 0; i < BI; i++)
+#pragma unroll
+    for (int j = 0; j < BJ; j++) acc[i][j] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+
+  if (kend > 0) {
+    const int ar = m0 + PA::r0(tid), ak = PA::k0(tid);
+    const int br = n0 + PB::r0(tid), bk = PB::k0(tid);
+    float ra[EA], rb[EB];
+    auto load = [&](int kb) {
+#pragma unroll
+      for (int q = 0; q < EA; q++) {
+        const int gi = ar + PA::dr(q), gk = kb + ak + PA::dk(q);
+        const bool ok = gi < p.m && gk < kend;
+        const long off = (TA == OP_N) ? (long)gk * p.lda + gi : (long)gi * p.lda + gk;
+        const float v = p.A[ok ? off : 0];
+        ra[q] = ok ? v : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        const int gj = br + PB::dr(q), gk = kb + bk + PB::dk(q);
+        const bool ok = gj < p.n && gk < kend;
+        const long off = (TB == OP_N) ? (long)gj * p.ldb + gk : (long)gk * p.ldb + gj;
+        const float v = p.B[ok ? off : 0];
+        rb[q] = ok ? v : 0.0f;
+      }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < EA; q += 4) {
+        SE *d = &sA[buf][(PA::r0(tid) + PA::dr(q)) * LK + ak + PA::dk(q)];
+#pragma unroll
+        for (int e = 0; e < 4; e++) d[e] = (SE)ra[q + e];
+      }
+#pragma unroll
+      for (int q = 0; q < EB; q += 4) {
+        SE *d = &sB[buf][(PB::r0(tid) + PB::dr(q)) * LK + bk + PB::dk(q)];
+#pragma unroll
+        for (int e = 0; e < 4; e++) d[e] = (SE)rb[q + e];
+      }
+    };
+
+    const int ntiles = (kend + KT - 1) / KT;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < ntiles; kt++) {
+      const bool more = kt + 1 < ntiles;
+      if (more) load((kt + 1) * KT);
+      const SE *cA = sA[kt & 1], *cB = sB[kt & 1];
+      if constexpr (MODE == MODE_F32) {
+#pragma unroll
+        for (int h = 0; h < KT; h += 16) {
+          v4f fa[BI], fb[BJ];
+#pragma unroll
+          for (int i = 0; i < BI; i++)
+            fa[i] = *reinterpret_cast<const v4f *>(&cA[(wm0 + 16 * i + lc) * LK + h + 4 * lg]);
+#pragma unroll
+          for (int j = 0; j < BJ; j++)
+            fb[j] = *reinterpret_cast<const v4f *>(&cB[(wn0 + 16 * j + lc) * LK + h + 4 * lg]);
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < BI; i++)
+#pragma unroll
+              for (int j = 0; j < BJ; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j][t], fa[i][t], acc[i][j], 0, 0, 0);
+        }
+      } else {
+        v8bf fa[BI], fb[BJ];
+#pragma unroll
+        for (int i = 0; i < BI; i++)
+          fa[i] = *reinterpret_cast<const v8bf *>(&cA[(wm0 + 16 * i + lc) * LK + 8 * lg]);
+#pragma unroll
+        for (int j = 0; j < BJ; j++)
+          fb[j] = *reinterpret_cast<const v8bf *>(&cB[(wn0 + 16 * j + lc) * LK + 8 * lg]);
+#pragma unroll
+        for (int i = 0; i < BI; i++)
+#pragma unroll
+          for (int j = 0; j < BJ; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+      }
+      if (more) store((kt + 1) & 1);
+      __syncthreads();
+    }
+  }
+
+  // ---- epilogue: lane lc holds row m0 + wm0 + 16 i + lc; register r of lane
+  // group lg holds column n0 + wn0 + 16 j + 4 lg + r ----
+  const bool use_beta = p.beta != 0.0f;
+#pragma unroll
+  for (int i = 0; i < BI; i++) {
+    const int gm = m0 + wm0 + 16 * i + lc;
+    if (gm >= p.m) continue;
+#pragma unroll
+    for (int j = 0; j < BJ; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gn = n0 + wn0 + 16 * j + 4 * lg + r;
+        if (gn >= p.n) continue;
+        if (p.tri_out == TRI_LOWER && gm < gn) continue;
+        if (p.tri_out == TRI_UPPER && gm > gn) continue;
+        float v = p.alpha * acc[i][j][r];
+        if (use_beta) v += p.beta * p.C[(long)gn * p.ldc + gm];
+        p.D[(long)gn * p.ldd + gm] = v;
+      }
+    }
+  }
+}
+
+template <int MODE, int TA, int TB, int MT, int NT>
+hipError_t launch(hipStream_t s, const LpParams &p, bool tri_grid) {
+  if (tri_grid) {
+    if constexpr (MT == NT) {
+      const long t = (p.m + MT - 1) / MT;
+      hipLaunchKernelGGL((gemm_lp_kernel<MODE, TA, TB, MT, NT, true>),
+                         dim3((unsigned)(t * (t + 1) / 2)), dim3(NTHREADS), 0, s, p);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
+  const dim3 grid((unsigned)((p.m + MT - 1) / MT), (unsigned)((p.n + NT - 1) / NT));
+  hipLaunchKernelGGL((gemm_lp_kernel<MODE, TA, TB, MT, NT, false>), grid, dim3(NTHREADS), 0, s, p);
+  return hipGetLastError();
+}
+
+template <int MODE, int TA, int TB>
+hipError_t dispatch_tile(hipStream_t s, const LpParams &p, int mt, int nt, bool tri_grid) {
+  if (mt == 128 && nt == 128) return launch<MODE, TA, TB, 128, 128>(s, p, tri_grid);
+  if (mt == 64 && nt == 64) return launch<MODE, TA, TB, 64, 64>(s, p, tri_grid);
+  if (mt == 64 && nt == 128) return launch<MODE, TA, TB, 64, 128>(s, p, tri_grid);
+  return launch<MODE, TA, TB, 128, 64>(s, p, tri_grid);
+}
+
+template <int MODE>
+hipError_t dispatch_ops(hipStream_t s, int ta, int tb, const LpParams &p, int mt, int nt,
+                        bool tri_grid) {
+  const bool a = ta != OP_N, b = tb != OP_N;
+  if (!a && !b) return dispatch_tile<MODE, OP_N, OP_N>(s, p, mt, nt, tri_grid);
+  if (!a && b) return dispatch_tile<MODE, OP_N, OP_T>(s, p, mt, nt, tri_grid);
+  if (a && !b) return dispatch_tile<MODE, OP_T, OP_N>(s, p, mt, nt, tri_grid);
+  return dispatch_tile<MODE, OP_T, OP_T>(s, p, mt, nt, tri_grid);
+}
+
+}  // namespace
+
+template <>
+hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, float alpha,
+                       const float *A, long lda, const float *B, long ldb, float beta,
+                       const float *C, long ldc, float *D, long ldd, const GemmOpts &o) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (o.tri_a != TRI_NONE || o.tri_b != TRI_NONE) return hipErrorInvalidValue;
+  LpParams p;
+  p.m = (int)m; p.n = (int)n; p.k = (int)(k > 0 ? k : 0);
+  p.alpha = alpha; p.beta = beta;
+  p.A = A; p.lda = lda; p.B = B; p.ldb = ldb; p.C = C; p.ldc = ldc; p.D = D; p.ldd = ldd;
+  p.tri_out = o.tri_out;
+  p.abort_flag = o.abort_flag;
+  const long tb_m = (m + 127) / 128, tb_n = (n + 127) / 128;
+  const long tiles_big = o.tri_grid ? tb_m * (tb_m + 1) / 2 : tb_m * tb_n;
+  int mt = tiles_big >= 240 ? 128 : 64, nt = mt;
+  if (o.min_mt > mt) mt = 128;
+  if (o.min_nt > nt) nt = 128;
+  if (o.min_mt > 128 || o.min_nt > 128) return hipErrorInvalidValue;
+  if (o.tri_grid) {
+    if (m != n) return hipErrorInvalidValue;
+    mt = nt = (mt > nt ? mt : nt);
+  }
+  if (o.bf16) return dispatch_ops<MODE_BF16>(s, ta, tb, p, mt, nt, o.tri_grid);
+  return dispatch_ops<MODE_F32>(s, ta, tb, p, mt, nt, o.tri_grid);
+}
+
+}  // namespace hc

@@ -57,6 +57,9 @@ CUresult cuLAPACKDestroy(CULAPACKhandle handle) {
   HC_CHECK(hipStreamSynchronize(handle->stream));
   HC_CHECK(hipStreamSynchronize(handle->side));
   if (handle->ws != nullptr) HC_CHECK(hipFree(handle->ws));
+  for (long i = 0; i < 2 * handle->tcap; i++) HC_CHECK(hipEventDestroy(handle->tev[i]));
+  std::free(handle->tev);
+  std::free(handle->tflops);
   HC_CHECK(hipFree(handle->dinfo));
   HC_CHECK(hipHostFree(handle->hinfo));
   HC_CHECK(hipEventDestroy(handle->ev_panel));
@@ -65,6 +68,34 @@ CUresult cuLAPACKDestroy(CULAPACKhandle handle) {
   HC_CHECK(hipStreamDestroy(handle->side));
   cuBLASDestroy(handle->blas);
   std::free(handle);
+  return CUDA_SUCCESS;
+}
+
+// Timing of the trailing updates inside cuDpotrf / cuZpotrf (an extension
+// with no counterpart in the reference; used by the benchmark).
+CUresult cuLAPACKSetTiming(CULAPACKhandle handle, int enable) {
+  if (handle == nullptr) return hc::bad_arg("cuLAPACKSetTiming", 1);
+  handle->timing = enable != 0;
+  handle->tcount = 0;
+  return CUDA_SUCCESS;
+}
+
+CUresult cuLAPACKGetTiming(CULAPACKhandle handle, double *ms, double *flops, long capacity,
+                           long *count) {
+  if (handle == nullptr) return hc::bad_arg("cuLAPACKGetTiming", 1);
+  if (capacity < 0) return hc::bad_arg("cuLAPACKGetTiming", 4);
+  if (count == nullptr) return hc::bad_arg("cuLAPACKGetTiming", 5);
+  hc::DeviceGuard g(handle->device);
+  HC_CHECK(hipStreamSynchronize(handle->stream));
+  const long n = handle->tcount < capacity ? handle->tcount : capacity;
+  for (long i = 0; i < n; i++) {
+    float t = 0.0f;
+    HC_CHECK(hipEventElapsedTime(&t, handle->tev[2 * i], handle->tev[2 * i + 1]));
+    if (ms != nullptr) ms[i] = t;
+    if (flops != nullptr) flops[i] = handle->tflops[i];
+  }
+  *count = handle->tcount;
+  handle->tcount = 0;
   return CUDA_SUCCESS;
 }
 

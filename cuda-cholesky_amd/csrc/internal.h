@@ -52,6 +52,9 @@ __host__ __device__ inline zd operator/(zd a, zd b) {
   const double r = b.x / b.y, d = b.y + b.x * r;
   return make_zd((a.x * r + a.y) / d, (a.y * r - a.x) / d);
 }
+__host__ __device__ inline float conjx(float a) { return a; }
+__host__ __device__ inline double realx(float a) { return a; }
+__host__ __device__ inline bool is_zero(float a) { return a == 0.0f; }
 __host__ __device__ inline bool is_zero(double a) { return a == 0.0; }
 __host__ __device__ inline bool is_zero(zd a) { return a.x == 0.0 && a.y == 0.0; }
 __host__ __device__ inline bool is_one(double a) { return a == 1.0; }
@@ -60,6 +63,12 @@ __host__ __device__ inline bool is_one(zd a) { return a.x == 1.0 && a.y == 0.0; 
 template <typename T> __host__ __device__ inline T from_real(double v);
 template <> __host__ __device__ inline double from_real<double>(double v) { return v; }
 template <> __host__ __device__ inline zd from_real<zd>(double v) { return make_zd(v, 0.0); }
+template <> __host__ __device__ inline float from_real<float>(double v) { return (float)v; }
+
+// Storage <-> compute conversion (the mixed-precision path keeps FP32 data
+// and computes the diagonal blocks in FP64).
+template <typename D, typename S> __host__ __device__ inline D convert(S v) { return (D)v; }
+template <> __host__ __device__ inline zd convert<zd, zd>(zd v) { return v; }
 
 template <typename T> struct is_complex { static constexpr bool value = false; };
 template <> struct is_complex<zd> { static constexpr bool value = true; };
@@ -84,15 +93,22 @@ struct GemmOpts {
   int tri_a = TRI_NONE;     // op(A) is triangular (lower- or upper-like): TRMM left
   int tri_b = TRI_NONE;     // op(B) is triangular: TRMM right
   bool unit = false;        // the triangular operand has an implicit unit diagonal
+  bool zero_diag = false;   // ... or its diagonal is excluded (strict triangle)
   int min_mt = 0;           // the tile must span at least this many rows of D
   int min_nt = 0;           // ... or columns of D (in-place safety)
   const int *abort_flag = nullptr;  // skip all work when *abort_flag != 0
+  bool bf16 = false;        // float data only: products in BF16 (FP32 accumulate)
 };
 
 template <typename T>
 hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
                 long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
                 const GemmOpts &o = GemmOpts());
+// FP32 storage (gemm_lp.hip): op in {N, T}; no triangular operand masks.
+template <>
+hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, float alpha,
+                       const float *A, long lda, const float *B, long ldb, float beta,
+                       const float *C, long ldc, float *D, long ldd, const GemmOpts &o);
 
 // 2-D copy D <- S (m x n)
 template <typename T>
@@ -122,7 +138,7 @@ hipError_t trsm(hipStream_t s, char side, char uplo, char trans, char diag, long
 template <typename T>
 hipError_t herk(hipStream_t s, char uplo, char trans, long n, long k, double alpha,
                 const T *A, long lda, double beta, T *C, long ldc,
-                const int *abort_flag = nullptr);
+                const int *abort_flag = nullptr, bool bf16 = false);
 
 // ---------------------------------------------------------------------------
 // diagonal-block kernels (diag.hip); the block is held in LDS
@@ -156,6 +172,44 @@ hipError_t diag_zero_scan(hipStream_t s, char uplo, long n, const T *A, long lda
 template <typename T>
 hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials, double *out);
 constexpr int kLogdetMaxBlocks = 1024;
+
+// ---------------------------------------------------------------------------
+// mixed-precision solver kernels (mixed.hip)
+// ---------------------------------------------------------------------------
+// S (FP32, lower form of A, ld lds) <- A (FP64, triangle uplo); *overflow <- 1
+// if a value exceeds the FP32 range.
+hipError_t to_float_lower(hipStream_t s, char uplo, long n, const double *A, long lda, float *S,
+                          long lds, int *overflow);
+hipError_t d2s(hipStream_t s, long m, long n, const double *X, long ldx, float *Y, long ldy,
+               int *overflow);
+hipError_t s2d(hipStream_t s, long m, long n, const float *Y, long ldy, double *X, long ldx);
+hipError_t add_s2d(hipStream_t s, long m, long n, const float *Y, long ldy, double *X, long ldx);
+// out[j] = max_i |X(i, j)|
+hipError_t colmax(hipStream_t s, long m, long n, const double *X, long ldx, double *out);
+// ||A||_inf of the symmetric matrix stored in triangle uplo (device scalar).
+constexpr long kNormChunk = 1024;
+long sym_norm_ws_doubles(long n);
+hipError_t sym_norm_inf(hipStream_t s, char uplo, long n, const double *A, long lda, double *ws,
+                        double *out);
+
+// ---------------------------------------------------------------------------
+// batched factorisation (batched.hip)
+// ---------------------------------------------------------------------------
+constexpr int kBatchedMaxN = 256;
+constexpr int kBatchedBlocksPerCU = 4;  // persistent workgroups per CU
+// Factor count packed matrices in place (matrix order[t] for t = 0..count-1,
+// n = nvec[i] <= kBatchedMaxN, at element offset offs[i]); info[i] and
+// logdet[i] (may be null) per matrix; `grid` persistent workgroups.
+template <typename ST>
+hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,
+                         const int *order, ST *A, double *logdet, int *info, int grid);
+
+// LAPACK internals shared by the solvers (lapack_api.cpp).
+hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out);
+// Blocked factorisation on the handle's streams; W: diag_max^2 elements.
+// bf16: FP32 data only, trailing updates with BF16 products.
+template <typename T>
+hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16);
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -198,4 +252,10 @@ struct __culapackhandle_st {
   size_t ws_bytes;
   int *dinfo;             // device info word
   int *hinfo;             // pinned host copy
+  // optional timing of the factorisation's trailing updates (event pairs on
+  // the main stream and the algorithmic flops of each launch)
+  int timing;
+  long tcount, tcap;
+  hipEvent_t *tev;
+  double *tflops;
 };

@@ -9,6 +9,8 @@
 // pivot see it and do nothing, and one copy at the end returns it.
 #include "api_util.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 
 using hc::bad_arg;
@@ -40,7 +42,14 @@ template <typename T> constexpr int conj_op() { return hc::is_complex<T>::value 
 // Outer panel width of the right-looking factorisation: the trailing update
 // has inner dimension NB, i.e. NB / 8 flops per byte of the trailing matrix
 // it streams (double), enough to stay matrix-core bound.
-template <typename T> constexpr long potrf_nb() { return hc::is_complex<T>::value ? 128 : 256; }
+template <typename T> long potrf_nb() {
+  static const long nb = [] {
+    const char *e = std::getenv("CUCHOLESKY_POTRF_NB");
+    const long v = e ? std::atol(e) : 0;
+    return v > 0 ? v : (hc::is_complex<T>::value ? 128L : 384L);
+  }();
+  return nb;
+}
 
 // Panel factorisation of columns [j, j + jb) (lower; rows [j, j + jb) for
 // upper), all rows below, in IB-column slices: diagonal block (LDS kernel,
@@ -95,9 +104,39 @@ hipError_t factor_panel(CULAPACKhandle h, hipStream_t s, bool lower, long n, T *
 // on everything beyond; meanwhile the look-ahead stream factors the next
 // panel.  The two streams touch disjoint column ranges, and each panel is
 // joined back into the main stream before its own trailing update.
+// Event pair around one trailing update when the handle's timing is on; the
+// event pool grows on demand (outside any timed work it is already sized).
+hipError_t timing_begin(CULAPACKhandle h, hipStream_t s, double flops) {
+  if (!h->timing) return hipSuccess;
+  if (h->tcount == h->tcap) {
+    const long cap = h->tcap ? 2 * h->tcap : 256;
+    auto *ev = static_cast<hipEvent_t *>(std::realloc(h->tev, 2 * cap * sizeof(hipEvent_t)));
+    if (ev == nullptr) return hipErrorOutOfMemory;
+    h->tev = ev;
+    auto *fl = static_cast<double *>(std::realloc(h->tflops, cap * sizeof(double)));
+    if (fl == nullptr) return hipErrorOutOfMemory;
+    h->tflops = fl;
+    for (long i = 2 * h->tcap; i < 2 * cap; i++) {
+      hipError_t e = hipEventCreate(&h->tev[i]);
+      if (e != hipSuccess) {
+        h->tcap = i / 2;
+        return e;
+      }
+    }
+    h->tcap = cap;
+  }
+  h->tflops[h->tcount] = flops;
+  return hipEventRecord(h->tev[2 * h->tcount], s);
+}
+
+hipError_t timing_end(CULAPACKhandle h, hipStream_t s) {
+  if (!h->timing) return hipSuccess;
+  return hipEventRecord(h->tev[2 * h->tcount++ + 1], s);
+}
+
 template <typename T>
-hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W) {
-  constexpr long NB = potrf_nb<T>();
+hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false) {
+  const long NB = potrf_nb<T>();
   hipStream_t s0 = h->stream, s1 = h->side;
   const bool lower = (uplo == 'L');
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0);
@@ -110,6 +149,7 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     const long jbn = std::min(NB, n - jn);
     hc::GemmOpts u;
     u.abort_flag = h->dinfo;
+    u.bf16 = bf16;
     if (lower) {  // next panel columns: A[jn:, jn:jn+jbn] -= P[jn:] P[jn:jn+jbn]^H
       u.tri_out = hc::TRI_LOWER;
       HC_TRY(hc::gemm<T>(s0, hc::OP_N, conj_op<T>(), n - jn, jbn, jb, mone, at(jn, j), lda, at(jn, j),
@@ -125,10 +165,15 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     HC_TRY(hipEventRecord(h->ev_panel, s1));
     const long t0 = jn + jbn, t = n - t0;   // trailing matrix beyond the next panel
     if (t > 0) {
+      const double flops = (hc::is_complex<T>::value ? 4.0 : 1.0) * (double)t * (double)(t + 1) * (double)jb;
+      HC_TRY(timing_begin(h, s0, flops));
       if (lower)
-        HC_TRY(hc::herk<T>(s0, 'L', 'N', t, jb, -1.0, at(t0, j), lda, 1.0, at(t0, t0), lda, h->dinfo));
+        HC_TRY(hc::herk<T>(s0, 'L', 'N', t, jb, -1.0, at(t0, j), lda, 1.0, at(t0, t0), lda, h->dinfo,
+                           bf16));
       else
-        HC_TRY(hc::herk<T>(s0, 'U', 'C', t, jb, -1.0, at(j, t0), lda, 1.0, at(t0, t0), lda, h->dinfo));
+        HC_TRY(hc::herk<T>(s0, 'U', 'C', t, jb, -1.0, at(j, t0), lda, 1.0, at(t0, t0), lda, h->dinfo,
+                           bf16));
+      HC_TRY(timing_end(h, s0));
     }
     HC_TRY(hipStreamWaitEvent(s0, h->ev_panel, 0));
   }
@@ -335,10 +380,28 @@ CUresult logdet_api(const char *name, CULAPACKhandle h, CUdeviceptr x, size_t in
 
 }  // namespace
 
+namespace hc {
+
+hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out) {
+  return workspace(h, bytes, out);
+}
+
+template <typename T>
+hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16) {
+  return potrf_device<T>(h, uplo, n, A, lda, W, bf16);
+}
+template hipError_t potrf_on_device<double>(CULAPACKhandle, char, long, double *, long, double *, bool);
+template hipError_t potrf_on_device<float>(CULAPACKhandle, char, long, float *, long, float *, bool);
+
+}  // namespace hc
+
 extern "C" {
 
 CUresult cuDpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
   return potrf_api<double>("cuDpotrf", h, uplo, n, A, lda, info);
+}
+CUresult cuSpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
+  return potrf_api<float>("cuSpotrf", h, uplo, n, A, lda, info);
 }
 CUresult cuZpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
   return potrf_api<zd>("cuZpotrf", h, uplo, n, A, lda, info);

@@ -102,6 +102,26 @@ CUresult cuZtrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
                  cuDoubleComplex_t, CUdeviceptr, size_t,
                  CUdeviceptr, size_t, CUstream);
 
+/*
+ * Single precision (FP32 data, FP32 accumulation on the matrix cores):
+ * replaces cuSsyrk, cuSgemm2 and cuStrsm of the reference (include/blas.h:158,
+ * 193, 235; blas/ssyrk.c, blas/sgemm.c, blas/strsm.c).  op(A) in {N, T}.
+ */
+CUresult cuSsyrk(CUBLAShandle, CBlasUplo, CBlasTranspose,
+                 size_t, size_t,
+                 float, CUdeviceptr, size_t,
+                 float, CUdeviceptr, size_t, CUstream);
+#define cuSgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream) \
+        cuSgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc, stream)
+CUresult cuSgemm2(CUBLAShandle, CBlasTranspose, CBlasTranspose,
+                  size_t, size_t, size_t,
+                  float, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  float, CUdeviceptr, size_t, CUdeviceptr, size_t, CUstream);
+CUresult cuStrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                 size_t, size_t,
+                 float, CUdeviceptr, size_t,
+                 CUdeviceptr, size_t, CUstream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,92 @@
+/*
+ * Extensions of the MI355X Cholesky library that have no counterpart in the
+ * reference's include/lapack.h or include/blas.h.  They share its handles,
+ * CUresult codes and CUdeviceptr (byte address) convention.
+ */
+#ifndef CUCHOLESKY_H
+#define CUCHOLESKY_H
+
+#include "lapack.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Event timing of the trailing-matrix updates (SYRK / HERK) that cuDpotrf and
+ * cuZpotrf launch on the handle's main stream.  cuLAPACKSetTiming(h, 1)
+ * clears the record and starts recording; 0 stops.  cuLAPACKGetTiming waits
+ * for the handle's main stream, copies up to `capacity` launch durations (ms)
+ * and algorithmic flop counts (real flops: t (t + 1) k for DSYRK, four times
+ * that for ZHERK), stores the number recorded in *count and clears the
+ * record.  Either output array may be NULL.
+ */
+CUresult cuLAPACKSetTiming(CULAPACKhandle handle, int enable);
+CUresult cuLAPACKGetTiming(CULAPACKhandle handle, double * ms, double * flops, long capacity,
+                           long * count);
+
+/*
+ * Solve A X = B with the Cholesky factor from cuDpotrf (A = L L^T or U^T U),
+ * B (n x nrhs) overwritten by X.  Arguments and INFO follow LAPACK DPOTRS
+ * (UPLO = 1, N = 2, NRHS = 3, A = 4, LDA = 5, B = 6, LDB = 7; the handle is
+ * not counted).  Synchronous.
+ */
+CUresult cuDpotrs(CULAPACKhandle handle, CBlasUplo uplo, size_t n, size_t nrhs, CUdeviceptr A,
+                  size_t lda, CUdeviceptr B, size_t ldb, long * info);
+
+/*
+ * Mixed-precision SPD solve A X = B (A FP64, n x n, triangle uplo; B, X
+ * n x nrhs), as LAPACK DSPOSV: A is copied to FP32 and factored there (the
+ * O(n^3) trailing updates on FP32 matrix cores, the diagonal blocks in FP64),
+ * then the FP64 solution is refined with FP64 residuals r = b - A x until
+ * every column has ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n), at most 30
+ * steps.  ITER >= 0: refinement steps taken.  ITER < 0: the low-precision
+ * path failed (-2: a value overflowed FP32, -3: the FP32 factorisation
+ * failed, -31: no convergence in 30 steps); then A is overwritten by its
+ * FP64 Cholesky factor and X comes from cuDpotrf + cuDpotrs, with INFO > 0 if
+ * A is not positive definite.  A is not modified when ITER >= 0.
+ */
+CUresult cuDsposv(CULAPACKhandle handle, CBlasUplo uplo, size_t n, size_t nrhs, CUdeviceptr A,
+                  size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx, long * iter,
+                  long * info);
+
+/*
+ * cuDsposv with the factorisation precision chosen: factor = 'S' (FP32
+ * products, as cuDsposv) or 'B' (trailing-update products in bfloat16, FP32
+ * accumulation).  A 'B' attempt that fails is retried with 'S' before the
+ * FP64 fall-back; *used (may be NULL) reports the precision of the returned
+ * solution: 'B', 'S' or 'D'.
+ */
+CUresult cuDsposvx(CULAPACKhandle handle, CBlasUplo uplo, size_t n, size_t nrhs, CUdeviceptr A,
+                   size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx, char factor,
+                   long * iter, char * used, long * info);
+
+/*
+ * Batched Cholesky factorisation + log-determinant of many SPD matrices of
+ * uneven sizes (n_i <= 256) packed in one device buffer: matrix i is
+ * n_i x n_i, column-major with leading dimension n_i, starting `offsets[i]`
+ * elements into the buffer; only triangle uplo is read and written.
+ * precision: 'D' (double data) or 'S' (float data; arithmetic in FP64).
+ *
+ * cuPotrfBatchedPlanCreate takes the sizes and offsets on the host, sorts the
+ * matrices longest first and sizes one persistent launch.  cuPotrfBatchedRun
+ * factors every matrix in place, writes info[i] (int, 0 or the 1-based
+ * failing column) and logdet[i] (double, log det A_i = 2 sum log L_jj; NaN
+ * on failure; logdet may be 0 to skip), asynchronously on `stream`: one HIP
+ * graph launch (the graph is captured on the first run with a given
+ * (A, logdet, info) and replayed afterwards).  A plan runs one batch at a
+ * time: runs that overlap on different streams race.
+ */
+typedef struct __cubatchedplan_st * CUbatchedplan;
+CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle handle, CBlasUplo uplo, char precision,
+                                  size_t count, const size_t * n, const size_t * offsets,
+                                  CUbatchedplan * plan);
+CUresult cuPotrfBatchedRun(CUbatchedplan plan, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
+                           CUstream stream);
+CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

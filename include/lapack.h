@@ -34,6 +34,9 @@ CUresult cuDpotrf(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *
 /* Hermitian Cholesky A = L L^H or U^H U
  * (replaces cuZpotrf, reference lapack/zpotrf.c:174-298). */
 CUresult cuZpotrf(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* Single precision Cholesky of FP32 data; the diagonal blocks are factored
+ * in FP64 (replaces cuSpotrf, reference lapack/spotrf.c:261-398). */
+CUresult cuSpotrf(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
 
 /* In-place triangular inverse; info > 0 names an exactly zero diagonal
  * (replaces cuDtrtri, reference lapack/dtrtri.c:326-468). */

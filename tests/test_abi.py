@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 def declared_functions():
     names = set()
-    for h in ("blas.h", "lapack.h", "error.h"):
+    for h in sorted(f for f in os.listdir(INCLUDE) if f.endswith(".h")):
         text = open(os.path.join(INCLUDE, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
@@ -45,7 +45,7 @@ def test_library_exports_every_declared_symbol(cc):
 def test_headers_compile_as_c_and_cpp(tmp_path):
     src = tmp_path / "use.c"
     src.write_text(
-        '#include "lapack.h"\n#include "error.h"\n'
+        '#include "lapack.h"\n#include "error.h"\n#include "cucholesky.h"\n'
         "_Static_assert(CBlasLower == 'L' && CBlasUpper == 'U', \"uplo\");\n"
         "_Static_assert(CBlasNoTrans == 'N' && CBlasTrans == 'T' && CBlasConjTrans == 'C', \"t\");\n"
         "_Static_assert(CBlasLeft == 'L' && CBlasRight == 'R', \"side\");\n"
@@ -61,7 +61,7 @@ def test_headers_compile_as_c_and_cpp(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{INCLUDE}", str(src)],
                    check=True)
     cpp = tmp_path / "use.cpp"
-    cpp.write_text('#include "lapack.h"\n#include "error.h"\n'
+    cpp.write_text('#include "lapack.h"\n#include "error.h"\n#include "cucholesky.h"\n'
                    "int g(CULAPACKhandle h) { long i; return cuDpotri(h, CBlasUpper, 0, 0, 1, &i); }\n")
     subprocess.run(["g++", "-std=c++17", "-Wall", "-fsyntax-only", f"-I{INCLUDE}", str(cpp)],
                    check=True)
@@ -155,3 +155,33 @@ def test_error_strings(cc):
     assert cc.error_string(0) == "No errors"
     assert cc.error_string(1) == "Invalid value"
     assert cc.error_string(2) == "Out of memory"
+
+
+def test_single_precision_argument_indices(cc):
+    """The FP32 surface checks arguments like its FP64 twin (same XERBLA indices)."""
+    with cc.Xerbla() as x:
+        assert cc.cuSgemm2(FAKE, cc.NoTrans, cc.NoTrans, 5, 4, 3, 1.0, 0, 4, 0, 3, 1.0, 0, 5, 0,
+                           5) == cc.CUDA_ERROR_INVALID_VALUE
+        assert cc.cuSsyrk(FAKE, cc.Lower, cc.NoTrans, 5, 3, 1.0, 0, 4, 1.0, 0, 5) == 1
+        assert cc.cuStrsm(FAKE, cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, 5, 3, 1.0, 0, 4, 0,
+                          5) == 1
+        st, info = cc.cuSpotrf(FAKE, cc.Lower, 10, 0, 9)
+    assert st == 1 and info == -4
+    assert x.calls == [("cuSgemm2", 8), ("cuSsyrk", 7), ("cuStrsm", 9), ("cuSpotrf", 4)]
+
+
+def test_solver_argument_indices(cc):
+    """cuDpotrs / cuDsposv follow LAPACK DPOTRS / DSPOSV numbering (handle not
+    counted): LDA -> 5, LDB -> 7, LDX -> 9."""
+    with cc.Xerbla() as x:
+        assert cc.cuDpotrs(FAKE, cc.Lower, 10, 2, 0, 9, 0, 10) == (1, -5)
+        assert cc.cuDpotrs(FAKE, cc.Lower, 10, 2, 0, 10, 0, 9) == (1, -7)
+        assert cc.cuDsposv(FAKE, cc.Upper, 10, 2, 0, 10, 0, 10, 0, 9) == (1, 0, -9)
+        st, it, used, info = cc.cuDsposvx(FAKE, cc.Lower, 10, 2, 0, 10, 0, 10, 0, 10, "H")
+        assert (st, info) == (1, -10)
+    assert x.calls == [("cuDpotrs", 5), ("cuDpotrs", 7), ("cuDsposv", 9), ("cuDsposvx", 10)]
+
+
+def test_solver_quick_returns(cc):
+    assert cc.cuDpotrs(FAKE, cc.Lower, 0, 3, 0, 1, 0, 1) == (0, 0)
+    assert cc.cuDsposv(FAKE, cc.Lower, 5, 0, 0, 5, 0, 5, 0, 5) == (0, 0, 0)

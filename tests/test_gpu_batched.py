@@ -1,0 +1,146 @@
+"""GPU parity of the batched factorisation + log-determinant (cuPotrfBatched*).
+
+Each matrix of the batch is checked against the oracle on the same input:
+the factor by backward error ||A - L L^T||_F / ||A||_F <= 10 n eps (eps of
+the storage precision), the log-determinant against 2 sum log L_jj of the
+oracle's own factor (relative 10 n eps), and info exactly (non-SPD matrices
+are planted in the batch).  The unreferenced triangle of every matrix and the
+gaps between matrices must come back bit-identical; replays of the captured
+graph must reproduce the first run bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import EPS
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def make_batch(seed, sizes, gap=3):
+    """Packed column-major matrices with `gap` sentinel elements between them."""
+    offs, pos = [], 0
+    for n in sizes:
+        offs.append(pos)
+        pos += n * n + gap
+    buf = np.full(pos, np.nan)
+    mats = []
+    for i, n in enumerate(sizes):
+        A = O.spd(1000 + seed * 97 + i, n, 0.5) if n > 0 else np.zeros((0, 0))
+        mats.append(A)
+        buf[offs[i]:offs[i] + n * n] = A.ravel(order="F")
+    return buf, offs, mats
+
+
+def triangle_only(buf, offs, sizes, uplo):
+    """NaN in the unreferenced triangle of every matrix (it must survive)."""
+    out = buf.copy()
+    for o, n in zip(offs, sizes):
+        M = out[o:o + n * n].reshape((n, n), order="F")
+        mask = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+        M[mask] = np.nan
+        out[o:o + n * n] = M.ravel(order="F")
+    return out
+
+
+def run(gpu, lapack, uplo, prec, sizes, buf, offs):
+    dtype = torch.float64 if prec == "D" else torch.float32
+    dA = torch.from_numpy(buf).to(dtype).cuda()
+    dlog = torch.zeros(len(sizes), dtype=torch.float64, device="cuda")
+    dinfo = torch.full((len(sizes),), -7, dtype=torch.int32, device="cuda")
+    with gpu.BatchedPlan(lapack, ord(uplo), prec, sizes, offs) as plan:
+        assert plan.status == 0
+        assert plan.run(gpu.ptr(dA), gpu.ptr(dlog), gpu.ptr(dinfo)) == 0
+        torch.cuda.synchronize()
+    return dA.double().cpu().numpy(), dlog.cpu().numpy(), dinfo.cpu().numpy()
+
+
+def check(buf_in, out, logdet, info, offs, sizes, mats, uplo, eps):
+    for i, (o, n) in enumerate(zip(offs, sizes)):
+        assert info[i] == 0, (i, n, info[i])
+        if n == 0:
+            assert logdet[i] == 0.0
+            continue
+        F = out[o:o + n * n].reshape((n, n), order="F")
+        A = mats[i] if eps == EPS else mats[i].astype(np.float32).astype(np.float64)
+        assert O.potrf_backward_error(A, F, uplo) <= 10 * n * eps, (i, n)
+        ref, rinfo = O.dpotrf(uplo, np.asfortranarray(A.copy()))
+        assert rinfo == 0
+        want = 2.0 * math.fsum(np.log(np.diag(ref)))
+        assert abs(logdet[i] - want) <= 10 * n * eps * max(1.0, abs(want)), (i, n, logdet[i], want)
+    # unreferenced triangles and gaps untouched (NaN stays NaN)
+    np.testing.assert_array_equal(np.isnan(out), np.isnan(buf_in))
+
+
+SIZES_EDGE = [1, 2, 15, 16, 17, 31, 32, 33, 100, 127, 128, 129, 200, 255, 256, 0, 8]
+
+
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("prec", "DS")
+def test_batched_edge_sizes(gpu, lapack, uplo, prec):
+    buf, offs, mats = make_batch(1, SIZES_EDGE)
+    buf = triangle_only(buf, offs, SIZES_EDGE, uplo)
+    out, logdet, info = run(gpu, lapack, uplo, prec, SIZES_EDGE, buf, offs)
+    check(buf if prec == "D" else buf.astype(np.float32).astype(np.float64), out, logdet, info, offs,
+          SIZES_EDGE, mats, uplo, EPS if prec == "D" else EPS32)
+
+
+@pytest.mark.parametrize("prec", "DS")
+def test_batched_random_sizes(gpu, lapack, prec):
+    sizes = list(np.random.default_rng(7).integers(8, 257, size=300))
+    buf, offs, mats = make_batch(2, sizes)
+    buf = triangle_only(buf, offs, sizes, "L")
+    out, logdet, info = run(gpu, lapack, "L", prec, sizes, buf, offs)
+    check(buf if prec == "D" else buf.astype(np.float32).astype(np.float64), out, logdet, info, offs,
+          sizes, mats, "L", EPS if prec == "D" else EPS32)
+
+
+def test_batched_info_and_neighbours(gpu, lapack):
+    """Non-SPD matrices report their failing column; their neighbours are unaffected."""
+    sizes = [50, 130, 64, 256, 40]
+    bad = {1: 77, 3: 200}           # matrix -> 0-based failing column
+    buf, offs, mats = make_batch(3, sizes)
+    for i, k in bad.items():
+        n = sizes[i]
+        A = mats[i]
+        L = np.linalg.cholesky(A)
+        A[k, k] = np.dot(L[k, :k], L[k, :k]) - 1.0
+        buf[offs[i]:offs[i] + n * n] = A.ravel(order="F")
+    out, logdet, info = run(gpu, lapack, "L", "D", sizes, buf, offs)
+    for i, n in enumerate(sizes):
+        if i in bad:
+            assert info[i] == bad[i] + 1 and np.isnan(logdet[i])
+        else:
+            assert info[i] == 0
+            F = out[offs[i]:offs[i] + n * n].reshape((n, n), order="F")
+            assert O.potrf_backward_error(mats[i], F, "L") <= 10 * n * EPS
+
+
+def test_batched_graph_replay(gpu, lapack):
+    """The captured graph replays bit-identically; new buffers re-capture."""
+    sizes = [64, 200, 17, 256, 128]
+    buf, offs, _ = make_batch(4, sizes)
+    results = []
+    with gpu.BatchedPlan(lapack, ord("L"), "D", sizes, offs) as plan:
+        dA = torch.from_numpy(buf).cuda()
+        dlog = torch.zeros(len(sizes), dtype=torch.float64, device="cuda")
+        dinfo = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+        for rep in range(3):
+            dA.copy_(torch.from_numpy(buf))          # eager work between replays
+            torch.cuda.synchronize()
+            assert plan.run(gpu.ptr(dA), gpu.ptr(dlog), gpu.ptr(dinfo)) == 0
+            torch.cuda.synchronize()
+            results.append((dA.cpu().numpy().copy(), dlog.cpu().numpy().copy()))
+        dB = torch.from_numpy(buf).cuda()             # other buffers: a new capture
+        dlog2 = torch.zeros_like(dlog)
+        assert plan.run(gpu.ptr(dB), gpu.ptr(dlog2), gpu.ptr(dinfo)) == 0
+        torch.cuda.synchronize()
+        results.append((dB.cpu().numpy().copy(), dlog2.cpu().numpy().copy()))
+    for A, ld in results[1:]:
+        np.testing.assert_array_equal(np.nan_to_num(A), np.nan_to_num(results[0][0]))
+        np.testing.assert_array_equal(ld, results[0][1])
