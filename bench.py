@@ -114,10 +114,11 @@ def roofline_live(records):
 
 
 def mixed_precision(torch, pkg, h, n, nrhs, alphas):
-    """SPD solve A X = B (n x n, nrhs right-hand sides) three ways: FP64
-    cuDpotrf + cuDpotrs, and the mixed-precision cuDsposvx with FP32 and BF16
-    trailing-update products (FP64 diagonal blocks, FP64 residuals, iterative
-    refinement to LAPACK dsposv's stopping test).  Wall time of each whole
+    """SPD solve A X = B (n x n, nrhs right-hand sides) four ways: FP64
+    cuDpotrf + cuDpotrs, and the mixed-precision cuDsposvx with FP32 or BF16
+    trailing-update products (FP64 diagonal blocks, FP64 residuals) refined to
+    LAPACK dsposv's stopping test by classical iterative refinement (FP32,
+    BF16) or GMRES-IR (BF16).  Wall time of each whole
     solve, refinement steps, and the normwise backward error
     max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)."""
     out = {"n": n, "nrhs": nrhs}
@@ -147,19 +148,19 @@ def mixed_precision(torch, pkg, h, n, nrhs, alphas):
         rec["fp64_ms"] = round(t64 * 1e3, 2)
         rec["fp64_backward_error"] = berr(X)
         del F
-        for factor in ("S", "B"):
+        for factor, refine in (("S", "I"), ("B", "I"), ("B", "G")):
             X = torch.zeros_like(B)
             for rep in range(2):       # the first call also grows the workspace
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 st, it, used, info = pkg.cuDsposvx(h, pkg.Lower, n, nrhs, pkg.ptr(A), n, pkg.ptr(B), n,
-                                                   pkg.ptr(X), n, factor)
+                                                   pkg.ptr(X), n, factor, refine)
                 torch.cuda.synchronize()
                 dt = time.perf_counter() - t0
                 assert st == 0 and info == 0, (st, info)
                 if it < 0:             # A now holds its FP64 factor: regenerate
                     A = gpu_spd(torch, n, alpha, seed=3)
-            key = "fp32" if factor == "S" else "bf16"
+            key = {"S": "fp32", "B": "bf16"}[factor] + ("_gmres" if refine == "G" else "")
             rec[key] = {"ms": round(dt * 1e3, 2), "iter": it, "used": used,
                         "backward_error": berr(X), "speedup_vs_fp64": round(t64 / dt, 2)}
         out[f"alpha={alpha:g}"] = rec

@@ -101,13 +101,14 @@ SIGNATURES = {
     # include/cucholesky.h (extensions)
     "cuDpotrs": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _longp],
     "cuDsposv": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, _longp, _longp],
-    "cuDsposvx": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, ctypes.c_char, _longp,
-                  ctypes.c_char_p, _longp],
+    "cuDsposvx": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, ctypes.c_char, ctypes.c_char,
+                  _longp, ctypes.c_char_p, _longp],
     "cuPotrfBatchedPlanCreate": [_h, _enum, ctypes.c_char, _sz, ctypes.POINTER(ctypes.c_size_t),
                                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_h)],
     "cuPotrfBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
     "cuPotrfBatchedPlanDestroy": [_h],
     "cuLAPACKSetTiming": [_h, ctypes.c_int],
+    "cuLAPACKSetBlockSize": [_h, ctypes.c_char, ctypes.c_long],
     "cuLAPACKGetTiming": [_h, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.c_long, _longp],
 }
@@ -202,6 +203,11 @@ def _hv(handle):
     return handle.h if hasattr(handle, "h") else handle
 
 
+def cuLAPACKSetBlockSize(handle, precision, nb):
+    return lib().cuLAPACKSetBlockSize(_hv(handle), precision.encode() if isinstance(precision, str)
+                                      else precision, int(nb))
+
+
 def cuLAPACKSetTiming(handle, enable):
     return lib().cuLAPACKSetTiming(_hv(handle), int(enable))
 
@@ -244,13 +250,13 @@ def cuDsposv(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx):
     return st, it.value, info.value
 
 
-def cuDsposvx(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor="S"):
+def cuDsposvx(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor="S", refine="I"):
     """(status, iter, used precision 'B' / 'S' / 'D', info)"""
     it, info = ctypes.c_long(0), ctypes.c_long(0)
     used = ctypes.create_string_buffer(1)
-    st = lib().cuDsposvx(_hv(handle), uplo, n, nrhs, A, lda, B, ldb, X, ldx,
-                         factor.encode() if isinstance(factor, str) else factor, ctypes.byref(it),
-                         used, ctypes.byref(info))
+    enc = (lambda c: c.encode() if isinstance(c, str) else c)
+    st = lib().cuDsposvx(_hv(handle), uplo, n, nrhs, A, lda, B, ldb, X, ldx, enc(factor), enc(refine),
+                         ctypes.byref(it), used, ctypes.byref(info))
     return st, it.value, used.raw[:1].decode(), info.value
 
 

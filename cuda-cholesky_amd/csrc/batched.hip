@@ -28,6 +28,7 @@ constexpr int BT_THREADS = 256;                // 4 waves per matrix
 constexpr int BT_NW = BT_THREADS / 64;
 constexpr int BT_NPMAX = kBatchedMaxN / LEAF;  // leaves per matrix at most
 constexpr int LL = LEAF + 1;                   // LDS tile leading dimension
+constexpr int TG = 4;                          // trailing tiles loaded together per wave
 
 // Lower-form view of one packed matrix: element (r, c), r >= c, is A(r, c)
 // for lower storage and A(c, r) for upper storage.
@@ -102,23 +103,43 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
         }
       }
       __syncthreads();
-      // trailing update of the lower tiles, operands from LDS
-      int tt = 0;
-      for (int J = p + 1; J < np; J++) {
-        for (int I = J; I < np; I++, tt++) {
-          if (tt % BT_NW != wave) continue;
-          TileAcc<double> acc;
+      // trailing update of the lower tiles, operands from LDS.  Tiles are
+      // dealt round-robin to the waves; each wave loads TG of its tiles
+      // before computing any, so their global-memory latencies overlap.
+      {
+        const int m = np - 1 - p;               // trailing tile rows / columns
+        const int ntile = m * (m + 1) / 2;
+        for (int t0 = wave * TG; t0 < ntile; t0 += BT_NW * TG) {
+          int ti[TG], tj[TG];
+          TileAcc<double> acc[TG];
 #pragma unroll
-          for (int r = 0; r < 4; r++) acc.re[r] = v.get(I * LEAF + lk + 4 * r, J * LEAF + li);
-          const double *PI = P + I * LEAF * LL, *PJ = P + J * LEAF * LL;
+          for (int g = 0; g < TG; g++) {
+            // tile t0 + g of the column-major lower triangle of the m x m tile block
+            int t = t0 + g, J = 0;
+            while (t >= m - J) {
+              t -= m - J;
+              J++;
+            }
+            ti[g] = p + 1 + J + t;
+            tj[g] = p + 1 + J;
+            const bool live = t0 + g < ntile;
 #pragma unroll
-          for (int k0 = 0; k0 < LEAF; k0 += 4) {
-            const double x = -PI[(k0 + lk) * LL + li];
-            const double y = PJ[(k0 + lk) * LL + li];
-            acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+            for (int r = 0; r < 4; r++)
+              acc[g].re[r] = live ? v.get(ti[g] * LEAF + lk + 4 * r, tj[g] * LEAF + li) : 0.0;
           }
 #pragma unroll
-          for (int r = 0; r < 4; r++) v.put(I * LEAF + lk + 4 * r, J * LEAF + li, acc.re[r]);
+          for (int g = 0; g < TG; g++) {
+            if (t0 + g >= ntile) break;
+            const double *PI = P + ti[g] * LEAF * LL, *PJ = P + tj[g] * LEAF * LL;
+#pragma unroll
+            for (int k0 = 0; k0 < LEAF; k0 += 4) {
+              const double x = -PI[(k0 + lk) * LL + li];
+              const double y = PJ[(k0 + lk) * LL + li];
+              acc[g].re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[g].re, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) v.put(ti[g] * LEAF + lk + 4 * r, tj[g] * LEAF + li, acc[g].re[r]);
+          }
         }
       }
       __syncthreads();
@@ -147,6 +168,18 @@ hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, c
   return hipGetLastError();
 }
 
+template <typename ST>
+int potrf_batched_blocks_per_cu(char uplo) {
+  int nb = 0;
+  const hipError_t e =
+      uplo == 'L'
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, true>, BT_THREADS, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, false>, BT_THREADS, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+template int potrf_batched_blocks_per_cu<double>(char);
+template int potrf_batched_blocks_per_cu<float>(char);
 template hipError_t potrf_batched<double>(hipStream_t, char, int, const int *, const long *,
                                           const int *, double *, double *, int *, int);
 template hipError_t potrf_batched<float>(hipStream_t, char, int, const int *, const long *,

@@ -84,10 +84,13 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
   }
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nv[a] > nv[b]; });
-  // persistent grid: every CU filled to the kernel's occupancy
+  // persistent grid: every CU filled to the kernel's occupancy, no more (a
+  // workgroup that waits for a slot would start its share late)
   int cus = 0;
   HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-  p->grid = std::max(1, std::min<int>((int)count, cus * hc::kBatchedBlocksPerCU));
+  const int per_cu = precision == 'D' ? hc::potrf_batched_blocks_per_cu<double>((char)uplo)
+                                      : hc::potrf_batched_blocks_per_cu<float>((char)uplo);
+  p->grid = std::max(1, std::min<int>((int)count, cus * per_cu));
   if (count > 0) {
     HC_CHECK(hipMalloc(&p->d_n, count * sizeof(int)));
     HC_CHECK(hipMalloc(&p->d_off, count * sizeof(long)));

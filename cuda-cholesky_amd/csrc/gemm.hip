@@ -45,6 +45,10 @@ struct KParams {
   long ldd;
   int tri_out, tri_grid, real_diag, tri_a, tri_b, unit;
   const int *abort_flag;
+  // split K: blockIdx.z takes K range [z kchunk, (z + 1) kchunk) and writes
+  // its raw sum to part + z * m * n (m x n column-major, ld m)
+  int kchunk;
+  T *part;
 };
 
 // Element (r, c) of op(X).
@@ -167,6 +171,10 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   else if (p.tri_a == TRI_UPPER) kbeg = m0;
   if (p.tri_b == TRI_LOWER) kbeg = max(kbeg, n0);
   else if (p.tri_b == TRI_UPPER) kend = min(kend, n0 + NT);
+  if (p.part != nullptr) {
+    kbeg = max(kbeg, (int)blockIdx.z * p.kchunk);
+    kend = min(kend, ((int)blockIdx.z + 1) * p.kchunk);
+  }
   kbeg &= ~(KT - 1);
   if (is_zero(p.alpha)) kend = kbeg;
 
@@ -285,6 +293,26 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   // ---- epilogue: lane lc holds row m; register r of lane group lg holds
   // column lg + 4 r (the 16x16x4 f64 result layout: block row lane/16 + 4 r,
   // block column lane % 16, with rows of the block indexing N here) ----
+  if (p.part != nullptr) {  // split K: raw partial sums
+    T *part = p.part + (long)blockIdx.z * p.m * p.n;
+#pragma unroll
+    for (int i = 0; i < BI; i++) {
+      const int gm = m0 + wm0 + 16 * i + lc;
+      if (gm >= p.m) continue;
+#pragma unroll
+      for (int j = 0; j < BJ; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int gn = n0 + wn0 + 16 * j + lg + 4 * r;
+          if (gn >= p.n) continue;
+          T v;
+          if constexpr (CPLX) v = make_zd(acc[0][i][j][r], acc[1][i][j][r]);
+          else v = acc[0][i][j][r];
+          part[(long)gn * p.m + gm] = v;
+        }
+    }
+    return;
+  }
   const bool use_beta = !is_zero(p.beta);
 #pragma unroll
   for (int i = 0; i < BI; i++) {
@@ -312,8 +340,31 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   }
 }
 
+// D = alpha * sum_z part[z] + beta * C (split-K epilogue, fixed order).
+template <typename T>
+__global__ void splitk_reduce_kernel(int m, int n, int nsplit, const T *part, T alpha, T beta,
+                                     const T *C, long ldc, T *D, long ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  T acc = from_real<T>(0.0);
+  for (int z = 0; z < nsplit; z++) acc = acc + part[(long)z * m * n + (long)j * m + i];
+  T v = alpha * acc;
+  if (!is_zero(beta)) v = v + beta * C[(long)j * ldc + i];
+  D[(long)j * ldd + i] = v;
+}
+
 template <typename T, int TA, int TB, int MT, int NT>
 hipError_t launch(hipStream_t s, const KParams<T> &p) {
+  if (p.part != nullptr) {
+    const int nsplit = (p.k + p.kchunk - 1) / p.kchunk;
+    const dim3 grid((unsigned)((p.m + MT - 1) / MT), (unsigned)((p.n + NT - 1) / NT), (unsigned)nsplit);
+    hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, NT, false>), grid, dim3(NTHREADS), 0, s, p);
+    HC_TRY(hipGetLastError());
+    hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)((p.m + 255) / 256), (unsigned)p.n),
+                       dim3(256), 0, s, p.m, p.n, nsplit, (const T *)p.part, p.alpha, p.beta, p.C,
+                       p.ldc, p.D, p.ldd);
+    return hipGetLastError();
+  }
   const dim3 grid((unsigned)((p.m + MT - 1) / MT), (unsigned)((p.n + NT - 1) / NT));
   hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, NT, false>), grid, dim3(NTHREADS), 0, s, p);
   return hipGetLastError();
@@ -389,6 +440,8 @@ hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, 
   p.tri_out = o.tri_out; p.tri_grid = o.tri_grid ? 1 : 0; p.real_diag = o.real_diag ? 1 : 0;
   p.tri_a = o.tri_a; p.tri_b = o.tri_b; p.unit = o.unit ? 1 : (o.zero_diag ? 2 : 0);
   p.abort_flag = o.abort_flag;
+  p.kchunk = 0;
+  p.part = nullptr;
 
   // Tile choice: the large tile when it still yields about one workgroup per
   // CU (256 CUs), otherwise the small one; in-place callers may demand a tile
@@ -403,6 +456,19 @@ hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, 
       (m + 127) / 128 >= 64) {
     mt = 128;
     nt = 16;
+    // long K: split it so about 2048 workgroups share the rows, partial sums
+    // in the caller's workspace (splitk_ws elements), reduced in fixed order
+    if (o.splitk_ws != nullptr && o.tri_out == TRI_NONE && !o.real_diag) {
+      const long row_tiles = (m + 127) / 128;
+      long nsplit = (2048 + row_tiles - 1) / row_tiles;
+      long kchunk = ((k + nsplit - 1) / nsplit + KT - 1) / KT * KT;
+      if (kchunk < 256) kchunk = 256;
+      nsplit = (k + kchunk - 1) / kchunk;
+      if (nsplit > 1 && nsplit * m * n <= o.splitk_elems) {
+        p.kchunk = (int)kchunk;
+        p.part = static_cast<T *>(o.splitk_ws);
+      }
+    }
   }
   if (o.min_mt > mt) mt = BIG;
   if (o.min_nt > nt) nt = BIG;

@@ -276,7 +276,239 @@ hipError_t dispatch_ops(hipStream_t s, int ta, int tb, const LpParams &p, int mt
   return dispatch_tile<MODE, OP_T, OP_T>(s, p, mt, nt, tri_grid);
 }
 
+// ---------------------------------------------------------------------------
+// BF16 products from BF16 operands stored with K contiguous (the panel of the
+// mixed-precision factorisation, converted once per panel):
+//   D (FP32, column-major) <- alpha * A B^T + beta * C,
+//   A: m x k, row r at A + r * lda; B: n x k, row c at B + c * ldb.
+// K is a multiple of BK and the operand rows are zero beyond k.  Both tiles
+// go global -> LDS with 16-byte loads and stores (8 k values), and every MFMA
+// fragment is one 16-byte LDS read.
+// ---------------------------------------------------------------------------
+constexpr int BK = 64;           // K step
+constexpr int BLK = BK + 8;      // LDS row stride (bf16 elements): 144 bytes
+
+typedef unsigned short u16;
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+struct Bf16Params {
+  int m, n, k;
+  float alpha, beta;
+  const u16 *A;
+  long lda;
+  const u16 *B;
+  long ldb;
+  const float *C;
+  long ldc;
+  float *D;
+  long ldd;
+  int tri_out;
+  const int *abort_flag;
+};
+
+template <int MT, int NT, bool TG>
+__global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p) {
+  if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+  constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
+  constexpr int CA = MT * BK / 8 / NTHREADS;  // 16-byte chunks per thread
+  constexpr int CB = NT * BK / 8 / NTHREADS;
+  static_assert(CA >= 1 && CB >= 1, "tile too small");
+  __shared__ __attribute__((aligned(16))) u16 sA[2][MT * BLK];
+  __shared__ __attribute__((aligned(16))) u16 sB[2][NT * BLK];
+
+  const int nwg = TG ? (int)gridDim.x : (int)(gridDim.x * gridDim.y);
+  const int wid = TG ? (int)blockIdx.x : (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  const int logical = xcd_contiguous(wid, nwg);
+  int bm, bn;
+  if constexpr (TG) {
+    const int id = logical;
+    int r = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= id) r++;
+    while (r * (r + 1) / 2 > id) r--;
+    const int nt = (int)((sqrt(8.0 * (double)nwg + 1.0) - 1.0) * 0.5 + 0.5);
+    const int first = (r / GROUP) * GROUP, gs = min(GROUP, nt - first);
+    int o = id - first * (first + 1) / 2, tr, tc;
+    if (o < gs * first) {
+      tc = o / gs;
+      tr = first + o % gs;
+    } else {
+      o -= gs * first;
+      int cl = 0;
+      while (o >= gs - cl) {
+        o -= gs - cl;
+        cl++;
+      }
+      tc = first + cl;
+      tr = tc + o;
+    }
+    if (p.tri_out == TRI_LOWER) { bm = tr; bn = tc; } else { bm = tc; bn = tr; }
+  } else {
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int per = GROUP * gy, first = (logical / per) * GROUP, gs = min(GROUP, gx - first);
+    const int o = logical % per;
+    bm = first + o % gs;
+    bn = o / gs;
+  }
+  const int m0 = bm * MT, n0 = bn * NT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+
+  v4f acc[BI][BJ];
+#pragma unroll
+  for (int i = 0; i < BI; i++)
+#pragma unroll
+    for (int j = 0; j < BJ; j++) acc[i][j] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+
+  const int kend = p.alpha == 0.0f ? 0 : p.k;
+  if (kend > 0) {
+    // chunk q of a thread: row (tid + q 256) / 8, k offset 8 ((tid + q 256) % 8)
+    v4u ra[CA], rb[CB];
+    auto load = [&](int kb) {
+#pragma unroll
+      for (int q = 0; q < CA; q++) {
+        const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        const int gr = m0 + row;
+        const v4u *src = reinterpret_cast<const v4u *>(p.A + (long)(gr < p.m ? gr : 0) * p.lda + kb + kc);
+        const v4u v = *src;
+        ra[q] = gr < p.m ? v : (v4u){0u, 0u, 0u, 0u};
+      }
+#pragma unroll
+      for (int q = 0; q < CB; q++) {
+        const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        const int gc = n0 + row;
+        const v4u *src = reinterpret_cast<const v4u *>(p.B + (long)(gc < p.n ? gc : 0) * p.ldb + kb + kc);
+        const v4u v = *src;
+        rb[q] = gc < p.n ? v : (v4u){0u, 0u, 0u, 0u};
+      }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < CA; q++) {
+        const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        *reinterpret_cast<v4u *>(&sA[buf][row * BLK + kc]) = ra[q];
+      }
+#pragma unroll
+      for (int q = 0; q < CB; q++) {
+        const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        *reinterpret_cast<v4u *>(&sB[buf][row * BLK + kc]) = rb[q];
+      }
+    };
+    const int ntiles = kend / BK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < ntiles; kt++) {
+      const bool more = kt + 1 < ntiles;
+      if (more) load((kt + 1) * BK);
+      const u16 *cA = sA[kt & 1], *cB = sB[kt & 1];
+#pragma unroll
+      for (int h = 0; h < BK; h += 32) {
+        v8bf fa[BI], fb[BJ];
+#pragma unroll
+        for (int i = 0; i < BI; i++)
+          fa[i] = *reinterpret_cast<const v8bf *>(&cA[(wm0 + 16 * i + lc) * BLK + h + 8 * lg]);
+#pragma unroll
+        for (int j = 0; j < BJ; j++)
+          fb[j] = *reinterpret_cast<const v8bf *>(&cB[(wn0 + 16 * j + lc) * BLK + h + 8 * lg]);
+#pragma unroll
+        for (int i = 0; i < BI; i++)
+#pragma unroll
+          for (int j = 0; j < BJ; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+      }
+      if (more) store((kt + 1) & 1);
+      __syncthreads();
+    }
+  }
+
+  const bool use_beta = p.beta != 0.0f;
+#pragma unroll
+  for (int i = 0; i < BI; i++) {
+    const int gm = m0 + wm0 + 16 * i + lc;
+    if (gm >= p.m) continue;
+#pragma unroll
+    for (int j = 0; j < BJ; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gn = n0 + wn0 + 16 * j + 4 * lg + r;
+        if (gn >= p.n) continue;
+        if (p.tri_out == TRI_LOWER && gm < gn) continue;
+        if (p.tri_out == TRI_UPPER && gm > gn) continue;
+        float v = p.alpha * acc[i][j][r];
+        if (use_beta) v += p.beta * p.C[(long)gn * p.ldc + gm];
+        p.D[(long)gn * p.ldd + gm] = v;
+      }
+    }
+  }
+}
+
+// T (bf16, row r at T + r * ldt, k contiguous) <- P (FP32, column-major, ld
+// ldp) for rows 0..m-1 and k < kp, zeros for kp <= k < ldt.
+__global__ void panel_to_bf16_kernel(int m, int kp, const float *P, long ldp, u16 *T, long ldt) {
+  __shared__ float t[32][33];
+  const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + tx, k = k0 + j;
+    t[j][tx] = (r < m && k < kp) ? P[(long)k * ldp + r] : 0.0f;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, k = k0 + tx;
+    if (r < m && k < ldt) {
+      const __bf16 b = (__bf16)t[tx][j];
+      T[(long)r * ldt + k] = __builtin_bit_cast(u16, b);
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t panel_to_bf16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt) {
+  if (m <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((m + 31) / 32), (unsigned)((ldt + 31) / 32));
+  hipLaunchKernelGGL(panel_to_bf16_kernel, grid, dim3(256), 0, s, (int)m, (int)kp, P, ldp,
+                     static_cast<u16 *>(T), ldt);
+  return hipGetLastError();
+}
+
+hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, const void *A, long lda,
+                        const void *B, long ldb, float beta, const float *C, long ldc, float *D,
+                        long ldd, int tri_out, bool tri_grid, const int *abort_flag) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (k % BK != 0 || lda % 8 != 0 || ldb % 8 != 0) return hipErrorInvalidValue;
+  Bf16Params p;
+  p.m = (int)m; p.n = (int)n; p.k = (int)k;
+  p.alpha = alpha; p.beta = beta;
+  p.A = static_cast<const u16 *>(A); p.lda = lda;
+  p.B = static_cast<const u16 *>(B); p.ldb = ldb;
+  p.C = C; p.ldc = ldc; p.D = D; p.ldd = ldd;
+  p.tri_out = tri_out;
+  p.abort_flag = abort_flag;
+  const long tm = (m + 127) / 128, tn = (n + 127) / 128;
+  const long tiles = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
+  const bool big = tiles >= 240;
+  if (tri_grid) {
+    if (m != n) return hipErrorInvalidValue;
+    if (big) {
+      hipLaunchKernelGGL((gemm_bf16_nt_kernel<128, 128, true>), dim3((unsigned)tiles), dim3(NTHREADS),
+                         0, s, p);
+    } else {
+      const long t = (m + 63) / 64;
+      hipLaunchKernelGGL((gemm_bf16_nt_kernel<64, 64, true>), dim3((unsigned)(t * (t + 1) / 2)),
+                         dim3(NTHREADS), 0, s, p);
+    }
+    return hipGetLastError();
+  }
+  if (big)
+    hipLaunchKernelGGL((gemm_bf16_nt_kernel<128, 128, false>), dim3((unsigned)tm, (unsigned)tn),
+                       dim3(NTHREADS), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_bf16_nt_kernel<64, 64, false>),
+                       dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(NTHREADS), 0, s, p);
+  return hipGetLastError();
+}
 
 template <>
 hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, float alpha,

@@ -98,6 +98,8 @@ struct GemmOpts {
   int min_nt = 0;           // ... or columns of D (in-place safety)
   const int *abort_flag = nullptr;  // skip all work when *abort_flag != 0
   bool bf16 = false;        // float data only: products in BF16 (FP32 accumulate)
+  void *splitk_ws = nullptr;  // optional workspace: few-column products may split K
+  long splitk_elems = 0;      // ... of this many elements
 };
 
 template <typename T>
@@ -173,6 +175,15 @@ template <typename T>
 hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials, double *out);
 constexpr int kLogdetMaxBlocks = 1024;
 
+// BF16 products from BF16 operands with K contiguous (gemm_lp.hip):
+// D (FP32) <- alpha A B^T + beta C with A m x k (row r at A + r lda), B n x k;
+// k a multiple of 64, lda and ldb multiples of 8, rows zero-padded to k.
+hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, const void *A, long lda,
+                        const void *B, long ldb, float beta, const float *C, long ldc, float *D,
+                        long ldd, int tri_out, bool tri_grid, const int *abort_flag);
+// T (bf16, row-major, ld ldt) <- the m x kp FP32 panel P, zeros for k >= kp.
+hipError_t panel_to_bf16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt);
+
 // ---------------------------------------------------------------------------
 // mixed-precision solver kernels (mixed.hip)
 // ---------------------------------------------------------------------------
@@ -193,10 +204,25 @@ hipError_t sym_norm_inf(hipStream_t s, char uplo, long n, const double *A, long 
                         double *out);
 
 // ---------------------------------------------------------------------------
+// triangular solve with few right-hand sides (trsm_multi.hip)
+// ---------------------------------------------------------------------------
+// Inverses of the 128 x 128 diagonal blocks of op(A) (uplo, trans), for
+// trsm_multi; trsm_multi_ws_elems(n) elements.
+long trsm_multi_ws_elems(long n);
+template <typename T>
+hipError_t trsm_multi_inverses(hipStream_t s, char uplo, char trans, long n, const T *A, long lda,
+                               T *Winv);
+// X = op(A)^-1 B (left side, non-unit); B is overwritten (work), X separate.
+template <typename T>
+hipError_t trsm_multi(hipStream_t s, char uplo, char trans, long n, long nrhs, const T *A, long lda,
+                      const T *Winv, T *B, long ldb, T *X, long ldx);
+
+// ---------------------------------------------------------------------------
 // batched factorisation (batched.hip)
 // ---------------------------------------------------------------------------
 constexpr int kBatchedMaxN = 256;
-constexpr int kBatchedBlocksPerCU = 4;  // persistent workgroups per CU
+// resident workgroups per CU of the batched kernel (its occupancy)
+template <typename ST> int potrf_batched_blocks_per_cu(char uplo);
 // Factor count packed matrices in place (matrix order[t] for t = 0..count-1,
 // n = nvec[i] <= kBatchedMaxN, at element offset offs[i]); info[i] and
 // logdet[i] (may be null) per matrix; `grid` persistent workgroups.
@@ -204,12 +230,29 @@ template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,
                          const int *order, ST *A, double *logdet, int *info, int grid);
 
+// GMRES basis helpers: right-hand side k owns V + (k nv + i) n, i < nv.
+hipError_t basis_dots(hipStream_t s, long n, long nrhs, int nv, int first, int nvec, const double *V,
+                      const double *W, long ldw, double *out);
+hipError_t basis_sub(hipStream_t s, long n, long nrhs, int nv, int nvec, const double *V,
+                     const double *h, double *W, long ldw);
+hipError_t basis_put(hipStream_t s, long n, long nrhs, int nv, int slot, const double *W, long ldw,
+                     const double *scale, double *V);
+hipError_t basis_combine(hipStream_t s, long n, long nrhs, int nv, int nvec, const double *V,
+                         const double *y, double *X, long ldx);
+
+// Measured outer block size for precision D / Z / S / B at order n, 0 when
+// the tuning table has no entry (tuning.cpp).
+long tuned_potrf_nb(char prec, long n);
+
 // LAPACK internals shared by the solvers (lapack_api.cpp).
 hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out);
 // Blocked factorisation on the handle's streams; W: diag_max^2 elements.
 // bf16: FP32 data only, trailing updates with BF16 products.
 template <typename T>
-hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16);
+hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
+                           void *lp_ws = nullptr);
+// Workspace of the bf16 factorisation (two bfloat16 panel copies).
+size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n);
 
 // ---------------------------------------------------------------------------
 // error plumbing
@@ -252,6 +295,8 @@ struct __culapackhandle_st {
   size_t ws_bytes;
   int *dinfo;             // device info word
   int *hinfo;             // pinned host copy
+  int masked;             // main stream created with a CU mask (not ordered after the null stream)
+  long nb_override[4];    // outer block size set by cuLAPACKSetBlockSize (D, Z, S, B), 0 = tuned
   // optional timing of the factorisation's trailing updates (event pairs on
   // the main stream and the algorithmic flops of each launch)
   int timing;

@@ -12,6 +12,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <type_traits>
 
 using hc::bad_arg;
 using hc::dptr;
@@ -42,13 +43,27 @@ template <typename T> constexpr int conj_op() { return hc::is_complex<T>::value 
 // Outer panel width of the right-looking factorisation: the trailing update
 // has inner dimension NB, i.e. NB / 8 flops per byte of the trailing matrix
 // it streams (double), enough to stay matrix-core bound.
-template <typename T> long potrf_nb() {
-  static const long nb = [] {
+// Outer block size: the handle's override, else $CUCHOLESKY_POTRF_NB, else
+// the measured table, else the built-in default.  prec: D, Z, S or B.
+int prec_slot(char prec) { return prec == 'D' ? 0 : prec == 'Z' ? 1 : prec == 'S' ? 2 : 3; }
+
+long potrf_nb(CULAPACKhandle h, char prec, long n) {
+  const long o = h != nullptr ? h->nb_override[prec_slot(prec)] : 0;
+  if (o > 0) return o;
+  static const long env = [] {
     const char *e = std::getenv("CUCHOLESKY_POTRF_NB");
-    const long v = e ? std::atol(e) : 0;
-    return v > 0 ? v : (hc::is_complex<T>::value ? 128L : 384L);
+    return e ? std::atol(e) : 0L;
   }();
-  return nb;
+  if (env > 0) return env;
+  const long t = hc::tuned_potrf_nb(prec, n);
+  if (t > 0) return t;
+  return prec == 'Z' ? 128L : 384L;
+}
+
+template <typename T> char prec_of(bool bf16) {
+  if (hc::is_complex<T>::value) return 'Z';
+  if (std::is_same<T, float>::value) return bf16 ? 'B' : 'S';
+  return 'D';
 }
 
 // Panel factorisation of columns [j, j + jb) (lower; rows [j, j + jb) for
@@ -134,15 +149,28 @@ hipError_t timing_end(CULAPACKhandle h, hipStream_t s) {
   return hipEventRecord(h->tev[2 * h->tcount++ + 1], s);
 }
 
+// bf16 (FP32 data, lower only): after each panel is factored it is also
+// converted to bfloat16 with k contiguous (two alternating buffers at lp_ws,
+// potrf_bf16_ws_bytes) and the next-panel and trailing updates take their
+// products from those copies on the BF16 matrix cores.
 template <typename T>
-hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false) {
-  const long NB = potrf_nb<T>();
+hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false,
+                        void *lp_ws = nullptr) {
+  const long NB = potrf_nb(h, prec_of<T>(bf16), n);
   hipStream_t s0 = h->stream, s1 = h->side;
   const bool lower = (uplo == 'L');
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0);
   auto at = [&](long r, long c) { return A + c * lda + r; };
+  const long ldt = (NB + 63) / 64 * 64;
+  unsigned short *Pt[2] = {static_cast<unsigned short *>(lp_ws),
+                           static_cast<unsigned short *>(lp_ws) + (lp_ws ? n * ldt : 0)};
+  if (bf16 && (!lower || lp_ws == nullptr || !std::is_same<T, float>::value)) return hipErrorInvalidValue;
   HC_TRY(factor_panel<T>(h, s0, lower, n, A, lda, W, 0, std::min(NB, n)));
+  if constexpr (std::is_same<T, float>::value) {
+    if (bf16) HC_TRY(hc::panel_to_bf16(s0, n, std::min(NB, n), A, lda, Pt[0], ldt));
+  }
   for (long j = 0; j < n; j += NB) {
+    const int pj = (int)((j / NB) & 1);
     const long jb = std::min(NB, n - j);
     const long jn = j + jb;                 // next panel
     if (jn >= n) break;
@@ -152,8 +180,18 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     u.bf16 = bf16;
     if (lower) {  // next panel columns: A[jn:, jn:jn+jbn] -= P[jn:] P[jn:jn+jbn]^H
       u.tri_out = hc::TRI_LOWER;
-      HC_TRY(hc::gemm<T>(s0, hc::OP_N, conj_op<T>(), n - jn, jbn, jb, mone, at(jn, j), lda, at(jn, j),
-                         lda, one, at(jn, jn), lda, at(jn, jn), lda, u));
+      bool done = false;
+      if constexpr (std::is_same<T, float>::value) {
+        if (bf16) {
+          const unsigned short *Pn = Pt[pj] + (jn - j) * ldt;
+          HC_TRY(hc::gemm_bf16_nt(s0, n - jn, jbn, ldt, -1.0f, Pn, ldt, Pn, ldt, 1.0f, at(jn, jn), lda,
+                                  at(jn, jn), lda, hc::TRI_LOWER, false, h->dinfo));
+          done = true;
+        }
+      }
+      if (!done)
+        HC_TRY(hc::gemm<T>(s0, hc::OP_N, conj_op<T>(), n - jn, jbn, jb, mone, at(jn, j), lda, at(jn, j),
+                           lda, one, at(jn, jn), lda, at(jn, jn), lda, u));
     } else {
       u.tri_out = hc::TRI_UPPER;
       HC_TRY(hc::gemm<T>(s0, conj_op<T>(), hc::OP_N, jbn, n - jn, jb, mone, at(j, jn), lda, at(j, jn),
@@ -162,12 +200,25 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     HC_TRY(hipEventRecord(h->ev_update, s0));
     HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
     HC_TRY(factor_panel<T>(h, s1, lower, n, A, lda, W, jn, jbn));
+    if constexpr (std::is_same<T, float>::value) {
+      if (bf16) HC_TRY(hc::panel_to_bf16(s1, n - jn, jbn, at(jn, jn), lda, Pt[pj ^ 1], ldt));
+    }
     HC_TRY(hipEventRecord(h->ev_panel, s1));
     const long t0 = jn + jbn, t = n - t0;   // trailing matrix beyond the next panel
     if (t > 0) {
       const double flops = (hc::is_complex<T>::value ? 4.0 : 1.0) * (double)t * (double)(t + 1) * (double)jb;
       HC_TRY(timing_begin(h, s0, flops));
-      if (lower)
+      bool done = false;
+      if constexpr (std::is_same<T, float>::value) {
+        if (bf16) {
+          const unsigned short *Pn = Pt[pj] + (t0 - j) * ldt;
+          HC_TRY(hc::gemm_bf16_nt(s0, t, t, ldt, -1.0f, Pn, ldt, Pn, ldt, 1.0f, at(t0, t0), lda,
+                                  at(t0, t0), lda, hc::TRI_LOWER, true, h->dinfo));
+          done = true;
+        }
+      }
+      if (done) {
+      } else if (lower)
         HC_TRY(hc::herk<T>(s0, 'L', 'N', t, jb, -1.0, at(t0, j), lda, 1.0, at(t0, t0), lda, h->dinfo,
                            bf16));
       else
@@ -258,6 +309,10 @@ CUresult potrf_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n,
   HC_CHECK(g.status());
   void *ws = nullptr;
   HC_CHECK(workspace(h, (size_t)hc::diag_max<T>() * hc::diag_max<T>() * sizeof(T), &ws));
+  if (h->masked) {  // order after work already queued on the null stream
+    HC_CHECK(hipEventRecord(h->ev_update, nullptr));
+    HC_CHECK(hipStreamWaitEvent(h->stream, h->ev_update, 0));
+  }
   HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), h->stream));
   HC_CHECK(potrf_device<T>(h, (char)uplo, (long)n, dptr<T>(A), (long)lda, static_cast<T *>(ws)));
   return finish(h, info);
@@ -387,11 +442,19 @@ hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out) {
 }
 
 template <typename T>
-hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16) {
-  return potrf_device<T>(h, uplo, n, A, lda, W, bf16);
+hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
+                           void *lp_ws) {
+  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws);
 }
-template hipError_t potrf_on_device<double>(CULAPACKhandle, char, long, double *, long, double *, bool);
-template hipError_t potrf_on_device<float>(CULAPACKhandle, char, long, float *, long, float *, bool);
+template hipError_t potrf_on_device<double>(CULAPACKhandle, char, long, double *, long, double *, bool,
+                                            void *);
+template hipError_t potrf_on_device<float>(CULAPACKhandle, char, long, float *, long, float *, bool,
+                                           void *);
+
+size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n) {
+  const long ldt = (potrf_nb(h, 'B', n) + 63) / 64 * 64;
+  return 2 * (size_t)n * (size_t)ldt * 2;
+}
 
 }  // namespace hc
 
@@ -400,6 +463,16 @@ extern "C" {
 CUresult cuDpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
   return potrf_api<double>("cuDpotrf", h, uplo, n, A, lda, info);
 }
+CUresult cuLAPACKSetBlockSize(CULAPACKhandle h, char precision, long nb) {
+  if (h == nullptr) return bad_arg("cuLAPACKSetBlockSize", 1);
+  if (precision != 'D' && precision != 'Z' && precision != 'S' && precision != 'B')
+    return bad_arg("cuLAPACKSetBlockSize", 2);
+  if (nb < 0 || (nb > 0 && nb < hc::diag_max<double>()) || nb > 8192)
+    return bad_arg("cuLAPACKSetBlockSize", 3);
+  h->nb_override[prec_slot(precision)] = nb;
+  return CUDA_SUCCESS;
+}
+
 CUresult cuSpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
   return potrf_api<float>("cuSpotrf", h, uplo, n, A, lda, info);
 }

@@ -207,3 +207,84 @@ hipError_t sym_norm_inf(hipStream_t s, char uplo, long n, const double *A, long 
 }
 
 }  // namespace hc
+
+// ---------------------------------------------------------------------------
+// Krylov-basis helpers of GMRES-based refinement.  Right-hand side k owns the
+// basis vectors V_k[:, i] = V + (k * nv + i) * n (each of length n).
+// ---------------------------------------------------------------------------
+namespace hc {
+namespace {
+
+// out[k * nv + i] = V_k[:, i] . w_k  for first <= i < first + nvec (w_k = W + k * ldw)
+__global__ void __launch_bounds__(256) basis_dots_kernel(int n, int nv, int first, const double *V,
+                                                          const double *W, long ldw, double *out) {
+  const int i = first + (int)blockIdx.x, k = blockIdx.y;
+  const double *v = V + ((long)k * nv + i) * n, *w = W + (long)k * ldw;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += 256) s += v[r] * w[r];
+  const double t = block_sum<256>(s);
+  if (threadIdx.x == 0) out[(long)k * nv + i] = t;
+}
+
+// w_k -= sum_{i < nvec} h[k * nv + i] V_k[:, i]
+__global__ void basis_sub_kernel(int n, int nv, int nvec, const double *V, const double *h, double *W,
+                                 long ldw) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < nvec; i++) s += h[(long)k * nv + i] * V[((long)k * nv + i) * n + r];
+  W[(long)k * ldw + r] -= s;
+}
+
+// V_k[:, slot] = w_k * scale[k]
+__global__ void basis_put_kernel(int n, int nv, int slot, const double *W, long ldw,
+                                 const double *scale, double *V) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (r >= n) return;
+  V[((long)k * nv + slot) * n + r] = W[(long)k * ldw + r] * scale[k];
+}
+
+// x_k += sum_{i < nvec} y[k * nv + i] V_k[:, i]
+__global__ void basis_combine_kernel(int n, int nv, int nvec, const double *V, const double *y,
+                                     double *X, long ldx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < nvec; i++) s += y[(long)k * nv + i] * V[((long)k * nv + i) * n + r];
+  X[(long)k * ldx + r] += s;
+}
+
+}  // namespace
+
+hipError_t basis_dots(hipStream_t s, long n, long nrhs, int nv, int first, int nvec, const double *V,
+                      const double *W, long ldw, double *out) {
+  if (nvec <= 0 || nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(basis_dots_kernel, dim3((unsigned)nvec, (unsigned)nrhs), dim3(256), 0, s, (int)n,
+                     nv, first, V, W, ldw, out);
+  return hipGetLastError();
+}
+
+hipError_t basis_sub(hipStream_t s, long n, long nrhs, int nv, int nvec, const double *V,
+                     const double *h, double *W, long ldw) {
+  if (nvec <= 0 || nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(basis_sub_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nrhs), dim3(256), 0,
+                     s, (int)n, nv, nvec, V, h, W, ldw);
+  return hipGetLastError();
+}
+
+hipError_t basis_put(hipStream_t s, long n, long nrhs, int nv, int slot, const double *W, long ldw,
+                     const double *scale, double *V) {
+  hipLaunchKernelGGL(basis_put_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nrhs), dim3(256), 0,
+                     s, (int)n, nv, slot, W, ldw, scale, V);
+  return hipGetLastError();
+}
+
+hipError_t basis_combine(hipStream_t s, long n, long nrhs, int nv, int nvec, const double *V,
+                         const double *y, double *X, long ldx) {
+  if (nvec <= 0 || nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(basis_combine_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nrhs), dim3(256),
+                     0, s, (int)n, nv, nvec, V, y, X, ldx);
+  return hipGetLastError();
+}
+
+}  // namespace hc

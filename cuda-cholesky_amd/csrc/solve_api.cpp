@@ -9,6 +9,7 @@
 // matrices, CUresult returns and XERBLA on bad arguments.
 #include "api_util.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -25,52 +26,101 @@ namespace {
 
 constexpr long kIterMax = 30;  // refinement steps before falling back (LAPACK ITERMAX)
 
-// B <- inv(A) B for the factor in triangle uplo (L L^T or U^T U), n x nrhs.
+// The two triangular factors of A = L L^T (uplo L) or U^T U (uplo U), in
+// solve order: first the lower-like one, then its transpose.
+struct PotrsOps {
+  char t1, t2;
+};
+PotrsOps potrs_ops(char uplo) { return uplo == 'L' ? PotrsOps{'N', 'T'} : PotrsOps{'T', 'N'}; }
+
+// Inverses of the diagonal blocks of both triangular factors
+// (2 trsm_multi_ws_elems(n) elements at W).
 template <typename T>
-hipError_t potrs_device(hipStream_t s, char uplo, long n, long nrhs, const T *F, long ldf, T *B,
-                        long ldb, T *ws) {
-  const T one = hc::from_real<T>(1.0);
-  if (uplo == 'L') {
-    HC_TRY(hc::trsm<T>(s, 'L', 'L', 'N', 'N', n, nrhs, one, F, ldf, B, ldb, ws));
-    return hc::trsm<T>(s, 'L', 'L', 'T', 'N', n, nrhs, one, F, ldf, B, ldb, ws);
-  }
-  HC_TRY(hc::trsm<T>(s, 'L', 'U', 'T', 'N', n, nrhs, one, F, ldf, B, ldb, ws));
-  return hc::trsm<T>(s, 'L', 'U', 'N', 'N', n, nrhs, one, F, ldf, B, ldb, ws);
+hipError_t potrs_prepare(hipStream_t s, char uplo, long n, const T *F, long ldf, T *W) {
+  const PotrsOps o = potrs_ops(uplo);
+  HC_TRY(hc::trsm_multi_inverses<T>(s, uplo, o.t1, n, F, ldf, W));
+  return hc::trsm_multi_inverses<T>(s, uplo, o.t2, n, F, ldf, W + hc::trsm_multi_ws_elems(n));
+}
+
+// B <- inv(A) B for the factor in triangle uplo, n x nrhs; Y: n x nrhs work.
+template <typename T>
+hipError_t potrs_device(hipStream_t s, char uplo, long n, long nrhs, const T *F, long ldf,
+                        const T *W, T *B, long ldb, T *Y) {
+  const PotrsOps o = potrs_ops(uplo);
+  HC_TRY(hc::trsm_multi<T>(s, uplo, o.t1, n, nrhs, F, ldf, W, B, ldb, Y, n));
+  return hc::trsm_multi<T>(s, uplo, o.t2, n, nrhs, F, ldf, W + hc::trsm_multi_ws_elems(n), Y, n, B,
+                           ldb);
 }
 
 // R <- B - A X with A symmetric in triangle uplo: the stored triangle times X
 // plus the transpose of its strict part times X (two masked GEMMs).
 hipError_t residual(hipStream_t s, char uplo, long n, long nrhs, const double *A, long lda,
-                    const double *B, long ldb, const double *X, long ldx, double *R, long ldr) {
+                    const double *B, long ldb, const double *X, long ldx, double *R, long ldr,
+                    double *part, long part_elems) {
   HC_TRY(hc::copy2d<double>(s, n, nrhs, B, ldb, R, ldr));
   hc::GemmOpts o1;
+  o1.splitk_ws = part;
+  o1.splitk_elems = part_elems;
   o1.tri_a = (uplo == 'L') ? hc::TRI_LOWER : hc::TRI_UPPER;
   HC_TRY(hc::gemm<double>(s, hc::OP_N, hc::OP_N, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, ldr, R,
                           ldr, o1));
   hc::GemmOpts o2;  // op(A) = A^T is upper-like (lower storage) / lower-like (upper)
+  o2.splitk_ws = part;
+  o2.splitk_elems = part_elems;
   o2.tri_a = (uplo == 'L') ? hc::TRI_UPPER : hc::TRI_LOWER;
   o2.zero_diag = true;
   return hc::gemm<double>(s, hc::OP_T, hc::OP_N, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, ldr, R,
                           ldr, o2);
 }
 
+// W <- A X with A symmetric in triangle uplo (same two masked products).
+hipError_t symm_apply(hipStream_t s, char uplo, long n, long nrhs, const double *A, long lda,
+                      const double *X, long ldx, double *W, long ldw, double *part, long part_elems) {
+  hc::GemmOpts o1;
+  o1.splitk_ws = part;
+  o1.splitk_elems = part_elems;
+  o1.tri_a = (uplo == 'L') ? hc::TRI_LOWER : hc::TRI_UPPER;
+  HC_TRY(hc::gemm<double>(s, hc::OP_N, hc::OP_N, n, nrhs, n, 1.0, A, lda, X, ldx, 0.0, nullptr, 0, W,
+                          ldw, o1));
+  hc::GemmOpts o2;
+  o2.splitk_ws = part;
+  o2.splitk_elems = part_elems;
+  o2.tri_a = (uplo == 'L') ? hc::TRI_UPPER : hc::TRI_LOWER;
+  o2.zero_diag = true;
+  return hc::gemm<double>(s, hc::OP_T, hc::OP_N, n, nrhs, n, 1.0, A, lda, X, ldx, 1.0, W, ldw, W, ldw,
+                          o2);
+}
+
+// GMRES-based refinement: restart length and inner relative tolerance.
+constexpr int kGmresM = 30;
+constexpr double kGmresTol = 1e-10;
+constexpr long kGmresOuterMax = 10;
+
 // Workspace layout of the mixed solver (byte offsets, 256-aligned).
 struct SposvLayout {
-  size_t S, W, T, R, RS, NORM, DEV, total;
+  size_t S, W, T, Y, R, RS, PART, NORM, DEV, P16, V, WV, HD, total;
 };
+
+constexpr long kPartElems = 2048 * 128 * 16;  // split-K partial sums of the residual
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-SposvLayout sposv_layout(long n, long nrhs) {
+SposvLayout sposv_layout(CULAPACKhandle h, long n, long nrhs) {
   SposvLayout L;
   size_t off = 0;
   L.S = off;    off = align256(off + (size_t)n * n * sizeof(float));              // factor
   L.W = off;    off = align256(off + 128 * 128 * sizeof(double));                 // diag inverse
-  L.T = off;    off = align256(off + (size_t)hc::trsm_ws_elems('L', n, nrhs) * sizeof(double));
+  L.T = off;    off = align256(off + 2 * (size_t)hc::trsm_multi_ws_elems(n) * sizeof(double));
+  L.Y = off;    off = align256(off + (size_t)n * nrhs * sizeof(double));          // solve work
   L.R = off;    off = align256(off + (size_t)n * nrhs * sizeof(double));          // residual
   L.RS = off;   off = align256(off + (size_t)n * nrhs * sizeof(float));           // FP32 rhs
+  L.PART = off; off = align256(off + (size_t)kPartElems * sizeof(double));
   L.NORM = off; off = align256(off + (size_t)hc::sym_norm_ws_doubles(n) * sizeof(double));
   L.DEV = off;  off = align256(off + (3 * (size_t)nrhs + 8) * sizeof(double));    // norms, flags
+  L.P16 = off;  off = align256(off + hc::potrf_bf16_ws_bytes(h, n));                // bf16 panels
+  L.V = off;    off = align256(off + (size_t)nrhs * (kGmresM + 1) * n * sizeof(double));  // Krylov bases
+  L.WV = off;   off = align256(off + (size_t)n * nrhs * sizeof(double));
+  L.HD = off;   off = align256(off + 3 * (size_t)nrhs * (kGmresM + 1) * sizeof(double));
   L.total = off;
   return L;
 }
@@ -96,10 +146,12 @@ CUresult cuDpotrs(CULAPACKhandle h, CBlasUplo uplo, size_t n, size_t nrhs, CUdev
   hc::DeviceGuard g(h->device);
   HC_CHECK(g.status());
   void *ws = nullptr;
-  HC_CHECK(hc::lapack_workspace(h, (size_t)hc::trsm_ws_elems('L', (long)n, (long)nrhs) * sizeof(double),
-                                &ws));
-  HC_CHECK(potrs_device<double>(h->stream, (char)uplo, (long)n, (long)nrhs, dptr<double>(A), (long)lda,
-                                dptr<double>(B), (long)ldb, static_cast<double *>(ws)));
+  const long N = (long)n, K = (long)nrhs, WI = 2 * hc::trsm_multi_ws_elems(N);
+  HC_CHECK(hc::lapack_workspace(h, (size_t)(WI + N * K) * sizeof(double), &ws));
+  double *W = static_cast<double *>(ws);
+  HC_CHECK(potrs_prepare<double>(h->stream, (char)uplo, N, dptr<double>(A), (long)lda, W));
+  HC_CHECK(potrs_device<double>(h->stream, (char)uplo, N, K, dptr<double>(A), (long)lda, W,
+                                dptr<double>(B), (long)ldb, W + WI));
   HC_CHECK(hipStreamSynchronize(h->stream));
   return CUDA_SUCCESS;
 }
@@ -110,8 +162,8 @@ namespace {
 
 CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, size_t nrhs,
                CUdeviceptr A, size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx,
-               char factor, long *iter, char *used, long *info) {
-  if (info == nullptr) return bad_arg(name, 14);
+               char factor, char refine, long *iter, char *used, long *info) {
+  if (info == nullptr) return bad_arg(name, 15);
   *info = 0;
   if (h == nullptr) { *info = -1; return bad_arg(name, 1); }
   if (!hc::valid_uplo(uplo)) { *info = -1; return bad_arg(name, 1); }
@@ -119,7 +171,8 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   if (ldb < (n > 1 ? n : 1)) { *info = -7; return bad_arg(name, 7); }
   if (ldx < (n > 1 ? n : 1)) { *info = -9; return bad_arg(name, 9); }
   if (factor != 'S' && factor != 'B') { *info = -10; return bad_arg(name, 10); }
-  if (iter == nullptr) { *info = -11; return bad_arg(name, 11); }
+  if (refine != 'I' && refine != 'G') { *info = -11; return bad_arg(name, 11); }
+  if (iter == nullptr) { *info = -12; return bad_arg(name, 12); }
   *iter = 0;
   if (used != nullptr) *used = factor;
   if (n == 0 || nrhs == 0) return CUDA_SUCCESS;
@@ -128,13 +181,19 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   HC_CHECK(g.status());
   const long N = (long)n, K = (long)nrhs;
   const char ul = (char)uplo;
-  const SposvLayout L = sposv_layout(N, K);
+  const SposvLayout L = sposv_layout(h, N, K);
   void *base = nullptr;
   HC_CHECK(hc::lapack_workspace(h, L.total, &base));
   char *wsb = static_cast<char *>(base);
   float *S = reinterpret_cast<float *>(wsb + L.S);
   float *W = reinterpret_cast<float *>(wsb + L.W);
   void *T = wsb + L.T;
+  void *Yw = wsb + L.Y;
+  double *PART = reinterpret_cast<double *>(wsb + L.PART);
+  double *V = reinterpret_cast<double *>(wsb + L.V);
+  double *WV = reinterpret_cast<double *>(wsb + L.WV);
+  double *HD = reinterpret_cast<double *>(wsb + L.HD);  // dots (2 sets) and scales
+  constexpr int NV = kGmresM + 1;
   double *R = reinterpret_cast<double *>(wsb + L.R);
   float *RS = reinterpret_cast<float *>(wsb + L.RS);
   double *NW = reinterpret_cast<double *>(wsb + L.NORM);
@@ -186,28 +245,136 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     CU_TRY(read_flag(&flag));
     if (flag) { *code = -2; return CUDA_SUCCESS; }       // narrowing overflowed
     HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), s));
-    HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16));
+    HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16, wsb + L.P16));
     long finfo = 0;
     HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipStreamSynchronize(s));
     finfo = h->hinfo[0];
     if (finfo != 0) { *code = -3; return CUDA_SUCCESS; }  // low-precision factorisation failed
-    // x0 = inv(S S^T) b in FP32
+    // x0 = inv(S S^T) b in FP32; the diagonal-block inverses serve every solve
+    HC_CHECK(potrs_prepare<float>(s, 'L', N, S, N, static_cast<float *>(T)));
     HC_CHECK(hc::d2s(s, N, K, Bd, (long)ldb, RS, N, ovf));
     CU_TRY(read_flag(&flag));
     if (flag) { *code = -2; return CUDA_SUCCESS; }
-    HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, RS, N, static_cast<float *>(T)));
+    HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
+                                 static_cast<float *>(Yw)));
     HC_CHECK(hc::s2d(s, N, K, RS, N, Xd, (long)ldx));
+    // w <- inv(S S^T) w in FP32 (w: n x nrhs FP64, ld n)
+    auto precondition = [&](double *w) -> CUresult {
+      HC_CHECK(hc::d2s(s, N, K, w, N, RS, N, ovf));
+      HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
+                                   static_cast<float *>(Yw)));
+      HC_CHECK(hc::s2d(s, N, K, RS, N, w, N));
+      return CUDA_SUCCESS;
+    };
+    std::vector<double> hbuf(2 * (size_t)K * NV), scale(K), beta(K);
+    auto fetch = [&](const double *d, size_t count, double *hst) -> CUresult {
+      HC_CHECK(hipMemcpyAsync(hst, d, count * sizeof(double), hipMemcpyDeviceToHost, s));
+      HC_CHECK(hipStreamSynchronize(s));
+      return CUDA_SUCCESS;
+    };
+    auto upload = [&](double *d, const double *hst, size_t count) -> CUresult {
+      HC_CHECK(hipMemcpyAsync(d, hst, count * sizeof(double), hipMemcpyHostToDevice, s));
+      HC_CHECK(hipStreamSynchronize(s));
+      return CUDA_SUCCESS;
+    };
+    // One restarted GMRES cycle on inv(M) A d = inv(M) r (M = S S^T), x += d.
+    // Classical Gram-Schmidt with one reorthogonalisation, Givens rotations on
+    // the Hessenberg matrix, every right-hand side its own Krylov space.
+    auto gmres_cycle = [&]() -> CUresult {
+      HC_CHECK(hipMemcpyAsync(WV, R, (size_t)N * K * sizeof(double), hipMemcpyDeviceToDevice, s));
+      CU_TRY(precondition(WV));
+      std::fill(scale.begin(), scale.end(), 1.0);
+      CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
+      HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, HD + 2 * K * NV, V));
+      HC_CHECK(hc::basis_dots(s, N, K, NV, 0, 1, V, WV, N, HD));
+      CU_TRY(fetch(HD, (size_t)K * NV, hbuf.data()));
+      std::vector<std::vector<double>> H(K, std::vector<double>((size_t)NV * kGmresM, 0.0));
+      std::vector<std::vector<double>> g(K, std::vector<double>(NV, 0.0)), cs(K), sn(K);
+      for (long k = 0; k < K; k++) {
+        beta[k] = std::sqrt(hbuf[k * NV]);
+        scale[k] = beta[k] > 0.0 ? 1.0 / beta[k] : 0.0;
+        g[k][0] = beta[k];
+        cs[k].assign(kGmresM, 1.0);
+        sn[k].assign(kGmresM, 0.0);
+      }
+      CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
+      HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, HD + 2 * K * NV, V));
+      int used_j = 0;
+      for (int j = 0; j < kGmresM; j++) {
+        CU_TRY(symm_apply(s, ul, N, K, Ad, (long)lda, V + (long)j * N, (long)NV * N, WV, N, PART,
+                          kPartElems) == hipSuccess ? CUDA_SUCCESS : CUDA_ERROR_UNKNOWN);
+        CU_TRY(precondition(WV));
+        for (int pass = 0; pass < 2; pass++) {
+          double *hd = HD + pass * K * NV;
+          HC_CHECK(hc::basis_dots(s, N, K, NV, 0, j + 1, V, WV, N, hd));
+          HC_CHECK(hc::basis_sub(s, N, K, NV, j + 1, V, hd, WV, N));
+        }
+        CU_TRY(fetch(HD, 2 * (size_t)K * NV, hbuf.data()));
+        std::fill(scale.begin(), scale.end(), 1.0);
+        CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
+        HC_CHECK(hc::basis_put(s, N, K, NV, j + 1, WV, N, HD + 2 * K * NV, V));
+        HC_CHECK(hc::basis_dots(s, N, K, NV, j + 1, 1, V, WV, N, HD));
+        std::vector<double> nrm(K * NV);
+        CU_TRY(fetch(HD, (size_t)K * NV, nrm.data()));
+        bool all_done = true;
+        for (long k = 0; k < K; k++) {
+          double *col = H[k].data() + (size_t)j * NV;  // column j of H_k
+          for (int i = 0; i <= j; i++) col[i] = hbuf[k * NV + i] + hbuf[K * NV + k * NV + i];
+          const double hn = std::sqrt(nrm[k * NV + j + 1]);
+          col[j + 1] = hn;
+          scale[k] = hn > 0.0 ? 1.0 / hn : 0.0;
+          for (int i = 0; i < j; i++) {  // previous rotations
+            const double a = col[i], b = col[i + 1];
+            col[i] = cs[k][i] * a + sn[k][i] * b;
+            col[i + 1] = -sn[k][i] * a + cs[k][i] * b;
+          }
+          const double a = col[j], b = col[j + 1], rr = std::hypot(a, b);
+          cs[k][j] = rr > 0.0 ? a / rr : 1.0;
+          sn[k][j] = rr > 0.0 ? b / rr : 0.0;
+          col[j] = rr;
+          col[j + 1] = 0.0;
+          g[k][j + 1] = -sn[k][j] * g[k][j];
+          g[k][j] = cs[k][j] * g[k][j];
+          if (!(std::fabs(g[k][j + 1]) <= kGmresTol * beta[k]) && hn > 0.0) all_done = false;
+        }
+        CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
+        HC_CHECK(hc::basis_put(s, N, K, NV, j + 1, WV, N, HD + 2 * K * NV, V));
+        used_j = j + 1;
+        if (all_done) break;
+      }
+      // y = H^-1 g (upper triangular), then x += V y
+      std::vector<double> y((size_t)K * NV, 0.0);
+      for (long k = 0; k < K; k++) {
+        for (int i = used_j - 1; i >= 0; i--) {
+          double t = g[k][i];
+          for (int c = i + 1; c < used_j; c++) t -= H[k][(size_t)c * NV + i] * y[k * NV + c];
+          const double d = H[k][(size_t)i * NV + i];
+          y[k * NV + i] = d != 0.0 ? t / d : 0.0;
+        }
+      }
+      CU_TRY(upload(HD, y.data(), (size_t)K * NV));
+      HC_CHECK(hc::basis_combine(s, N, K, NV, used_j, V, HD, Xd, (long)ldx));
+      return CUDA_SUCCESS;
+    };
+
     Norms nm;
+    const long itmax = refine == 'G' ? kGmresOuterMax : kIterMax;
     for (long it = 0;; it++) {
-      HC_CHECK(residual(s, ul, N, K, Ad, (long)lda, Bd, (long)ldb, Xd, (long)ldx, R, N));
+      HC_CHECK(residual(s, ul, N, K, Ad, (long)lda, Bd, (long)ldb, Xd, (long)ldx, R, N, PART,
+                        kPartElems));
       CU_TRY(read_norms(nm));
       if (converged(nm)) { *code = it; return CUDA_SUCCESS; }
-      if (it == kIterMax) { *code = -(kIterMax + 1); return CUDA_SUCCESS; }
+      if (it == itmax) { *code = -(itmax + 1); return CUDA_SUCCESS; }
+      if (refine == 'G') {
+        CU_TRY(gmres_cycle());
+        continue;
+      }
       HC_CHECK(hc::d2s(s, N, K, R, N, RS, N, ovf));
       CU_TRY(read_flag(&flag));
       if (flag) { *code = -2; return CUDA_SUCCESS; }
-      HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, RS, N, static_cast<float *>(T)));
+      HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
+                                   static_cast<float *>(Yw)));
       HC_CHECK(hc::add_s2d(s, N, K, RS, N, Xd, (long)ldx));
     }
   };
@@ -235,8 +402,9 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   *info = h->hinfo[0];
   if (*info != 0) return CUDA_SUCCESS;
   HC_CHECK(hc::copy2d<double>(s, N, K, Bd, (long)ldb, Xd, (long)ldx));
-  HC_CHECK(potrs_device<double>(s, ul, N, K, Aw, (long)lda, Xd, (long)ldx,
-                                static_cast<double *>(T)));
+  HC_CHECK(potrs_prepare<double>(s, ul, N, Aw, (long)lda, static_cast<double *>(T)));
+  HC_CHECK(potrs_device<double>(s, ul, N, K, Aw, (long)lda, static_cast<const double *>(T), Xd,
+                                (long)ldx, static_cast<double *>(Yw)));
   HC_CHECK(hipStreamSynchronize(s));
   return CUDA_SUCCESS;
 }
@@ -247,14 +415,15 @@ extern "C" {
 
 CUresult cuDsposvx(CULAPACKhandle h, CBlasUplo uplo, size_t n, size_t nrhs, CUdeviceptr A,
                    size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx, char factor,
-                   long *iter, char *used, long *info) {
-  return sposv("cuDsposvx", h, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor, iter, used, info);
+                   char refine, long *iter, char *used, long *info) {
+  return sposv("cuDsposvx", h, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor, refine, iter, used,
+               info);
 }
 
 CUresult cuDsposv(CULAPACKhandle h, CBlasUplo uplo, size_t n, size_t nrhs, CUdeviceptr A,
                   size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx, long *iter,
                   long *info) {
-  return sposv("cuDsposv", h, uplo, n, nrhs, A, lda, B, ldb, X, ldx, 'S', iter, nullptr, info);
+  return sposv("cuDsposv", h, uplo, n, nrhs, A, lda, B, ldb, X, ldx, 'S', 'I', iter, nullptr, info);
 }
 
 }  // extern "C"

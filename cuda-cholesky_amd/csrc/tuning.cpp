@@ -1,0 +1,73 @@
+// Block sizes chosen by measurement.
+//
+// cuda-cholesky_amd/autotune.py times the factorisations over a grid of
+// outer block sizes on the GPU at hand and writes the winners to a text
+// table, one line per entry:
+//     potrf_nb <precision> <n_max> <nb>
+// precision: D (FP64), Z (complex FP64), S (FP32 products), B (BF16
+// products).  For an order n the first line of that precision with
+// n <= n_max applies.  The table is read once, from $CUCHOLESKY_TUNING or
+// else tuning.txt beside the library; without it the built-in defaults hold.
+#include "internal.h"
+
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace hc {
+namespace {
+
+struct Entry {
+  char prec;
+  long n_max;
+  long nb;
+};
+
+std::vector<Entry> &table() {
+  static std::vector<Entry> t;
+  return t;
+}
+
+void load() {
+  std::string path;
+  if (const char *e = std::getenv("CUCHOLESKY_TUNING")) {
+    path = e;
+  } else {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&load), &info) != 0 && info.dli_fname != nullptr) {
+      path = info.dli_fname;
+      const size_t slash = path.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning.txt";
+    }
+  }
+  if (path.empty()) return;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (f == nullptr) return;
+  char key[64], prec[8];
+  long n_max = 0, nb = 0;
+  char line[256];
+  while (std::fgets(line, sizeof line, f) != nullptr) {
+    if (line[0] == '#') continue;
+    if (std::sscanf(line, "%63s %7s %ld %ld", key, prec, &n_max, &nb) == 4 &&
+        std::strcmp(key, "potrf_nb") == 0 && nb > 0 && n_max > 0)
+      table().push_back(Entry{prec[0], n_max, nb});
+  }
+  std::fclose(f);
+}
+
+}  // namespace
+
+long tuned_potrf_nb(char prec, long n) {
+  static std::once_flag once;
+  std::call_once(once, load);
+  for (const Entry &e : table())
+    if (e.prec == prec && n <= e.n_max) return e.nb;
+  return 0;
+}
+
+}  // namespace hc

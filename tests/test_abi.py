@@ -179,7 +179,10 @@ def test_solver_argument_indices(cc):
         assert cc.cuDsposv(FAKE, cc.Upper, 10, 2, 0, 10, 0, 10, 0, 9) == (1, 0, -9)
         st, it, used, info = cc.cuDsposvx(FAKE, cc.Lower, 10, 2, 0, 10, 0, 10, 0, 10, "H")
         assert (st, info) == (1, -10)
-    assert x.calls == [("cuDpotrs", 5), ("cuDpotrs", 7), ("cuDsposv", 9), ("cuDsposvx", 10)]
+        st, it, used, info = cc.cuDsposvx(FAKE, cc.Lower, 10, 2, 0, 10, 0, 10, 0, 10, "S", "X")
+        assert (st, info) == (1, -11)
+    assert x.calls == [("cuDpotrs", 5), ("cuDpotrs", 7), ("cuDsposv", 9), ("cuDsposvx", 10),
+                       ("cuDsposvx", 11)]
 
 
 def test_solver_quick_returns(cc):

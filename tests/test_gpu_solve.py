@@ -164,7 +164,7 @@ def normwise_backward(A, X, B):
 
 
 @pytest.mark.parametrize("uplo", "LU")
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (65, 3), (300, 16), (1000, 5)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (65, 3), (300, 16), (1000, 5), (700, 40)])
 def test_dpotrs(gpu, lapack, uplo, n, nrhs):
     A = O.spd(n + 3, n, 0.5)
     B = np.asfortranarray(rng(4, n, nrhs).standard_normal((n, nrhs)))
@@ -196,7 +196,7 @@ def test_dpotrs(gpu, lapack, uplo, n, nrhs):
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 10 * n * EPS * cond
 
 
-def run_sposv(gpu, lapack, uplo, n, nrhs, alpha, factor, seed=0, A=None):
+def run_sposv(gpu, lapack, uplo, n, nrhs, alpha, factor, seed=0, A=None, refine="I"):
     A = O.spd(100 + seed, n, alpha) if A is None else A
     B = np.asfortranarray(rng(5, n, nrhs, seed).standard_normal((n, nrhs)))
     Astore = np.asfortranarray(np.tril(A) if uplo == "L" else np.triu(A))  # one triangle only
@@ -205,7 +205,7 @@ def run_sposv(gpu, lapack, uplo, n, nrhs, alpha, factor, seed=0, A=None):
     dB = gpu.to_device(B)
     dX = torch.zeros_like(dB)
     st, it, used, info = gpu.cuDsposvx(lapack, ord(uplo), n, nrhs, gpu.ptr(dA), n, gpu.ptr(dB), n,
-                                       gpu.ptr(dX), n, factor)
+                                       gpu.ptr(dX), n, factor, refine)
     torch.cuda.synchronize()
     assert st == 0 and info == 0
     X = gpu.from_device(dX)
@@ -266,3 +266,41 @@ def test_dsposv_not_spd(gpu, lapack):
     st, it, info = gpu.cuDsposv(lapack, ord("L"), n, 1, gpu.ptr(dA), n, gpu.ptr(dB), n, gpu.ptr(dX), n)
     torch.cuda.synchronize()
     assert st == 0 and it == -3 and info == 151
+
+
+@pytest.mark.parametrize("factor", "SB")
+def test_dsposv_large_split_residual(gpu, lapack, factor):
+    """n >= 8192 takes the split-K residual path (128 x 16 tiles, partial sums
+    reduced in a fixed order) and, with factor B, the bfloat16 panel copies of
+    every trailing update; the solution must still meet dsposv's test."""
+    n, nrhs = 8448, 16
+    g = torch.Generator(device="cuda").manual_seed(11)
+    G = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+    A = G @ G.T / n
+    A.diagonal().add_(1.0)
+    del G
+    B = torch.randn(nrhs, n, generator=g, device="cuda", dtype=torch.float64)
+    X = torch.zeros_like(B)
+    st, it, used, info = gpu.cuDsposvx(lapack, ord("L"), n, nrhs, gpu.ptr(A), n, gpu.ptr(B), n,
+                                       gpu.ptr(X), n, factor)
+    torch.cuda.synchronize()
+    assert st == 0 and info == 0 and it >= 0 and used == factor
+    R = B - X @ A
+    anorm = A.abs().sum(dim=1).max().item()
+    cte = anorm * UNIT * np.sqrt(n)
+    assert torch.all(R.abs().max(dim=1).values <= 4 * cte * X.abs().max(dim=1).values)
+
+
+@pytest.mark.parametrize("uplo", "LU")
+def test_dsposv_gmres_bf16(gpu, lapack, uplo):
+    """GMRES-IR keeps the bfloat16-product factor where classical refinement
+    with it stalls (cond ~ 4e3): the solution still meets dsposv's test."""
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, uplo, 1000, 4, 1e-3, "B", seed=4, refine="G")
+    assert it >= 0 and used == "B"
+    assert dsposv_converged(A, X, B)
+
+
+def test_dsposv_gmres_fp32_ill_conditioned(gpu, lapack):
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", 900, 3, 1e-6, "S", seed=5, refine="G")
+    assert it >= 0 and used == "S"
+    assert dsposv_converged(A, X, B)
