@@ -115,14 +115,14 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
 #pragma unroll
           for (int g = 0; g < TG; g++) {
             // tile t0 + g of the column-major lower triangle of the m x m tile block
-            int t = t0 + g, J = 0;
+            const bool live = t0 + g < ntile;
+            int t = live ? t0 + g : 0, J = 0;  // t < m (m >= 1 here): the walk ends
             while (t >= m - J) {
               t -= m - J;
               J++;
             }
             ti[g] = p + 1 + J + t;
             tj[g] = p + 1 + J;
-            const bool live = t0 + g < ntile;
 #pragma unroll
             for (int r = 0; r < 4; r++)
               acc[g].re[r] = live ? v.get(ti[g] * LEAF + lk + 4 * r, tj[g] * LEAF + li) : 0.0;

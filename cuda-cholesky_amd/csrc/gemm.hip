@@ -190,6 +190,22 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
 #pragma unroll
       for (int j = 0; j < BJ; j++) acc[q][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
+  // Real data with alpha = +-1 and beta != 0: during K step b < BI * BJ the
+  // wave also reads the C values of its accumulator block b and folds
+  // (beta / alpha) C into it (exact scaling), so reading C is spread over the
+  // K loop instead of landing on the epilogue in one burst; the epilogue then
+  // only stores alpha * acc for those blocks.  npre blocks are pre-added.
+  // Not with split K: there each split writes its raw partial sum and C
+  // enters once, in the reduction.
+  int npre = 0;
+  double cfac = 0.0;
+  if constexpr (!CPLX) {
+    if (!is_zero(p.beta) && (p.alpha == 1.0 || p.alpha == -1.0) && p.part == nullptr) {
+      cfac = p.beta / p.alpha;
+      npre = BI * BJ;  // reduced to the number of K steps below
+    }
+  }
+
   if (kbeg < kend) {
     // Per-thread source pointers at k = kbeg, and their per-element /
     // per-K-step strides; every address is formed by additions only.
@@ -211,7 +227,26 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     T ra[EA], rb[EB];
     // Out-of-range elements read a valid address (the operand's origin) and
     // are replaced by zero, so the loads carry no branches.
+    // Interior tiles with whole K steps and no triangular operand take
+    // unpredicated loads; edge tiles check every element.
+    const bool fast = m0 + MT <= p.m && n0 + NT <= p.n && (kend - kbeg) % KT == 0 && p.tri_a == TRI_NONE &&
+                      p.tri_b == TRI_NONE;
     auto load = [&](int kt_off, const T *qa, const T *qb) {
+      if (fast) {
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+          T v = qa[q * a_qstride];
+          if (TA == OP_C) v = conjx(v);
+          ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < EB; q++) {
+          T v = qb[q * b_qstride];
+          if (TB == OP_C) v = conjx(v);
+          rb[q] = v;
+        }
+        return;
+      }
 #pragma unroll
       for (int q = 0; q < EA; q++) {
         const int gi = ar0 + q * PA::RSTEP, gk = ak0 + kt_off + q * PA::KSTEP;
@@ -241,6 +276,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     };
 
     const int ntiles = (kend - kbeg + KT - 1) / KT;
+    if (npre > ntiles) npre = ntiles;
     load(0, pa, pb);
     store(0);
     __syncthreads();
@@ -250,6 +286,19 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
         pa += a_kstride;
         pb += b_kstride;
         load((kt + 1) * KT, pa, pb);
+      }
+      double cpre[4] = {0.0, 0.0, 0.0, 0.0};
+      if constexpr (!CPLX) {
+        if (kt < npre) {  // C values of accumulator block kt (uniform branch)
+          const int gm = m0 + wm0 + 16 * (kt / BJ) + lc;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int gn = n0 + wn0 + 16 * (kt % BJ) + lg + 4 * r;
+            const bool ok = gm < p.m && gn < p.n;
+            const double v = p.C[ok ? (long)gn * p.ldc + gm : 0];
+            cpre[r] = ok ? v : 0.0;
+          }
+        }
       }
       const double *cA = sA + (kt & 1) * BUFA, *cB = sB + (kt & 1) * BUFB;
 #pragma unroll
@@ -283,11 +332,26 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
             }
         }
       }
+      if constexpr (!CPLX) {
+        if (kt < npre) {  // select, not multiply: C may hold NaN outside the output triangle
+#pragma unroll
+          for (int i = 0; i < BI; i++)
+#pragma unroll
+            for (int j = 0; j < BJ; j++)
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const double a = acc[0][i][j][r];
+                acc[0][i][j][r] = (i * BJ + j == kt) ? a + cfac * cpre[r] : a;
+              }
+        }
+      }
       // The other buffer was last read in the previous step, which every
       // thread finished before the previous barrier.
       if (more) store((kt + 1) & 1);
       __syncthreads();
     }
+  } else {
+    npre = 0;
   }
 
   // ---- epilogue: lane lc holds row m; register r of lane group lg holds
@@ -330,7 +394,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
         if constexpr (CPLX) v = make_zd(acc[0][i][j][r], acc[1][i][j][r]);
         else v = acc[0][i][j][r];
         v = p.alpha * v;
-        if (use_beta) v = v + p.beta * p.C[(long)gn * p.ldc + gm];
+        if (use_beta && i * BJ + j >= npre) v = v + p.beta * p.C[(long)gn * p.ldc + gm];
         if constexpr (CPLX) {
           if (p.real_diag && gm == gn) v.y = 0.0;
         }

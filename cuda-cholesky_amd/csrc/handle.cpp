@@ -8,7 +8,6 @@
 #include "api_util.h"
 
 #include <cstdlib>
-#include <vector>
 
 extern "C" {
 
@@ -42,26 +41,7 @@ CUresult cuLAPACKCreate(CULAPACKhandle *handle) {
   h->device = h->blas->device;
   // Streams are created blocking (ordered after work on the null stream), at
   // the default priority.
-  // CUCHOLESKY_RESERVE_CUS = k > 0 keeps the main stream (trailing updates)
-  // off k CUs spread over the device, so the look-ahead stream's small panel
-  // kernels always find free CUs.  Such a stream does not order itself after
-  // the null stream; the factorisation waits for the null stream explicitly.
-  const char *rc = std::getenv("CUCHOLESKY_RESERVE_CUS");
-  const int reserve = rc ? std::atoi(rc) : 0;
-  int cus = 0;
-  HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-  if (reserve > 0 && reserve < cus) {
-    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-    for (int c = 0; c < cus; c++) mask[c / 32] |= 1u << (c % 32);
-    for (int i = 0; i < reserve; i++) {
-      const int c = (int)((long)i * cus / reserve);
-      mask[c / 32] &= ~(1u << (c % 32));
-    }
-    HC_CHECK(hipExtStreamCreateWithCUMask(&h->stream, (uint32_t)mask.size(), mask.data()));
-    h->masked = 1;
-  } else {
-    HC_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamDefault));
-  }
+  HC_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamDefault));
   HC_CHECK(hipStreamCreateWithFlags(&h->side, hipStreamDefault));
   HC_CHECK(hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming));
   HC_CHECK(hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming));

@@ -295,7 +295,6 @@ struct __culapackhandle_st {
   size_t ws_bytes;
   int *dinfo;             // device info word
   int *hinfo;             // pinned host copy
-  int masked;             // main stream created with a CU mask (not ordered after the null stream)
   long nb_override[4];    // outer block size set by cuLAPACKSetBlockSize (D, Z, S, B), 0 = tuned
   // optional timing of the factorisation's trailing updates (event pairs on
   // the main stream and the algorithmic flops of each launch)

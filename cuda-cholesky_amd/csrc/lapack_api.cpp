@@ -73,7 +73,12 @@ template <typename T> char prec_of(bool bf16) {
 template <typename T>
 hipError_t factor_panel(CULAPACKhandle h, hipStream_t s, bool lower, long n, T *A, long lda, T *W,
                         long j, long jb) {
-  constexpr int IB = hc::diag_max<T>();
+  // panel slice width: one diagonal block in LDS ($CUCHOLESKY_IB may narrow it)
+  static const int IB = [] {
+    const char *e = std::getenv("CUCHOLESKY_IB");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 16 && v <= hc::diag_max<T>()) ? v : hc::diag_max<T>();
+  }();
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0), zero = hc::from_real<T>(0.0);
   auto at = [&](long r, long c) { return A + c * lda + r; };
   const char uplo = lower ? 'L' : 'U';
@@ -309,10 +314,6 @@ CUresult potrf_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n,
   HC_CHECK(g.status());
   void *ws = nullptr;
   HC_CHECK(workspace(h, (size_t)hc::diag_max<T>() * hc::diag_max<T>() * sizeof(T), &ws));
-  if (h->masked) {  // order after work already queued on the null stream
-    HC_CHECK(hipEventRecord(h->ev_update, nullptr));
-    HC_CHECK(hipStreamWaitEvent(h->stream, h->ev_update, 0));
-  }
   HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), h->stream));
   HC_CHECK(potrf_device<T>(h, (char)uplo, (long)n, dptr<T>(A), (long)lda, static_cast<T *>(ws)));
   return finish(h, info);

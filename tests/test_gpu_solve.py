@@ -270,10 +270,11 @@ def test_dsposv_not_spd(gpu, lapack):
 
 @pytest.mark.parametrize("factor", "SB")
 def test_dsposv_large_split_residual(gpu, lapack, factor):
-    """n >= 8192 takes the split-K residual path (128 x 16 tiles, partial sums
-    reduced in a fixed order) and, with factor B, the bfloat16 panel copies of
-    every trailing update; the solution must still meet dsposv's test."""
-    n, nrhs = 8448, 16
+    """n = 16384 takes the split-K residual path (128 x 16 tiles, 16 splits,
+    partial sums reduced in a fixed order) and, with factor B, the bfloat16
+    panel copies of every trailing update; the solution must still meet
+    dsposv's test without falling back."""
+    n, nrhs = 16384, 16
     g = torch.Generator(device="cuda").manual_seed(11)
     G = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
     A = G @ G.T / n
