@@ -212,10 +212,14 @@ long trsm_multi_ws_elems(long n);
 template <typename T>
 hipError_t trsm_multi_inverses(hipStream_t s, char uplo, char trans, long n, const T *A, long lda,
                                T *Winv);
-// X = op(A)^-1 B (left side, non-unit); B is overwritten (work), X separate.
+// X = op(A)^-1 B (left side, non-unit), one chained launch; X may be B
+// (ldx == ldb).  sync: trsm_multi_sync_bytes(n, nrhs) bytes, 256-aligned;
+// its first int is an error word that a bounded wait sets (clear it before,
+// read it after).
+size_t trsm_multi_sync_bytes(long n, long nrhs);
 template <typename T>
 hipError_t trsm_multi(hipStream_t s, char uplo, char trans, long n, long nrhs, const T *A, long lda,
-                      const T *Winv, T *B, long ldb, T *X, long ldx);
+                      const T *Winv, const T *B, long ldb, T *X, long ldx, int *sync);
 
 // ---------------------------------------------------------------------------
 // batched factorisation (batched.hip)

@@ -42,14 +42,23 @@ hipError_t potrs_prepare(hipStream_t s, char uplo, long n, const T *F, long ldf,
   return hc::trsm_multi_inverses<T>(s, uplo, o.t2, n, F, ldf, W + hc::trsm_multi_ws_elems(n));
 }
 
-// B <- inv(A) B for the factor in triangle uplo, n x nrhs; Y: n x nrhs work.
+// B <- inv(A) B for the factor in triangle uplo, n x nrhs; Y: n x nrhs work;
+// sync: trsm_multi_sync_bytes(n, nrhs) bytes.
 template <typename T>
 hipError_t potrs_device(hipStream_t s, char uplo, long n, long nrhs, const T *F, long ldf,
-                        const T *W, T *B, long ldb, T *Y) {
+                        const T *W, T *B, long ldb, T *Y, int *sync) {
   const PotrsOps o = potrs_ops(uplo);
-  HC_TRY(hc::trsm_multi<T>(s, uplo, o.t1, n, nrhs, F, ldf, W, B, ldb, Y, n));
+  HC_TRY(hc::trsm_multi<T>(s, uplo, o.t1, n, nrhs, F, ldf, W, B, ldb, Y, n, sync));
   return hc::trsm_multi<T>(s, uplo, o.t2, n, nrhs, F, ldf, W + hc::trsm_multi_ws_elems(n), Y, n, B,
-                           ldb);
+                           ldb, sync);
+}
+
+// The solves' error word (set when a chained solve abandoned a wait).
+CUresult solve_status(hipStream_t s, const int *sync) {
+  int v = 0;
+  HC_CHECK(hipMemcpyAsync(&v, sync, sizeof(int), hipMemcpyDeviceToHost, s));
+  HC_CHECK(hipStreamSynchronize(s));
+  return v == 0 ? CUDA_SUCCESS : CUDA_ERROR_LAUNCH_FAILED;
 }
 
 // R <- B - A X with A symmetric in triangle uplo: the stored triangle times X
@@ -98,7 +107,7 @@ constexpr long kGmresOuterMax = 10;
 
 // Workspace layout of the mixed solver (byte offsets, 256-aligned).
 struct SposvLayout {
-  size_t S, W, T, Y, R, RS, PART, NORM, DEV, P16, V, WV, HD, total;
+  size_t S, W, T, Y, R, RS, PART, NORM, DEV, P16, V, WV, HD, SYNC, total;
 };
 
 constexpr long kPartElems = 2048 * 128 * 16;  // split-K partial sums of the residual
@@ -121,6 +130,7 @@ SposvLayout sposv_layout(CULAPACKhandle h, long n, long nrhs) {
   L.V = off;    off = align256(off + (size_t)nrhs * (kGmresM + 1) * n * sizeof(double));  // Krylov bases
   L.WV = off;   off = align256(off + (size_t)n * nrhs * sizeof(double));
   L.HD = off;   off = align256(off + 3 * (size_t)nrhs * (kGmresM + 1) * sizeof(double));
+  L.SYNC = off; off = align256(off + hc::trsm_multi_sync_bytes(n, nrhs));
   L.total = off;
   return L;
 }
@@ -147,13 +157,15 @@ CUresult cuDpotrs(CULAPACKhandle h, CBlasUplo uplo, size_t n, size_t nrhs, CUdev
   HC_CHECK(g.status());
   void *ws = nullptr;
   const long N = (long)n, K = (long)nrhs, WI = 2 * hc::trsm_multi_ws_elems(N);
-  HC_CHECK(hc::lapack_workspace(h, (size_t)(WI + N * K) * sizeof(double), &ws));
+  const size_t sync_off = align256((size_t)(WI + N * K) * sizeof(double));
+  HC_CHECK(hc::lapack_workspace(h, sync_off + hc::trsm_multi_sync_bytes(N, K), &ws));
   double *W = static_cast<double *>(ws);
+  int *sync = reinterpret_cast<int *>(static_cast<char *>(ws) + sync_off);
+  HC_CHECK(hipMemsetAsync(sync, 0, sizeof(int), h->stream));
   HC_CHECK(potrs_prepare<double>(h->stream, (char)uplo, N, dptr<double>(A), (long)lda, W));
   HC_CHECK(potrs_device<double>(h->stream, (char)uplo, N, K, dptr<double>(A), (long)lda, W,
-                                dptr<double>(B), (long)ldb, W + WI));
-  HC_CHECK(hipStreamSynchronize(h->stream));
-  return CUDA_SUCCESS;
+                                dptr<double>(B), (long)ldb, W + WI, sync));
+  return solve_status(h->stream, sync);
 }
 
 }  // extern "C"
@@ -195,6 +207,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   double *HD = reinterpret_cast<double *>(wsb + L.HD);  // dots (2 sets) and scales
   constexpr int NV = kGmresM + 1;
   double *R = reinterpret_cast<double *>(wsb + L.R);
+  int *sync = reinterpret_cast<int *>(wsb + L.SYNC);
   float *RS = reinterpret_cast<float *>(wsb + L.RS);
   double *NW = reinterpret_cast<double *>(wsb + L.NORM);
   double *DEV = reinterpret_cast<double *>(wsb + L.DEV);
@@ -257,13 +270,13 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     CU_TRY(read_flag(&flag));
     if (flag) { *code = -2; return CUDA_SUCCESS; }
     HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
-                                 static_cast<float *>(Yw)));
+                                 static_cast<float *>(Yw), sync));
     HC_CHECK(hc::s2d(s, N, K, RS, N, Xd, (long)ldx));
     // w <- inv(S S^T) w in FP32 (w: n x nrhs FP64, ld n)
     auto precondition = [&](double *w) -> CUresult {
       HC_CHECK(hc::d2s(s, N, K, w, N, RS, N, ovf));
       HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
-                                   static_cast<float *>(Yw)));
+                                   static_cast<float *>(Yw), sync));
       HC_CHECK(hc::s2d(s, N, K, RS, N, w, N));
       return CUDA_SUCCESS;
     };
@@ -374,11 +387,12 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
       CU_TRY(read_flag(&flag));
       if (flag) { *code = -2; return CUDA_SUCCESS; }
       HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
-                                   static_cast<float *>(Yw)));
+                                   static_cast<float *>(Yw), sync));
       HC_CHECK(hc::add_s2d(s, N, K, RS, N, Xd, (long)ldx));
     }
   };
 
+  HC_CHECK(hipMemsetAsync(sync, 0, sizeof(int), s));
   long code = 0;
   CUresult r = attempt(factor == 'B', &code);
   if (r != CUDA_SUCCESS) return r;
@@ -388,7 +402,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     if (used != nullptr) *used = 'S';
   }
   *iter = code;
-  if (code >= 0) return CUDA_SUCCESS;
+  if (code >= 0) return solve_status(s, sync);
 
   // Full FP64 factorisation and solve; A is overwritten by its factor.
   if (used != nullptr) *used = 'D';
@@ -404,9 +418,8 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   HC_CHECK(hc::copy2d<double>(s, N, K, Bd, (long)ldb, Xd, (long)ldx));
   HC_CHECK(potrs_prepare<double>(s, ul, N, Aw, (long)lda, static_cast<double *>(T)));
   HC_CHECK(potrs_device<double>(s, ul, N, K, Aw, (long)lda, static_cast<const double *>(T), Xd,
-                                (long)ldx, static_cast<double *>(Yw)));
-  HC_CHECK(hipStreamSynchronize(s));
-  return CUDA_SUCCESS;
+                                (long)ldx, static_cast<double *>(Yw), sync));
+  return solve_status(s, sync);
 }
 
 }  // namespace

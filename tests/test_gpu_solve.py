@@ -164,18 +164,23 @@ def normwise_backward(A, X, B):
 
 
 @pytest.mark.parametrize("uplo", "LU")
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (65, 3), (300, 16), (1000, 5), (700, 40)])
-def test_dpotrs(gpu, lapack, uplo, n, nrhs):
+@pytest.mark.parametrize("n,nrhs,pad", [(1, 1, 0), (65, 3, 0), (300, 16, 0), (1000, 5, 0), (700, 40, 0),
+                                        (3000, 16, 0), (2049, 17, 1), (1024, 2, 1)])
+def test_dpotrs(gpu, lapack, uplo, n, nrhs, pad):
+    """pad: lda = n + pad (an odd lda takes the element-wise load path of the chained solve)."""
     A = O.spd(n + 3, n, 0.5)
     B = np.asfortranarray(rng(4, n, nrhs).standard_normal((n, nrhs)))
-    dA = gpu.to_device(A)
-    st, info = gpu.cuDpotrf(lapack, ord(uplo), n, gpu.ptr(dA), n)
+    lda = n + pad
+    Ap = np.full((lda, n), np.nan, order="F")
+    Ap[:n] = A
+    dA = gpu.to_device(Ap)
+    st, info = gpu.cuDpotrf(lapack, ord(uplo), n, gpu.ptr(dA), lda)
     assert st == 0 and info == 0
     ldb = n + 2
     Bp = np.full((ldb, nrhs), 5.0, order="F")
     Bp[:n] = B
     dB = gpu.to_device(Bp)
-    st, info = gpu.cuDpotrs(lapack, ord(uplo), n, nrhs, gpu.ptr(dA), n, gpu.ptr(dB), ldb)
+    st, info = gpu.cuDpotrs(lapack, ord(uplo), n, nrhs, gpu.ptr(dA), lda, gpu.ptr(dB), ldb)
     torch.cuda.synchronize()
     assert st == 0 and info == 0
     Xg = gpu.from_device(dB)
