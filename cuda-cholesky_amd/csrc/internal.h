@@ -204,6 +204,17 @@ hipError_t sym_norm_inf(hipStream_t s, char uplo, long n, const double *A, long 
                         double *out);
 
 // ---------------------------------------------------------------------------
+// symmetric matrix times a few right-hand sides (symm.hip)
+// ---------------------------------------------------------------------------
+// R = beta C + alpha A X, A symmetric FP64 in triangle uplo (read once), X and
+// R n x nrhs; ws: symm_tall_ws_bytes(n, nrhs) bytes.  C is not read when
+// beta == 0; R may alias X.
+size_t symm_tall_ws_bytes(long n, long nrhs);
+hipError_t symm_tall(hipStream_t s, char uplo, long n, long nrhs, double alpha, const double *A,
+                     long lda, const double *X, long ldx, double beta, const double *C, long ldc,
+                     double *R, long ldr, void *ws);
+
+// ---------------------------------------------------------------------------
 // triangular solve with few right-hand sides (trsm_multi.hip)
 // ---------------------------------------------------------------------------
 // Inverses of the 128 x 128 diagonal blocks of op(A) (uplo, trans), for

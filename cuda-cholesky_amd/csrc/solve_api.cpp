@@ -61,43 +61,17 @@ CUresult solve_status(hipStream_t s, const int *sync) {
   return v == 0 ? CUDA_SUCCESS : CUDA_ERROR_LAUNCH_FAILED;
 }
 
-// R <- B - A X with A symmetric in triangle uplo: the stored triangle times X
-// plus the transpose of its strict part times X (two masked GEMMs).
+// R <- B - A X with A symmetric in triangle uplo (the triangle read once).
 hipError_t residual(hipStream_t s, char uplo, long n, long nrhs, const double *A, long lda,
                     const double *B, long ldb, const double *X, long ldx, double *R, long ldr,
-                    double *part, long part_elems) {
-  HC_TRY(hc::copy2d<double>(s, n, nrhs, B, ldb, R, ldr));
-  hc::GemmOpts o1;
-  o1.splitk_ws = part;
-  o1.splitk_elems = part_elems;
-  o1.tri_a = (uplo == 'L') ? hc::TRI_LOWER : hc::TRI_UPPER;
-  HC_TRY(hc::gemm<double>(s, hc::OP_N, hc::OP_N, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, ldr, R,
-                          ldr, o1));
-  hc::GemmOpts o2;  // op(A) = A^T is upper-like (lower storage) / lower-like (upper)
-  o2.splitk_ws = part;
-  o2.splitk_elems = part_elems;
-  o2.tri_a = (uplo == 'L') ? hc::TRI_UPPER : hc::TRI_LOWER;
-  o2.zero_diag = true;
-  return hc::gemm<double>(s, hc::OP_T, hc::OP_N, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, ldr, R,
-                          ldr, o2);
+                    void *ws) {
+  return hc::symm_tall(s, uplo, n, nrhs, -1.0, A, lda, X, ldx, 1.0, B, ldb, R, ldr, ws);
 }
 
-// W <- A X with A symmetric in triangle uplo (same two masked products).
+// W <- A X with A symmetric in triangle uplo.
 hipError_t symm_apply(hipStream_t s, char uplo, long n, long nrhs, const double *A, long lda,
-                      const double *X, long ldx, double *W, long ldw, double *part, long part_elems) {
-  hc::GemmOpts o1;
-  o1.splitk_ws = part;
-  o1.splitk_elems = part_elems;
-  o1.tri_a = (uplo == 'L') ? hc::TRI_LOWER : hc::TRI_UPPER;
-  HC_TRY(hc::gemm<double>(s, hc::OP_N, hc::OP_N, n, nrhs, n, 1.0, A, lda, X, ldx, 0.0, nullptr, 0, W,
-                          ldw, o1));
-  hc::GemmOpts o2;
-  o2.splitk_ws = part;
-  o2.splitk_elems = part_elems;
-  o2.tri_a = (uplo == 'L') ? hc::TRI_UPPER : hc::TRI_LOWER;
-  o2.zero_diag = true;
-  return hc::gemm<double>(s, hc::OP_T, hc::OP_N, n, nrhs, n, 1.0, A, lda, X, ldx, 1.0, W, ldw, W, ldw,
-                          o2);
+                      const double *X, long ldx, double *W, long ldw, void *ws) {
+  return hc::symm_tall(s, uplo, n, nrhs, 1.0, A, lda, X, ldx, 0.0, nullptr, 0, W, ldw, ws);
 }
 
 // GMRES-based refinement: restart length and inner relative tolerance.
@@ -110,7 +84,6 @@ struct SposvLayout {
   size_t S, W, T, Y, R, RS, PART, NORM, DEV, P16, V, WV, HD, SYNC, total;
 };
 
-constexpr long kPartElems = 2048 * 128 * 16;  // split-K partial sums of the residual
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -123,7 +96,7 @@ SposvLayout sposv_layout(CULAPACKhandle h, long n, long nrhs) {
   L.Y = off;    off = align256(off + (size_t)n * nrhs * sizeof(double));          // solve work
   L.R = off;    off = align256(off + (size_t)n * nrhs * sizeof(double));          // residual
   L.RS = off;   off = align256(off + (size_t)n * nrhs * sizeof(float));           // FP32 rhs
-  L.PART = off; off = align256(off + (size_t)kPartElems * sizeof(double));
+  L.PART = off; off = align256(off + hc::symm_tall_ws_bytes(n, nrhs));  // symmetric products
   L.NORM = off; off = align256(off + (size_t)hc::sym_norm_ws_doubles(n) * sizeof(double));
   L.DEV = off;  off = align256(off + (3 * (size_t)nrhs + 8) * sizeof(double));    // norms, flags
   L.P16 = off;  off = align256(off + hc::potrf_bf16_ws_bytes(h, n));                // bf16 panels
@@ -315,8 +288,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
       HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, HD + 2 * K * NV, V));
       int used_j = 0;
       for (int j = 0; j < kGmresM; j++) {
-        CU_TRY(symm_apply(s, ul, N, K, Ad, (long)lda, V + (long)j * N, (long)NV * N, WV, N, PART,
-                          kPartElems) == hipSuccess ? CUDA_SUCCESS : CUDA_ERROR_UNKNOWN);
+        HC_CHECK(symm_apply(s, ul, N, K, Ad, (long)lda, V + (long)j * N, (long)NV * N, WV, N, PART));
         CU_TRY(precondition(WV));
         for (int pass = 0; pass < 2; pass++) {
           double *hd = HD + pass * K * NV;
@@ -374,8 +346,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     Norms nm;
     const long itmax = refine == 'G' ? kGmresOuterMax : kIterMax;
     for (long it = 0;; it++) {
-      HC_CHECK(residual(s, ul, N, K, Ad, (long)lda, Bd, (long)ldb, Xd, (long)ldx, R, N, PART,
-                        kPartElems));
+      HC_CHECK(residual(s, ul, N, K, Ad, (long)lda, Bd, (long)ldb, Xd, (long)ldx, R, N, PART));
       CU_TRY(read_norms(nm));
       if (converged(nm)) { *code = it; return CUDA_SUCCESS; }
       if (it == itmax) { *code = -(itmax + 1); return CUDA_SUCCESS; }
