@@ -354,60 +354,89 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   const int lc = lane & 15, lg = lane >> 4;
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
 
+  // With alpha = +-1 the accumulators start from beta C (loaded at the start,
+  // so the C read overlaps the whole K loop) and the sign of alpha goes into
+  // the A fragments; otherwise C is combined in the epilogue.
+  const bool seed = (p.alpha == 1.0f || p.alpha == -1.0f);
+  const bool use_beta = p.beta != 0.0f;
   v4f acc[BI][BJ];
+  // C loads from clamped (valid) addresses with the condition outside the
+  // loops, so that all of them are in flight together
+  auto load_c = [&](bool unit, float scale) __attribute__((always_inline)) {
 #pragma unroll
-  for (int i = 0; i < BI; i++)
+    for (int i = 0; i < BI; i++) {
+      const int gm = min(m0 + wm0 + 16 * i + lc, p.m - 1);
 #pragma unroll
-    for (int j = 0; j < BJ; j++) acc[i][j] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+      for (int j = 0; j < BJ; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int gn = min(n0 + wn0 + 16 * j + 4 * lg + r, p.n - 1);
+          const float c = p.C[(long)gn * p.ldc + gm];
+          acc[i][j][r] = unit ? c : scale * c;
+        }
+    }
+  };
+  // issued after the first operand loads, consumed by the first products
+  auto seed_acc = [&]() __attribute__((always_inline)) {
+    if (seed && use_beta) {
+      if (p.beta == 1.0f) load_c(true, 1.0f);
+      else load_c(false, p.beta);
+    } else {
+#pragma unroll
+      for (int i = 0; i < BI; i++)
+#pragma unroll
+        for (int j = 0; j < BJ; j++) acc[i][j] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+    }
+  };
+  const unsigned sgn = p.alpha < 0.0f ? 0x80008000u : 0u;  // bf16 sign bits of a fragment
 
   const int kend = p.alpha == 0.0f ? 0 : p.k;
   if (kend > 0) {
-    // chunk q of a thread: row (tid + q 256) / 8, k offset 8 ((tid + q 256) % 8)
-    v4u ra[CA], rb[CB];
-    auto load = [&](int kb) {
+    // chunk q of a thread: row (tid + q 256) / 8, k offset 8 ((tid + q 256) % 8).
+    // Two register sets: the loads of K step kt + 2 are issued while step kt
+    // is computed and step kt + 1 waits in registers for its LDS buffer.
+    v4u ra[2][CA], rb[2][CB];
+    auto load = [&](int set, int kb) __attribute__((always_inline)) {
 #pragma unroll
       for (int q = 0; q < CA; q++) {
         const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gr = m0 + row;
-        const v4u *src = reinterpret_cast<const v4u *>(p.A + (long)(gr < p.m ? gr : 0) * p.lda + kb + kc);
-        const v4u v = *src;
-        ra[q] = gr < p.m ? v : (v4u){0u, 0u, 0u, 0u};
+        const v4u *src = reinterpret_cast<const v4u *>(p.A + (long)min(gr, p.m - 1) * p.lda + kb + kc);
+        const unsigned msk = gr < p.m ? 0xffffffffu : 0u;  // rows past m read as zero
+        ra[set][q] = *src & (v4u){msk, msk, msk, msk};
       }
 #pragma unroll
       for (int q = 0; q < CB; q++) {
         const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gc = n0 + row;
-        const v4u *src = reinterpret_cast<const v4u *>(p.B + (long)(gc < p.n ? gc : 0) * p.ldb + kb + kc);
-        const v4u v = *src;
-        rb[q] = gc < p.n ? v : (v4u){0u, 0u, 0u, 0u};
+        const v4u *src = reinterpret_cast<const v4u *>(p.B + (long)min(gc, p.n - 1) * p.ldb + kb + kc);
+        const unsigned msk = gc < p.n ? 0xffffffffu : 0u;
+        rb[set][q] = *src & (v4u){msk, msk, msk, msk};
       }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int set, int buf) __attribute__((always_inline)) {
 #pragma unroll
       for (int q = 0; q < CA; q++) {
         const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
-        *reinterpret_cast<v4u *>(&sA[buf][row * BLK + kc]) = ra[q];
+        *reinterpret_cast<v4u *>(&sA[buf][row * BLK + kc]) = ra[set][q];
       }
 #pragma unroll
       for (int q = 0; q < CB; q++) {
         const int e = tid + q * NTHREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
-        *reinterpret_cast<v4u *>(&sB[buf][row * BLK + kc]) = rb[q];
+        *reinterpret_cast<v4u *>(&sB[buf][row * BLK + kc]) = rb[set][q];
       }
     };
-    const int ntiles = kend / BK;
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int kt = 0; kt < ntiles; kt++) {
-      const bool more = kt + 1 < ntiles;
-      if (more) load((kt + 1) * BK);
-      const u16 *cA = sA[kt & 1], *cB = sB[kt & 1];
+    auto compute = [&](int buf) __attribute__((always_inline)) {
+      const u16 *cA = sA[buf], *cB = sB[buf];
 #pragma unroll
       for (int h = 0; h < BK; h += 32) {
         v8bf fa[BI], fb[BJ];
 #pragma unroll
-        for (int i = 0; i < BI; i++)
-          fa[i] = *reinterpret_cast<const v8bf *>(&cA[(wm0 + 16 * i + lc) * BLK + h + 8 * lg]);
+        for (int i = 0; i < BI; i++) {
+          v4u x = *reinterpret_cast<const v4u *>(&cA[(wm0 + 16 * i + lc) * BLK + h + 8 * lg]);
+          x ^= (v4u){sgn, sgn, sgn, sgn};
+          fa[i] = __builtin_bit_cast(v8bf, x);
+        }
 #pragma unroll
         for (int j = 0; j < BJ; j++)
           fb[j] = *reinterpret_cast<const v8bf *>(&cB[(wn0 + 16 * j + lc) * BLK + h + 8 * lg]);
@@ -417,12 +446,49 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
           for (int j = 0; j < BJ; j++)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
       }
-      if (more) store((kt + 1) & 1);
+    };
+    const int ntiles = kend / BK;
+    load(0, 0);
+    load(1, ntiles > 1 ? BK : 0);  // a repeat of step 0 when there is no step 1 (never stored)
+    seed_acc();
+    store(0, 0);
+    __syncthreads();
+    // step kt: compute buffer kt & 1 (set kt & 1 is free: it was stored);
+    // set (kt + 1) & 1 holds step kt + 1.  Two steps per pass keep the set
+    // and buffer indices static.
+    for (int kt = 0; kt < ntiles; kt += 2) {
+      if (kt + 2 < ntiles) load(0, (kt + 2) * BK);
+      compute(0);
+      if (kt + 1 < ntiles) store(1, 1);
+      __syncthreads();
+      if (kt + 1 >= ntiles) break;
+      if (kt + 3 < ntiles) load(1, (kt + 3) * BK);
+      compute(1);
+      if (kt + 2 < ntiles) store(0, 0);
       __syncthreads();
     }
+  } else {
+    seed_acc();
   }
 
-  const bool use_beta = p.beta != 0.0f;
+  if (!seed) {  // D = alpha acc + beta C: scale, then add C loaded in one batch
+#pragma unroll
+    for (int i = 0; i < BI; i++)
+#pragma unroll
+      for (int j = 0; j < BJ; j++) acc[i][j] *= p.alpha;
+    if (use_beta) {
+      v4f prod[BI][BJ];
+#pragma unroll
+      for (int i = 0; i < BI; i++)
+#pragma unroll
+        for (int j = 0; j < BJ; j++) prod[i][j] = acc[i][j];
+      load_c(false, p.beta);
+#pragma unroll
+      for (int i = 0; i < BI; i++)
+#pragma unroll
+        for (int j = 0; j < BJ; j++) acc[i][j] += prod[i][j];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < BI; i++) {
     const int gm = m0 + wm0 + 16 * i + lc;
@@ -435,9 +501,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
         if (gn >= p.n) continue;
         if (p.tri_out == TRI_LOWER && gm < gn) continue;
         if (p.tri_out == TRI_UPPER && gm > gn) continue;
-        float v = p.alpha * acc[i][j][r];
-        if (use_beta) v += p.beta * p.C[(long)gn * p.ldc + gm];
-        p.D[(long)gn * p.ldd + gm] = v;
+        p.D[(long)gn * p.ldd + gm] = acc[i][j][r];
       }
     }
   }
