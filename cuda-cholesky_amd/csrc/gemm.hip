@@ -380,6 +380,20 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   const bool use_beta = !is_zero(p.beta);
 #pragma unroll
   for (int i = 0; i < BI; i++) {
+    // C of the blocks not pre-added, one batch per block row (clamped, always
+    // valid addresses) so that its loads are in flight together
+    T cv[BJ][4];
+    const bool need_c = use_beta && (i + 1) * BJ > npre;  // uniform
+    if (need_c) {
+      const int gmc = min(m0 + wm0 + 16 * i + lc, p.m - 1);
+#pragma unroll
+      for (int j = 0; j < BJ; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int gnc = min(n0 + wn0 + 16 * j + lg + 4 * r, p.n - 1);
+          cv[j][r] = p.C[(long)gnc * p.ldc + gmc];
+        }
+    }
     const int gm = m0 + wm0 + 16 * i + lc;
     if (gm >= p.m) continue;
 #pragma unroll
@@ -394,7 +408,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
         if constexpr (CPLX) v = make_zd(acc[0][i][j][r], acc[1][i][j][r]);
         else v = acc[0][i][j][r];
         v = p.alpha * v;
-        if (use_beta && i * BJ + j >= npre) v = v + p.beta * p.C[(long)gn * p.ldc + gm];
+        if (need_c && i * BJ + j >= npre) v = v + p.beta * cv[j][r];
         if constexpr (CPLX) {
           if (p.real_diag && gm == gn) v.y = 0.0;
         }

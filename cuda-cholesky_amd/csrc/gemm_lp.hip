@@ -220,8 +220,26 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_lp_kernel(LpParams p) {
   }
 
   // ---- epilogue: lane lc holds row m0 + wm0 + 16 i + lc; register r of lane
-  // group lg holds column n0 + wn0 + 16 j + 4 lg + r ----
-  const bool use_beta = p.beta != 0.0f;
+  // group lg holds column n0 + wn0 + 16 j + 4 lg + r.  With beta != 0, C is
+  // read in one batch from clamped (valid) addresses, so that all its loads
+  // are in flight together, then combined ----
+#pragma unroll
+  for (int i = 0; i < BI; i++)
+#pragma unroll
+    for (int j = 0; j < BJ; j++) acc[i][j] *= p.alpha;
+  if (p.beta != 0.0f) {
+#pragma unroll
+    for (int i = 0; i < BI; i++) {
+      const int gm = min(m0 + wm0 + 16 * i + lc, p.m - 1);
+#pragma unroll
+      for (int j = 0; j < BJ; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int gn = min(n0 + wn0 + 16 * j + 4 * lg + r, p.n - 1);
+          acc[i][j][r] += p.beta * p.C[(long)gn * p.ldc + gm];
+        }
+    }
+  }
 #pragma unroll
   for (int i = 0; i < BI; i++) {
     const int gm = m0 + wm0 + 16 * i + lc;
@@ -234,9 +252,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_lp_kernel(LpParams p) {
         if (gn >= p.n) continue;
         if (p.tri_out == TRI_LOWER && gm < gn) continue;
         if (p.tri_out == TRI_UPPER && gm > gn) continue;
-        float v = p.alpha * acc[i][j][r];
-        if (use_beta) v += p.beta * p.C[(long)gn * p.ldc + gm];
-        p.D[(long)gn * p.ldd + gm] = v;
+        p.D[(long)gn * p.ldd + gm] = acc[i][j][r];
       }
     }
   }
