@@ -106,6 +106,7 @@ SIGNATURES = {
     "cuPotrfBatchedPlanCreate": [_h, _enum, ctypes.c_char, _sz, ctypes.POINTER(ctypes.c_size_t),
                                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_h)],
     "cuPotrfBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
+    "cuPotriBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
     "cuPotrfBatchedPlanDestroy": [_h],
     "cuLAPACKSetTiming": [_h, ctypes.c_int],
     "cuLAPACKSetBlockSize": [_h, ctypes.c_char, ctypes.c_long],
@@ -276,6 +277,10 @@ class BatchedPlan:
 
     def run(self, A, logdet, info, stream=None):
         return lib().cuPotrfBatchedRun(self.h, A, logdet, info, stream)
+
+    def run_inverse(self, A, logdet, info, stream=None):
+        """cuPotriBatchedRun: factor, logdet, then A_i <- inv(A_i) in the same launch."""
+        return lib().cuPotriBatchedRun(self.h, A, logdet, info, stream)
 
     def close(self):
         if self.h:

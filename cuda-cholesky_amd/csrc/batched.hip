@@ -12,6 +12,14 @@
 //     (written to the matrix and kept in LDS);
 //   * all waves update the trailing lower tiles A_IJ -= L_Ip L_Jp^T with
 //     operands from LDS.
+// With `inverse`, the same workgroup then overwrites the factor with A^-1:
+//   * inv(L) in place, leaves right to left: X_pp = inv(L_pp) (wave 0,
+//     registers), X_Ip = -(sum_{p<K<=I} X_IK L_Kp) X_pp for the tiles below
+//     (all Y_I = sum X_IK L_Kp formed into LDS before column p is
+//     overwritten);
+//   * A^-1 = inv(L)^T inv(L), tile rows top-down: row i of the result is
+//     sum_{K>=i} X_Ki^T X_KC (C <= i), formed into LDS, then written; later
+//     rows never read row i again.
 // Storage is FP64 or FP32 (ST); arithmetic is FP64 throughout.  Only the
 // referenced triangle (uplo) is read or written; tiles reaching past n read
 // the identity and are not written back.
@@ -47,7 +55,7 @@ struct View {
   }
 };
 
-template <typename ST, bool LOWER>
+template <typename ST, bool LOWER, bool INVERSE>
 __global__ void __launch_bounds__(BT_THREADS)
 potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *order, ST *A,
                      double *logdet, int *info) {
@@ -144,6 +152,81 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
       }
       __syncthreads();
     }
+    if (INVERSE && bad < 0) {
+      // ---- inv(L) in place, leaves right to left ----
+      for (int p = np - 1; p >= 0; p--) {
+        const int g0 = p * LEAF;
+        if (wave == 0) {
+          double a[4], x[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) a[j] = v.get(g0 + li, g0 + lk + 4 * j);
+          t16::leaf_inverse<double>(a, x);
+#pragma unroll
+          for (int j = 0; j < 4; j++) Li[(lk + 4 * j) * LL + li] = x[j];
+        }
+        // Y_I = sum_{K=p+1}^{I} X_IK L_Kp into LDS (L_Kp still the factor)
+        for (int I = p + 1 + wave; I < np; I += BT_NW) {
+          TileAcc<double> acc;
+          t16::acc_zero(acc);
+          for (int K = p + 1; K <= I; K++) {
+#pragma unroll
+            for (int k0 = 0; k0 < LEAF; k0 += 4) {
+              const double x = v.get(I * LEAF + li, K * LEAF + k0 + lk);
+              const double y = v.get(K * LEAF + k0 + lk, g0 + li);
+              acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+            }
+          }
+          double *Pt = P + I * LEAF * LL;
+#pragma unroll
+          for (int r = 0; r < 4; r++) Pt[li * LL + lk + 4 * r] = acc.re[r];
+        }
+        __syncthreads();
+        // X_Ip = -Y_I X_pp, then X_pp itself
+        for (int I = p + 1 + wave; I < np; I += BT_NW) {
+          TileAcc<double> acc;
+          t16::acc_zero(acc);
+          const double *Pt = P + I * LEAF * LL;
+#pragma unroll
+          for (int k0 = 0; k0 < LEAF; k0 += 4) {
+            const double x = -Pt[(k0 + lk) * LL + li];
+            const double y = Li[li * LL + k0 + lk];
+            acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++) v.put(I * LEAF + lk + 4 * r, g0 + li, acc.re[r]);
+        }
+        if (wave == 0) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) v.put(g0 + li, g0 + lk + 4 * j, Li[(lk + 4 * j) * LL + li]);
+        }
+        __syncthreads();
+      }
+      // ---- A^-1 = X^T X, tile rows top-down ----
+      for (int i = 0; i < np; i++) {
+        for (int C = wave; C <= i; C += BT_NW) {
+          TileAcc<double> acc;
+          t16::acc_zero(acc);
+          for (int K = i; K < np; K++) {
+#pragma unroll
+            for (int k0 = 0; k0 < LEAF; k0 += 4) {
+              const double x = v.get(K * LEAF + k0 + lk, i * LEAF + li);  // X_Ki^T
+              const double y = v.get(K * LEAF + k0 + lk, C * LEAF + li);  // X_KC
+              acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+            }
+          }
+          double *Pt = P + C * LEAF * LL;
+#pragma unroll
+          for (int r = 0; r < 4; r++) Pt[li * LL + lk + 4 * r] = acc.re[r];
+        }
+        __syncthreads();
+        for (int C = wave; C <= i; C += BT_NW) {
+          const double *Pt = P + C * LEAF * LL;
+#pragma unroll
+          for (int r = 0; r < 4; r++) v.put(i * LEAF + lk + 4 * r, C * LEAF + li, Pt[li * LL + lk + 4 * r]);
+        }
+        __syncthreads();
+      }
+    }
     if (tid == 0) {
       info[idx] = bad >= 0 ? bad + 1 : 0;
       if (logdet != nullptr) logdet[idx] = bad >= 0 ? __builtin_nan("") : logsum;
@@ -156,33 +239,50 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
 
 template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,
-                         const int *order, ST *A, double *logdet, int *info, int grid) {
+                         const int *order, ST *A, double *logdet, int *info, int grid, bool inverse) {
   if (count <= 0) return hipSuccess;
   if (grid < 1) grid = 1;
-  if (uplo == 'L')
-    hipLaunchKernelGGL((potrf_batched_kernel<ST, true>), dim3((unsigned)grid), dim3(BT_THREADS), 0, s,
-                       count, nvec, offs, order, A, logdet, info);
-  else
-    hipLaunchKernelGGL((potrf_batched_kernel<ST, false>), dim3((unsigned)grid), dim3(BT_THREADS), 0,
-                       s, count, nvec, offs, order, A, logdet, info);
+  const dim3 g((unsigned)grid), b(BT_THREADS);
+  if (uplo == 'L') {
+    if (inverse)
+      hipLaunchKernelGGL((potrf_batched_kernel<ST, true, true>), g, b, 0, s, count, nvec, offs, order, A,
+                         logdet, info);
+    else
+      hipLaunchKernelGGL((potrf_batched_kernel<ST, true, false>), g, b, 0, s, count, nvec, offs, order, A,
+                         logdet, info);
+  } else {
+    if (inverse)
+      hipLaunchKernelGGL((potrf_batched_kernel<ST, false, true>), g, b, 0, s, count, nvec, offs, order, A,
+                         logdet, info);
+    else
+      hipLaunchKernelGGL((potrf_batched_kernel<ST, false, false>), g, b, 0, s, count, nvec, offs, order,
+                         A, logdet, info);
+  }
   return hipGetLastError();
 }
 
 template <typename ST>
-int potrf_batched_blocks_per_cu(char uplo) {
+int potrf_batched_blocks_per_cu(char uplo, bool inverse) {
   int nb = 0;
-  const hipError_t e =
-      uplo == 'L'
-          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, true>, BT_THREADS, 0)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, false>, BT_THREADS, 0);
+  hipError_t e;
+  if (uplo == 'L')
+    e = inverse ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, true, true>,
+                                                               BT_THREADS, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, true, false>,
+                                                               BT_THREADS, 0);
+  else
+    e = inverse ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, false, true>,
+                                                               BT_THREADS, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, potrf_batched_kernel<ST, false, false>,
+                                                               BT_THREADS, 0);
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-template int potrf_batched_blocks_per_cu<double>(char);
-template int potrf_batched_blocks_per_cu<float>(char);
+template int potrf_batched_blocks_per_cu<double>(char, bool);
+template int potrf_batched_blocks_per_cu<float>(char, bool);
 template hipError_t potrf_batched<double>(hipStream_t, char, int, const int *, const long *,
-                                          const int *, double *, double *, int *, int);
+                                          const int *, double *, double *, int *, int, bool);
 template hipError_t potrf_batched<float>(hipStream_t, char, int, const int *, const long *,
-                                         const int *, float *, double *, int *, int);
+                                         const int *, float *, double *, int *, int, bool);
 
 }  // namespace hc

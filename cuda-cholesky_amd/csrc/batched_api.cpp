@@ -1,8 +1,8 @@
-// Batched Cholesky + log-determinant of many small SPD matrices of uneven
-// sizes (no counterpart in the reference; its routines take one matrix per
-// call).  A plan fixes the sizes, offsets and schedule once; each run is one
-// persistent kernel, captured in a HIP graph on first use and replayed with a
-// single graph launch.
+// Batched Cholesky + log-determinant (+ inverse) of many small SPD matrices
+// of uneven sizes (no counterpart in the reference; its routines take one
+// matrix per call).  A plan fixes the sizes, offsets and schedule once; each
+// run is one persistent kernel, captured in a HIP graph on first use and
+// replayed with a single graph launch.
 #include "api_util.h"
 
 #include <algorithm>
@@ -19,28 +19,31 @@ struct __cubatchedplan_st {
   char uplo;
   char precision;  // 'D' (FP64 storage) or 'S' (FP32 storage)
   int count;
-  int grid;
+  int grid, grid_inverse;  // persistent grid of the factor / factor + inverse kernel
   int *d_n;
   long *d_off;
   int *d_order;
-  // graph for the last (A, logdet, info) triple
+  // graph for the last (A, logdet, info, inverse) run
   hipStream_t capture;
   hipGraph_t graph;
   hipGraphExec_t exec;
   CUdeviceptr gA, gLog, gInfo;
+  bool gInverse;
 };
 
 namespace {
 
 hipError_t launch(const __cubatchedplan_st *p, hipStream_t s, CUdeviceptr A, CUdeviceptr logdet,
-                  CUdeviceptr info) {
+                  CUdeviceptr info, bool inverse) {
   double *ld = reinterpret_cast<double *>(logdet);
   int *inf = reinterpret_cast<int *>(info);
   if (p->precision == 'D')
     return hc::potrf_batched<double>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
-                                     reinterpret_cast<double *>(A), ld, inf, p->grid);
+                                     reinterpret_cast<double *>(A), ld, inf,
+                                     inverse ? p->grid_inverse : p->grid, inverse);
   return hc::potrf_batched<float>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
-                                  reinterpret_cast<float *>(A), ld, inf, p->grid);
+                                  reinterpret_cast<float *>(A), ld, inf,
+                                  inverse ? p->grid_inverse : p->grid, inverse);
 }
 
 void release_graph(__cubatchedplan_st *p) {
@@ -48,6 +51,36 @@ void release_graph(__cubatchedplan_st *p) {
   if (p->graph != nullptr) (void)hipGraphDestroy(p->graph);
   p->exec = nullptr;
   p->graph = nullptr;
+}
+
+// One graph launch of the batch; the graph is re-captured when the buffers
+// or the operation change.
+CUresult run(const char *name, CUbatchedplan p, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
+             CUstream stream, bool inverse) {
+  if (p == nullptr) return bad_arg(name, 1);
+  if (p->count > 0 && A == 0) return bad_arg(name, 2);
+  if (p->count > 0 && info == 0) return bad_arg(name, 4);
+  if (p->count == 0) return CUDA_SUCCESS;
+  hc::DeviceGuard g(p->device);
+  HC_CHECK(g.status());
+  if (p->exec == nullptr || p->gA != A || p->gLog != logdet || p->gInfo != info ||
+      p->gInverse != inverse) {
+    release_graph(p);
+    HC_CHECK(hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = launch(p, p->capture, A, logdet, info, inverse);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(p->capture, &graph);
+    HC_CHECK(e);
+    HC_CHECK(e2);
+    p->graph = graph;
+    HC_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    p->gA = A;
+    p->gLog = logdet;
+    p->gInfo = info;
+    p->gInverse = inverse;
+  }
+  HC_CHECK(hipGraphLaunch(p->exec, stream));
+  return CUDA_SUCCESS;
 }
 
 }  // namespace
@@ -88,9 +121,12 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
   // workgroup that waits for a slot would start its share late)
   int cus = 0;
   HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-  const int per_cu = precision == 'D' ? hc::potrf_batched_blocks_per_cu<double>((char)uplo)
-                                      : hc::potrf_batched_blocks_per_cu<float>((char)uplo);
-  p->grid = std::max(1, std::min<int>((int)count, cus * per_cu));
+  auto per_cu = [&](bool inverse) {
+    return precision == 'D' ? hc::potrf_batched_blocks_per_cu<double>((char)uplo, inverse)
+                            : hc::potrf_batched_blocks_per_cu<float>((char)uplo, inverse);
+  };
+  p->grid = std::max(1, std::min<int>((int)count, cus * per_cu(false)));
+  p->grid_inverse = std::max(1, std::min<int>((int)count, cus * per_cu(true)));
   if (count > 0) {
     HC_CHECK(hipMalloc(&p->d_n, count * sizeof(int)));
     HC_CHECK(hipMalloc(&p->d_off, count * sizeof(long)));
@@ -106,29 +142,12 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
 
 CUresult cuPotrfBatchedRun(CUbatchedplan p, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
                            CUstream stream) {
-  const char *name = "cuPotrfBatchedRun";
-  if (p == nullptr) return bad_arg(name, 1);
-  if (p->count > 0 && A == 0) return bad_arg(name, 2);
-  if (p->count > 0 && info == 0) return bad_arg(name, 4);
-  if (p->count == 0) return CUDA_SUCCESS;
-  hc::DeviceGuard g(p->device);
-  HC_CHECK(g.status());
-  if (p->exec == nullptr || p->gA != A || p->gLog != logdet || p->gInfo != info) {
-    release_graph(p);
-    HC_CHECK(hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal));
-    const hipError_t e = launch(p, p->capture, A, logdet, info);
-    hipGraph_t graph = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(p->capture, &graph);
-    HC_CHECK(e);
-    HC_CHECK(e2);
-    p->graph = graph;
-    HC_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-    p->gA = A;
-    p->gLog = logdet;
-    p->gInfo = info;
-  }
-  HC_CHECK(hipGraphLaunch(p->exec, stream));
-  return CUDA_SUCCESS;
+  return run("cuPotrfBatchedRun", p, A, logdet, info, stream, false);
+}
+
+CUresult cuPotriBatchedRun(CUbatchedplan p, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
+                           CUstream stream) {
+  return run("cuPotriBatchedRun", p, A, logdet, info, stream, true);
 }
 
 CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan p) {

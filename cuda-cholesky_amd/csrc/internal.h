@@ -237,13 +237,14 @@ hipError_t trsm_multi(hipStream_t s, char uplo, char trans, long n, long nrhs, c
 // ---------------------------------------------------------------------------
 constexpr int kBatchedMaxN = 256;
 // resident workgroups per CU of the batched kernel (its occupancy)
-template <typename ST> int potrf_batched_blocks_per_cu(char uplo);
+template <typename ST> int potrf_batched_blocks_per_cu(char uplo, bool inverse);
 // Factor count packed matrices in place (matrix order[t] for t = 0..count-1,
 // n = nvec[i] <= kBatchedMaxN, at element offset offs[i]); info[i] and
-// logdet[i] (may be null) per matrix; `grid` persistent workgroups.
+// logdet[i] (may be null) per matrix; `grid` persistent workgroups.  With
+// `inverse`, each factored matrix is then overwritten by its inverse.
 template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,
-                         const int *order, ST *A, double *logdet, int *info, int grid);
+                         const int *order, ST *A, double *logdet, int *info, int grid, bool inverse);
 
 // GMRES basis helpers: right-hand side k owns V + (k nv + i) n, i < nv.
 hipError_t basis_dots(hipStream_t s, long n, long nrhs, int nv, int first, int nvec, const double *V,

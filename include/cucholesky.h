@@ -97,6 +97,13 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle handle, CBlasUplo uplo, char pr
                                   CUbatchedplan * plan);
 CUresult cuPotrfBatchedRun(CUbatchedplan plan, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
                            CUstream stream);
+/*
+ * cuPotriBatchedRun: the same factorisation and logdet, then every factored
+ * matrix (info[i] == 0) is overwritten by its inverse (triangle uplo of
+ * A_i^-1, LAPACK xPOTRI semantics), in the same single launch.
+ */
+CUresult cuPotriBatchedRun(CUbatchedplan plan, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
+                           CUstream stream);
 CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan plan);
 
 #ifdef __cplusplus

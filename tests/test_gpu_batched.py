@@ -144,3 +144,109 @@ def test_batched_graph_replay(gpu, lapack):
     for A, ld in results[1:]:
         np.testing.assert_array_equal(np.nan_to_num(A), np.nan_to_num(results[0][0]))
         np.testing.assert_array_equal(ld, results[0][1])
+
+
+# ---------------------------------------------------------------- inverse (cuPotriBatchedRun)
+
+def run_inverse(gpu, lapack, uplo, prec, sizes, buf, offs):
+    dtype = torch.float64 if prec == "D" else torch.float32
+    dA = torch.from_numpy(buf).to(dtype).cuda()
+    dlog = torch.zeros(len(sizes), dtype=torch.float64, device="cuda")
+    dinfo = torch.full((len(sizes),), -7, dtype=torch.int32, device="cuda")
+    with gpu.BatchedPlan(lapack, ord(uplo), prec, sizes, offs) as plan:
+        assert plan.status == 0
+        assert plan.run_inverse(gpu.ptr(dA), gpu.ptr(dlog), gpu.ptr(dinfo)) == 0
+        torch.cuda.synchronize()
+    return dA.double().cpu().numpy(), dlog.cpu().numpy(), dinfo.cpu().numpy()
+
+
+def check_inverse(buf_in, out, logdet, info, offs, sizes, mats, uplo, eps):
+    """Per matrix: ||I - A X|| / cond(A) <= 10 n eps, X against the oracle's
+    dpotri of its own factor (relative 10 n eps cond), logdet as for the
+    factorisation; unreferenced triangles and gaps untouched."""
+    for i, (o, n) in enumerate(zip(offs, sizes)):
+        assert info[i] == 0, (i, n, info[i])
+        if n == 0:
+            continue
+        X = out[o:o + n * n].reshape((n, n), order="F")
+        A = mats[i] if eps == EPS else mats[i].astype(np.float32).astype(np.float64)
+        condA = np.linalg.cond(A)
+        assert O.inverse_residual(A, X, uplo) <= 10 * n * eps, (i, n)
+        F, rinfo = O.dpotrf(uplo, np.asfortranarray(A.copy()))
+        assert rinfo == 0
+        want = 2.0 * math.fsum(np.log(np.diag(F)))
+        assert abs(logdet[i] - want) <= 10 * n * eps * max(1.0, abs(want)), (i, n)
+        ref, _ = O.dpotri(uplo, F.copy(order="F"))   # the oracle inverts in place
+        Xf, Rf = O.herm_from(X, uplo), O.herm_from(ref, uplo)
+        assert np.linalg.norm(Xf - Rf) <= 10 * n * eps * condA * np.linalg.norm(Rf), (i, n)
+    np.testing.assert_array_equal(np.isnan(out), np.isnan(buf_in))
+
+
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("prec", "DS")
+def test_batched_inverse_edge_sizes(gpu, lapack, uplo, prec):
+    buf, offs, mats = make_batch(5, SIZES_EDGE)
+    buf = triangle_only(buf, offs, SIZES_EDGE, uplo)
+    out, logdet, info = run_inverse(gpu, lapack, uplo, prec, SIZES_EDGE, buf, offs)
+    check_inverse(buf if prec == "D" else buf.astype(np.float32).astype(np.float64), out, logdet, info,
+                  offs, SIZES_EDGE, mats, uplo, EPS if prec == "D" else EPS32)
+
+
+def test_batched_inverse_closed_form(gpu, lapack):
+    """dlatmc matrices (H D H, cond 2): the inverse is H D^-1 H exactly."""
+    sizes = [7, 48, 130, 256]
+    offs, pos, mats, want = [], 0, [], []
+    for n in sizes:
+        offs.append(pos)
+        pos += n * n
+        O.srand(n)
+        A = O.dlatmc(n, 2.0)
+        mats.append(A)
+        want.append(np.linalg.inv(A))
+    buf = np.concatenate([A.ravel(order="F") for A in mats])
+    out, logdet, info = run_inverse(gpu, lapack, "L", "D", sizes, buf, offs)
+    for i, (o, n) in enumerate(zip(offs, sizes)):
+        assert info[i] == 0
+        X = O.herm_from(out[o:o + n * n].reshape((n, n), order="F"), "L")
+        assert np.linalg.norm(X - want[i]) <= 10 * n * EPS * 2 * np.linalg.norm(want[i]), (i, n)
+
+
+def test_batched_inverse_skips_failed(gpu, lapack):
+    """A non-SPD matrix reports info and is left as the partial factor; the others are inverted."""
+    sizes = [40, 90, 33]
+    buf, offs, mats = make_batch(6, sizes)
+    n, k = sizes[1], 60
+    A = mats[1]
+    L = np.linalg.cholesky(A)
+    A[k, k] = np.dot(L[k, :k], L[k, :k]) - 1.0
+    buf[offs[1]:offs[1] + n * n] = A.ravel(order="F")
+    out, logdet, info = run_inverse(gpu, lapack, "L", "D", sizes, buf, offs)
+    assert info[1] == k + 1 and np.isnan(logdet[1])
+    for i in (0, 2):
+        m = sizes[i]
+        X = out[offs[i]:offs[i] + m * m].reshape((m, m), order="F")
+        assert info[i] == 0 and O.inverse_residual(mats[i], X, "L") <= 10 * m * EPS
+
+
+def test_batched_inverse_then_factor_replays(gpu, lapack):
+    """One plan alternates factor-only and factor+inverse runs (each re-captures)."""
+    sizes = [64, 200, 17]
+    buf, offs, mats = make_batch(7, sizes)
+    with gpu.BatchedPlan(lapack, ord("L"), "D", sizes, offs) as plan:
+        dA = torch.from_numpy(buf).cuda()
+        dlog = torch.zeros(len(sizes), dtype=torch.float64, device="cuda")
+        dinfo = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+        for rep in range(2):
+            for inverse in (True, False):
+                dA.copy_(torch.from_numpy(buf))
+                torch.cuda.synchronize()
+                fn = plan.run_inverse if inverse else plan.run
+                assert fn(gpu.ptr(dA), gpu.ptr(dlog), gpu.ptr(dinfo)) == 0
+                torch.cuda.synchronize()
+                out = dA.cpu().numpy()
+                for i, (o, n) in enumerate(zip(offs, sizes)):
+                    M = out[o:o + n * n].reshape((n, n), order="F")
+                    if inverse:
+                        assert O.inverse_residual(mats[i], M, "L") <= 10 * n * EPS
+                    else:
+                        assert O.potrf_backward_error(mats[i], M, "L") <= 10 * n * EPS
