@@ -173,11 +173,11 @@ def mixed_precision(torch, pkg, h, n, nrhs, alphas):
 def batched(torch, pkg, h, count, nmin, nmax, reps, loop_sample):
     """Batched POTRF + logdet of `count` SPD matrices, n_i ~ U[nmin, nmax],
     A_i = G_i G_i^T / n_i + 0.5 I, packed contiguously (column-major, ld n_i),
-    in FP64 and FP32 storage: one graph launch per batch (cuPotrfBatchedRun),
-    timed with HIP events on the launch stream over `reps` runs (each on a
-    fresh copy of the batch, made outside the timed launch).  Compared with
-    a per-matrix loop of cuDpotrf + cuDlogdet (FP64) over the first
-    `loop_sample` matrices."""
+    in FP64 and FP32 storage: one graph launch per batch (cuPotrfBatchedRun;
+    and POTRF + logdet + POTRI, cuPotriBatchedRun), timed with HIP events on
+    the launch stream over `reps` runs (each on a fresh copy of the batch,
+    made outside the timed launch).  Compared with a per-matrix loop of
+    cuDpotrf + cuDlogdet (FP64) over the first `loop_sample` matrices."""
     import numpy as np
     rng = np.random.default_rng(5)
     sizes = rng.integers(nmin, nmax + 1, size=count)
@@ -203,28 +203,35 @@ def batched(torch, pkg, h, count, nmin, nmax, reps, loop_sample):
         work = torch.empty_like(src)
         logd = torch.empty(count, dtype=torch.float64, device="cuda")
         info = torch.empty(count, dtype=torch.int32, device="cuda")
-        times = []
+        res = {}
         with pkg.BatchedPlan(h, pkg.Lower, prec, sizes.tolist(), offs.tolist()) as plan:
             assert plan.status == 0
-            for r in range(reps + 1):
-                work.copy_(src)
-                torch.cuda.synchronize()
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(s)
-                assert plan.run(pkg.ptr(work), pkg.ptr(logd), pkg.ptr(info), s.cuda_stream) == 0
-                e1.record(s)
-                s.synchronize()
-                if r > 0:            # the first run captures the graph
-                    times.append(e0.elapsed_time(e1) * 1e-3)
-        assert int(info.abs().sum().item()) == 0
-        t = sorted(times)[len(times) // 2]
+            for inverse in (False, True):
+                times = []
+                fn = plan.run_inverse if inverse else plan.run
+                for r in range(reps + 1):
+                    work.copy_(src)
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(s)
+                    assert fn(pkg.ptr(work), pkg.ptr(logd), pkg.ptr(info), s.cuda_stream) == 0
+                    e1.record(s)
+                    s.synchronize()
+                    if r > 0:            # the first run captures the graph
+                        times.append(e0.elapsed_time(e1) * 1e-3)
+                assert int(info.abs().sum().item()) == 0
+                res[inverse] = sorted(times)[len(times) // 2]
+                if prec == "D" and not inverse:
+                    ref_log = logd.clone()
+        t, ti = res[False], res[True]
         byts = 2 * tri_bytes * esz            # read and write one triangle per matrix
         out["fp64" if prec == "D" else "fp32"] = {
             "ms": round(t * 1e3, 3), "matrices_per_s": round(count / t), "tflops": round(flops / t / 1e12, 3),
             "hbm_gbs_algorithmic": round(byts / t / 1e9, 1),
-            "hbm_frac": round(byts / t / 8e12, 4)}
-        if prec == "D":
-            ref_log = logd.clone()
+            "hbm_frac": round(byts / t / 8e12, 4),
+            # POTRF + logdet + POTRI in one launch (cuPotriBatchedRun); flops n^3/3 + n^3
+            "with_inverse": {"ms": round(ti * 1e3, 3), "matrices_per_s": round(count / ti),
+                             "tflops_potrf_plus_potri": round(4 * flops / ti / 1e12, 3)}}
         del src, work
     # per-matrix FP64 loop: cuDpotrf + cuDlogdet on each matrix
     k = min(loop_sample, count)
@@ -273,6 +280,80 @@ def cpu_baseline(torch, n_cpu):
     }
 
 
+def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs):
+    """Batched mixed-precision solve + logdet + inverse (cuDsposvxBatched) of
+    `count` FP64 SPD matrices A_i = G_i G_i^T / n_i + alpha_i I with n_i
+    log-uniform in [nmin, nmax] and alpha_i log-uniform in [1e-4, 1], `nrhs`
+    N(0,1) right-hand sides each.  Wall time of the (synchronous) call; per
+    matrix: the normwise backward error of X (max over right-hand sides,
+    ||b - A x||_inf / (||A||_inf ||x||_inf)), the inverse residual
+    ||I - A A^-1||_F / (||A||_F ||A^-1||_F) and the logdet against the FP64
+    Cholesky factor of the same matrix (torch.linalg.cholesky, FP64).  The
+    same pipeline with plain FP64 per matrix (cuDpotrf + cuDpotrs + cuDlogdet
+    + cuDpotri on a copy) is timed beside it."""
+    import numpy as np
+    rng = np.random.default_rng(9)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count)).astype(np.int64)
+    alphas = np.exp(rng.uniform(np.log(1e-4), 0.0, size=count))
+    offA = np.zeros(count, dtype=np.int64)
+    offA[1:] = np.cumsum(sizes ** 2)[:-1]
+    offB = np.zeros(count, dtype=np.int64)
+    offB[1:] = np.cumsum(sizes * nrhs)[:-1]
+    A = torch.empty(int((sizes ** 2).sum()), dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(10)
+    for i, n in enumerate(sizes.tolist()):
+        G = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+        M = G @ G.T / n
+        M.diagonal().add_(float(alphas[i]))
+        A[offA[i]:offA[i] + n * n] = M.reshape(-1)
+    B = torch.randn(int((sizes * nrhs).sum()), generator=g, device="cuda", dtype=torch.float64)
+    A0 = A.clone()
+    X = torch.zeros_like(B)
+    Ainv = torch.empty_like(A)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, it, ld, info = pkg.cuDsposvxBatched(h, pkg.Lower, sizes.tolist(), offA.tolist(), pkg.ptr(A), nrhs,
+                                            offB.tolist(), pkg.ptr(B), pkg.ptr(X), pkg.ptr(Ainv))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    assert st == 0 and all(v == 0 for v in info), (st, info[:8])
+    berr, ires, lderr = [], [], []
+    for i, n in enumerate(sizes.tolist()):
+        M = A0[offA[i]:offA[i] + n * n].view(n, n)          # symmetric
+        Xi = X[offB[i]:offB[i] + n * nrhs].view(nrhs, n).T
+        Bi = B[offB[i]:offB[i] + n * nrhs].view(nrhs, n).T
+        R = Bi - M @ Xi
+        berr.append((R.abs().amax(0) / (M.abs().sum(1).max() * Xi.abs().amax(0))).max().item())
+        T = Ainv[offA[i]:offA[i] + n * n].view(n, n).T      # column-major lower triangle
+        S = torch.tril(T) + torch.tril(T, -1).T
+        ires.append((torch.linalg.matrix_norm(torch.eye(n, device="cuda", dtype=torch.float64) - M @ S)
+                     / (torch.linalg.matrix_norm(M) * torch.linalg.matrix_norm(S))).item())
+        ref = 2.0 * torch.log(torch.diagonal(torch.linalg.cholesky(M))).sum().item()
+        lderr.append(abs(ld[i] - ref) / max(1.0, abs(ref)))
+    # plain FP64 per matrix: factor, solve, logdet, inverse
+    A64 = A0.clone()
+    X64 = B.clone()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i, n in enumerate(sizes.tolist()):
+        Ai = pkg.ptr(A64) + int(offA[i]) * 8
+        st1, inf1 = pkg.cuDpotrf(h, pkg.Lower, n, Ai, n)
+        st2, inf2 = pkg.cuDpotrs(h, pkg.Lower, n, nrhs, Ai, n, pkg.ptr(X64) + int(offB[i]) * 8, n)
+        st3, _ = pkg.cuDlogdet(h, Ai, n + 1, n)
+        st4, inf4 = pkg.cuDpotri(h, pkg.Lower, n, Ai, n)
+        assert st1 == st2 == st3 == st4 == 0 and inf1 == inf2 == inf4 == 0
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t1
+    flops = float((sizes.astype(np.float64) ** 3).sum())
+    q = lambda v: {"max": float(np.max(v)), "median": float(np.median(v))}
+    return {"count": count, "n_range": [nmin, nmax], "alpha_range": [1e-4, 1.0], "nrhs": nrhs,
+            "ms": round(t * 1e3, 2), "matrices_per_s": round(count / t, 1),
+            "fp64_per_matrix_ms": round(t64 * 1e3, 2), "speedup_vs_fp64_per_matrix": round(t64 / t, 2),
+            "flops_potrf_n3_3_plus_potri_n3": 4 * flops / 3,
+            "iter_histogram": {str(k): int(v) for k, v in zip(*np.unique(it, return_counts=True))},
+            "backward_error": q(berr), "inverse_residual": q(ires), "logdet_rel_error": q(lderr)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +368,9 @@ def main():
     ap.add_argument("--n-mixed", type=int, default=32768)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched factorisation (20000 matrices, n in [8, 256])")
+    ap.add_argument("--no-batched-mixed", action="store_true",
+                    help="skip the batched mixed-precision solve + logdet + inverse (512 matrices, "
+                         "n log-uniform in [128, 2048])")
     ap.add_argument("--counter-calibration", action="store_true",
                     help="run one elementwise copy of A (n^2 * 8 bytes read and written) before the "
                          "steps, as the known-traffic reference kernel of a rocprofv3 --pmc run")
@@ -386,6 +470,9 @@ def main():
 
     if not args.no_batched:
         extra["batched_potrf_logdet"] = batched(torch, pkg, h, 20000, 8, 256, 5, 2000)
+
+    if not args.no_batched_mixed:
+        extra["batched_mixed_solve_inverse"] = batched_mixed(torch, pkg, h, 512, 128, 2048, 4)
 
     cpu = None
     if not args.no_cpu:

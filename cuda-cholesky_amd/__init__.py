@@ -103,6 +103,9 @@ SIGNATURES = {
     "cuDsposv": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, _longp, _longp],
     "cuDsposvx": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, ctypes.c_char, ctypes.c_char,
                   _longp, ctypes.c_char_p, _longp],
+    "cuDsposvxBatched": [_h, _enum, _sz, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                         _ptr, _sz, ctypes.POINTER(ctypes.c_size_t), _ptr, _ptr, _ptr, _longp,
+                         ctypes.POINTER(ctypes.c_double), _longp],
     "cuPotrfBatchedPlanCreate": [_h, _enum, ctypes.c_char, _sz, ctypes.POINTER(ctypes.c_size_t),
                                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_h)],
     "cuPotrfBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
@@ -259,6 +262,19 @@ def cuDsposvx(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor="S", refine=
     st = lib().cuDsposvx(_hv(handle), uplo, n, nrhs, A, lda, B, ldb, X, ldx, enc(factor), enc(refine),
                          ctypes.byref(it), used, ctypes.byref(info))
     return st, it.value, used.raw[:1].decode(), info.value
+
+
+def cuDsposvxBatched(handle, uplo, n, offA, A, nrhs, offB, B, X, Ainv=0):
+    """(status, iter[], logdet[], info[]) of the batched mixed-precision solve
+    + logdet (+ inverse into Ainv when nonzero); n, offA, offB host sequences."""
+    count = len(n)
+    arr = lambda v: (ctypes.c_size_t * max(count, 1))(*[int(x) for x in v])
+    it = (ctypes.c_long * max(count, 1))()
+    ld = (ctypes.c_double * max(count, 1))()
+    info = (ctypes.c_long * max(count, 1))()
+    st = lib().cuDsposvxBatched(_hv(handle), uplo, count, arr(n), arr(offA), A, nrhs, arr(offB), B, X,
+                                Ainv, it, ld, info)
+    return st, list(it)[:count], list(ld)[:count], list(info)[:count]
 
 
 class BatchedPlan:

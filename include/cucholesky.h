@@ -106,6 +106,26 @@ CUresult cuPotriBatchedRun(CUbatchedplan plan, CUdeviceptr A, CUdeviceptr logdet
                            CUstream stream);
 CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan plan);
 
+/*
+ * Batched mixed-precision solve + log-determinant + inverse of `count` SPD
+ * matrices of any sizes: matrix i is n[i] x n[i] (column-major, ld n[i]) at
+ * element offset offA[i] of A; its right-hand sides and solution are
+ * n[i] x nrhs (ld n[i]) at element offset offB[i] of B and X.  Per matrix:
+ *   X_i = A_i^-1 B_i by an FP32 factor with FP64 iterative refinement
+ *   (cuDsposvx, factor 'S', refine 'I'; iter[i] its ITER code; A_i unchanged
+ *   when iter[i] >= 0, its FP64 factor otherwise);
+ *   logdet[i] = log det A_i from an FP64 factorisation of a copy of A_i;
+ *   when Ainv != 0, triangle uplo of A_i^-1 at element offset offA[i] of Ainv;
+ *   info[i] = 0 or the order of the leading minor that is not positive
+ *   definite (logdet[i] is then NaN).
+ * iter, logdet and info are host arrays of `count` elements.  Synchronous;
+ * the matrices are shared out to worker threads with their own streams.
+ */
+CUresult cuDsposvxBatched(CULAPACKhandle handle, CBlasUplo uplo, size_t count, const size_t * n,
+                          const size_t * offA, CUdeviceptr A, size_t nrhs, const size_t * offB,
+                          CUdeviceptr B, CUdeviceptr X, CUdeviceptr Ainv, long * iter,
+                          double * logdet, long * info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,3 +250,66 @@ def test_batched_inverse_then_factor_replays(gpu, lapack):
                         assert O.inverse_residual(mats[i], M, "L") <= 10 * n * EPS
                     else:
                         assert O.potrf_backward_error(mats[i], M, "L") <= 10 * n * EPS
+
+
+# ---------------------------------------------------------------- mixed-precision batched solve
+
+def test_batched_mixed_solve_inverse_logdet(gpu, lapack):
+    """cuDsposvxBatched on uneven sizes and conditions (A = G G^T / n + alpha I):
+    per matrix, X converges to the DSPOSV criterion with an FP64-level
+    normwise backward error, logdet matches the oracle's factor, the inverse
+    passes ||I - A X|| <= 10 n eps cond and matches the oracle's dpotri."""
+    sizes = [1, 7, 129, 300, 520, 64, 1000]
+    alphas = [1.0, 1e-2, 1e-4, 1.0, 1e-3, 1e-1, 1e-4]
+    nrhs = 4
+    offA = list(np.cumsum([0] + [n * n for n in sizes[:-1]]))
+    offB = list(np.cumsum([0] + [n * nrhs for n in sizes[:-1]]))
+    mats = [O.spd(300 + i, n, a) for i, (n, a) in enumerate(zip(sizes, alphas))]
+    rhs = [np.asfortranarray(np.random.default_rng(i).standard_normal((n, nrhs))) for i, n in enumerate(sizes)]
+    bufA = np.concatenate([np.tril(A).ravel(order="F") for A in mats])
+    bufB = np.concatenate([B.ravel(order="F") for B in rhs])
+    dA = torch.from_numpy(bufA).cuda()
+    dB = torch.from_numpy(bufB).cuda()
+    dX = torch.zeros_like(dB)
+    dI = torch.full_like(dA, np.nan)
+    st, it, ld, info = gpu.cuDsposvxBatched(lapack, ord("L"), sizes, offA, gpu.ptr(dA), nrhs, offB,
+                                            gpu.ptr(dB), gpu.ptr(dX), gpu.ptr(dI))
+    torch.cuda.synchronize()
+    assert st == 0
+    X, Inv = dX.cpu().numpy(), dI.cpu().numpy()
+    for i, n in enumerate(sizes):
+        A, B = mats[i], rhs[i]
+        assert info[i] == 0 and it[i] >= 0, (i, n, info[i], it[i])
+        Xi = X[offB[i]:offB[i] + n * nrhs].reshape((n, nrhs), order="F")
+        R = B - A @ Xi
+        berr = np.abs(R).max(axis=0) / (np.abs(A).sum(axis=1).max() * np.abs(Xi).max(axis=0))
+        assert np.all(berr <= 4 * np.sqrt(n) * EPS), (i, n, berr)
+        F, rinfo = O.dpotrf("L", np.asfortranarray(A.copy()))
+        want = 2.0 * math.fsum(np.log(np.diag(F)))
+        assert abs(ld[i] - want) <= 10 * n * EPS * max(1.0, abs(want)), (i, n, ld[i], want)
+        Xinv = Inv[offA[i]:offA[i] + n * n].reshape((n, n), order="F")
+        assert O.inverse_residual(A, Xinv, "L") <= 10 * n * EPS, (i, n)
+        ref, _ = O.dpotri("L", F.copy(order="F"))
+        Rf = O.herm_from(ref, "L")
+        assert np.linalg.norm(O.herm_from(Xinv, "L") - Rf) <= 10 * n * EPS * np.linalg.cond(A) * np.linalg.norm(Rf)
+    # A is untouched where refinement converged (here: all)
+    np.testing.assert_array_equal(dA.cpu().numpy(), bufA)
+
+
+def test_batched_mixed_solve_not_spd(gpu, lapack):
+    """A matrix that is not positive definite reports info and NaN logdet; its neighbours solve."""
+    sizes, nrhs = [40, 90, 33], 2
+    mats = [O.spd(400 + i, n, 0.5) for i, n in enumerate(sizes)]
+    k = 50
+    L = np.linalg.cholesky(mats[1])
+    mats[1][k, k] = np.dot(L[k, :k], L[k, :k]) - 1.0
+    offA = list(np.cumsum([0] + [n * n for n in sizes[:-1]]))
+    offB = list(np.cumsum([0] + [n * nrhs for n in sizes[:-1]]))
+    dA = torch.from_numpy(np.concatenate([A.ravel(order="F") for A in mats])).cuda()
+    dB = torch.ones(sum(n * nrhs for n in sizes), dtype=torch.float64, device="cuda")
+    dX = torch.zeros_like(dB)
+    st, it, ld, info = gpu.cuDsposvxBatched(lapack, ord("L"), sizes, offA, gpu.ptr(dA), nrhs, offB,
+                                            gpu.ptr(dB), gpu.ptr(dX), 0)
+    assert st == 0
+    assert info[1] == k + 1 and np.isnan(ld[1])
+    assert info[0] == 0 and info[2] == 0 and it[0] >= 0 and it[2] >= 0
