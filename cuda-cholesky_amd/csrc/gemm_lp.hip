@@ -543,7 +543,49 @@ __global__ void panel_to_bf16_kernel(int m, int kp, const float *P, long ldp, u1
   }
 }
 
+// Split copy for three-product BF16 updates: x = hi + lo (hi = bf16(x),
+// lo = bf16(x - hi)), stored as rows of 3 ldt values
+//   TA row r: [hi | hi | lo],  TB row r: [hi | lo | hi]
+// so that TA TB^T = hi hi^T + hi lo^T + lo hi^T (the lo lo^T term, 2^-16
+// relative, is dropped); zeros for kp <= k < ldt.
+__global__ void panel_to_bf16_split_kernel(int m, int kp, const float *P, long ldp, u16 *TA, u16 *TB,
+                                           long ldt) {
+  __shared__ float t[32][33];
+  const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + tx, k = k0 + j;
+    t[j][tx] = (r < m && k < kp) ? P[(long)k * ldp + r] : 0.0f;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, k = k0 + tx;
+    if (r < m && k < ldt) {
+      const float x = t[tx][j];
+      const __bf16 hi = (__bf16)x;
+      const __bf16 lo = (__bf16)(x - (float)hi);
+      const u16 h = __builtin_bit_cast(u16, hi), l = __builtin_bit_cast(u16, lo);
+      u16 *a = TA + (long)r * 3 * ldt + k, *b = TB + (long)r * 3 * ldt + k;
+      a[0] = h;
+      a[ldt] = h;
+      a[2 * ldt] = l;
+      b[0] = h;
+      b[ldt] = l;
+      b[2 * ldt] = h;
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t panel_to_bf16_split(hipStream_t s, long m, long kp, const float *P, long ldp, void *TA, void *TB,
+                               long ldt) {
+  if (m <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((m + 31) / 32), (unsigned)((ldt + 31) / 32));
+  hipLaunchKernelGGL(panel_to_bf16_split_kernel, grid, dim3(256), 0, s, (int)m, (int)kp, P, ldp,
+                     static_cast<u16 *>(TA), static_cast<u16 *>(TB), ldt);
+  return hipGetLastError();
+}
 
 hipError_t panel_to_bf16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt) {
   if (m <= 0) return hipSuccess;

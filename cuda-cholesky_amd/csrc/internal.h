@@ -183,6 +183,11 @@ hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, cons
                         long ldd, int tri_out, bool tri_grid, const int *abort_flag);
 // T (bf16, row-major, ld ldt) <- the m x kp FP32 panel P, zeros for k >= kp.
 hipError_t panel_to_bf16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt);
+// Split copies (rows of 3 ldt bf16 values): TA = [hi | hi | lo], TB = [hi | lo | hi]
+// with x = hi + lo, so that a BF16 product TA TB^T over 3 ldt carries ~16
+// significant bits of the panel.
+hipError_t panel_to_bf16_split(hipStream_t s, long m, long kp, const float *P, long ldp, void *TA, void *TB,
+                               long ldt);
 
 // ---------------------------------------------------------------------------
 // mixed-precision solver kernels (mixed.hip)
@@ -263,12 +268,13 @@ long tuned_potrf_nb(char prec, long n);
 // LAPACK internals shared by the solvers (lapack_api.cpp).
 hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out);
 // Blocked factorisation on the handle's streams; W: diag_max^2 elements.
-// bf16: FP32 data only, trailing updates with BF16 products.
+// bf16: FP32 data only, trailing updates with BF16 products (split: three
+// BF16 products of a two-term split of every panel value).
 template <typename T>
 hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
-                           void *lp_ws = nullptr);
-// Workspace of the bf16 factorisation (two bfloat16 panel copies).
-size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n);
+                           void *lp_ws = nullptr, bool split = false);
+// Workspace of the bf16 factorisation (two bfloat16 panel copies; six with split).
+size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n, bool split = false);
 
 // ---------------------------------------------------------------------------
 // error plumbing

@@ -157,23 +157,36 @@ hipError_t timing_end(CULAPACKhandle h, hipStream_t s) {
 // bf16 (FP32 data, lower only): after each panel is factored it is also
 // converted to bfloat16 with k contiguous (two alternating buffers at lp_ws,
 // potrf_bf16_ws_bytes) and the next-panel and trailing updates take their
-// products from those copies on the BF16 matrix cores.
+// products from those copies on the BF16 matrix cores.  split: each copy is
+// the pair of rows [hi | hi | lo] and [hi | lo | hi] (x = hi + lo), and the
+// products run over 3 ldt: three BF16 products, ~16 significant bits.
 template <typename T>
 hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false,
-                        void *lp_ws = nullptr) {
+                        void *lp_ws = nullptr, bool split = false) {
   const long NB = potrf_nb(h, prec_of<T>(bf16), n);
   hipStream_t s0 = h->stream, s1 = h->side;
   const bool lower = (uplo == 'L');
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0);
   auto at = [&](long r, long c) { return A + c * lda + r; };
   const long ldt = (NB + 63) / 64 * 64;
-  unsigned short *Pt[2] = {static_cast<unsigned short *>(lp_ws),
-                           static_cast<unsigned short *>(lp_ws) + (lp_ws ? n * ldt : 0)};
-  if (bf16 && (!lower || lp_ws == nullptr || !std::is_same<T, float>::value)) return hipErrorInvalidValue;
-  HC_TRY(factor_panel<T>(h, s0, lower, n, A, lda, W, 0, std::min(NB, n)));
-  if constexpr (std::is_same<T, float>::value) {
-    if (bf16) HC_TRY(hc::panel_to_bf16(s0, n, std::min(NB, n), A, lda, Pt[0], ldt));
+  split = split && bf16;
+  const long kk = split ? 3 * ldt : ldt;  // K of the bf16 products (and row stride of the copies)
+  // per panel buffer: the A-side copy, then (split) the B-side copy
+  unsigned short *Pt[2], *Qt[2];
+  for (int b = 0; b < 2; b++) {
+    Pt[b] = static_cast<unsigned short *>(lp_ws) + (lp_ws ? (split ? 2 : 1) * b * n * kk : 0);
+    Qt[b] = split ? Pt[b] + n * kk : Pt[b];
   }
+  if (bf16 && (!lower || lp_ws == nullptr || !std::is_same<T, float>::value)) return hipErrorInvalidValue;
+  auto convert = [&](hipStream_t s, long m, long kp, const T *P, int b) -> hipError_t {
+    if constexpr (std::is_same<T, float>::value) {
+      return split ? hc::panel_to_bf16_split(s, m, kp, P, lda, Pt[b], Qt[b], ldt)
+                   : hc::panel_to_bf16(s, m, kp, P, lda, Pt[b], ldt);
+    }
+    return hipErrorInvalidValue;
+  };
+  HC_TRY(factor_panel<T>(h, s0, lower, n, A, lda, W, 0, std::min(NB, n)));
+  if (bf16) HC_TRY(convert(s0, n, std::min(NB, n), A, 0));
   for (long j = 0; j < n; j += NB) {
     const int pj = (int)((j / NB) & 1);
     const long jb = std::min(NB, n - j);
@@ -188,8 +201,8 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
       bool done = false;
       if constexpr (std::is_same<T, float>::value) {
         if (bf16) {
-          const unsigned short *Pn = Pt[pj] + (jn - j) * ldt;
-          HC_TRY(hc::gemm_bf16_nt(s0, n - jn, jbn, ldt, -1.0f, Pn, ldt, Pn, ldt, 1.0f, at(jn, jn), lda,
+          const unsigned short *Pn = Pt[pj] + (jn - j) * kk, *Qn = Qt[pj] + (jn - j) * kk;
+          HC_TRY(hc::gemm_bf16_nt(s0, n - jn, jbn, kk, -1.0f, Pn, kk, Qn, kk, 1.0f, at(jn, jn), lda,
                                   at(jn, jn), lda, hc::TRI_LOWER, false, h->dinfo));
           done = true;
         }
@@ -205,9 +218,7 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     HC_TRY(hipEventRecord(h->ev_update, s0));
     HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
     HC_TRY(factor_panel<T>(h, s1, lower, n, A, lda, W, jn, jbn));
-    if constexpr (std::is_same<T, float>::value) {
-      if (bf16) HC_TRY(hc::panel_to_bf16(s1, n - jn, jbn, at(jn, jn), lda, Pt[pj ^ 1], ldt));
-    }
+    if (bf16) HC_TRY(convert(s1, n - jn, jbn, at(jn, jn), pj ^ 1));
     HC_TRY(hipEventRecord(h->ev_panel, s1));
     const long t0 = jn + jbn, t = n - t0;   // trailing matrix beyond the next panel
     if (t > 0) {
@@ -216,8 +227,8 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
       bool done = false;
       if constexpr (std::is_same<T, float>::value) {
         if (bf16) {
-          const unsigned short *Pn = Pt[pj] + (t0 - j) * ldt;
-          HC_TRY(hc::gemm_bf16_nt(s0, t, t, ldt, -1.0f, Pn, ldt, Pn, ldt, 1.0f, at(t0, t0), lda,
+          const unsigned short *Pn = Pt[pj] + (t0 - j) * kk, *Qn = Qt[pj] + (t0 - j) * kk;
+          HC_TRY(hc::gemm_bf16_nt(s0, t, t, kk, -1.0f, Pn, kk, Qn, kk, 1.0f, at(t0, t0), lda,
                                   at(t0, t0), lda, hc::TRI_LOWER, true, h->dinfo));
           done = true;
         }
@@ -444,17 +455,18 @@ hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out) {
 
 template <typename T>
 hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
-                           void *lp_ws) {
-  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws);
+                           void *lp_ws, bool split) {
+  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws, split);
 }
 template hipError_t potrf_on_device<double>(CULAPACKhandle, char, long, double *, long, double *, bool,
-                                            void *);
+                                            void *, bool);
 template hipError_t potrf_on_device<float>(CULAPACKhandle, char, long, float *, long, float *, bool,
-                                           void *);
+                                           void *, bool);
 
-size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n) {
+size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n, bool split) {
   const long ldt = (potrf_nb(h, 'B', n) + 63) / 64 * 64;
-  return 2 * (size_t)n * (size_t)ldt * 2;
+  // two alternating panel buffers of n x ldt bf16; split: A- and B-side copies of n x 3 ldt each
+  return 2 * (size_t)n * (size_t)ldt * 2 * (split ? 6 : 1);
 }
 
 }  // namespace hc

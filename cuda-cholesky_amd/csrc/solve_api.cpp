@@ -99,7 +99,7 @@ SposvLayout sposv_layout(CULAPACKhandle h, long n, long nrhs) {
   L.PART = off; off = align256(off + hc::symm_tall_ws_bytes(n, nrhs));  // symmetric products
   L.NORM = off; off = align256(off + (size_t)hc::sym_norm_ws_doubles(n) * sizeof(double));
   L.DEV = off;  off = align256(off + (3 * (size_t)nrhs + 8) * sizeof(double));    // norms, flags
-  L.P16 = off;  off = align256(off + hc::potrf_bf16_ws_bytes(h, n));                // bf16 panels
+  L.P16 = off;  off = align256(off + hc::potrf_bf16_ws_bytes(h, n, true));          // bf16 panels
   L.V = off;    off = align256(off + (size_t)nrhs * (kGmresM + 1) * n * sizeof(double));  // Krylov bases
   L.WV = off;   off = align256(off + (size_t)n * nrhs * sizeof(double));
   L.HD = off;   off = align256(off + 3 * (size_t)nrhs * (kGmresM + 1) * sizeof(double));
@@ -155,7 +155,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   if (lda < (n > 1 ? n : 1)) { *info = -5; return bad_arg(name, 5); }
   if (ldb < (n > 1 ? n : 1)) { *info = -7; return bad_arg(name, 7); }
   if (ldx < (n > 1 ? n : 1)) { *info = -9; return bad_arg(name, 9); }
-  if (factor != 'S' && factor != 'B') { *info = -10; return bad_arg(name, 10); }
+  if (factor != 'S' && factor != 'B' && factor != 'X') { *info = -10; return bad_arg(name, 10); }
   if (refine != 'I' && refine != 'G') { *info = -11; return bad_arg(name, 11); }
   if (iter == nullptr) { *info = -12; return bad_arg(name, 12); }
   *iter = 0;
@@ -224,14 +224,14 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   };
 
   // One attempt with a low-precision factor; returns the LAPACK ITER code.
-  auto attempt = [&](bool bf16, long *code) -> CUresult {
+  auto attempt = [&](bool bf16, bool split, long *code) -> CUresult {
     HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
     HC_CHECK(hc::to_float_lower(s, ul, N, Ad, (long)lda, S, N, ovf));
     int flag = 0;
     CU_TRY(read_flag(&flag));
     if (flag) { *code = -2; return CUDA_SUCCESS; }       // narrowing overflowed
     HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), s));
-    HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16, wsb + L.P16));
+    HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16, wsb + L.P16, split));
     long finfo = 0;
     HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipStreamSynchronize(s));
@@ -365,10 +365,10 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
 
   HC_CHECK(hipMemsetAsync(sync, 0, sizeof(int), s));
   long code = 0;
-  CUresult r = attempt(factor == 'B', &code);
+  CUresult r = attempt(factor != 'S', factor == 'X', &code);
   if (r != CUDA_SUCCESS) return r;
-  if (code < 0 && factor == 'B') {  // BF16 products too coarse: FP32 next
-    r = attempt(false, &code);
+  if (code < 0 && factor != 'S') {  // BF16 products too coarse: FP32 next
+    r = attempt(false, false, &code);
     if (r != CUDA_SUCCESS) return r;
     if (used != nullptr) *used = 'S';
   }

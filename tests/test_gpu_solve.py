@@ -241,6 +241,22 @@ def test_dsposv_bf16(gpu, lapack, n, nrhs):
     assert dsposv_converged(A, X, B)
 
 
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("n,nrhs,alpha", [(300, 3, 1.0), (1200, 16, 1e-3), (2100, 4, 1e-4)])
+def test_dsposv_split_bf16(gpu, lapack, uplo, n, nrhs, alpha):
+    # three-product split BF16 (~16 bits): refines conditions that plain BF16 cannot (cond ~ 4 / alpha)
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, uplo, n, nrhs, alpha, "X", seed=3)
+    assert it >= 0 and used == "X", (it, used)
+    assert dsposv_converged(A, X, B)
+
+
+def test_dsposv_split_bf16_large(gpu, lapack):
+    # several panels with look-ahead (n > 2 NB); alpha = 1e-3 as in the n = 32768 workload
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", 4500, 16, 1e-3, "X", seed=4)
+    assert it >= 0 and used == "X", (it, used)
+    assert dsposv_converged(A, X, B)
+
+
 def test_dsposv_bf16_ill_conditioned_falls_back_to_fp32(gpu, lapack):
     # cond ~ 4e5: beyond what a bf16-product factor refines, within FP32's reach
     A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", 800, 2, 1e-5, "B", seed=2)
