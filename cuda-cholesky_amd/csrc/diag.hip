@@ -43,6 +43,24 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Gather of `total` global elements: use(idx, A[addr(idx)]) for idx in
+// [0, total), U loads per thread in flight together (addresses clamped into
+// range, so every load is valid and none waits on the previous one; a plain
+// strided loop would pay one memory round trip per element).
+template <int U, typename T, typename Addr, typename Use>
+__device__ __forceinline__ void batched_gather(int total, const T *A, Addr addr, Use use) {
+  for (int base = threadIdx.x; base < total; base += U * (int)blockDim.x) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = A[addr(min(base + u * (int)blockDim.x, total - 1))];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int idx = base + u * (int)blockDim.x;
+      if (idx < total) use(idx, v[u]);
+    }
+  }
+}
+
 // S is column-major with leading dimension NMAX + 1: S(r, c) = S[c * LS + r].
 template <typename T, int NMAX>
 struct Lds {
@@ -55,15 +73,17 @@ struct Lds {
 template <typename T, int NMAX>
 __device__ __forceinline__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
   constexpr int LS = Lds<T, NMAX>::LS;
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int a = idx % n, b = idx / n;  // a: contiguous index in memory
-    if (lower) {
-      if (a >= b) S[b * LS + a] = A[(long)b * lda + a];
-    } else {
-      // A(a, b) with a <= b is upper; it is conj of lower-form (b, a)
-      if (a <= b) S[a * LS + b] = conjx(A[(long)b * lda + a]);
-    }
-  }
+  // a: contiguous index in memory; the unreferenced triangle is read but not used
+  batched_gather<16>(n * n, A, [&](int idx) { return (long)(idx / n) * lda + idx % n; },
+                     [&](int idx, T v) {
+                       const int a = idx % n, b = idx / n;
+                       if (lower) {
+                         if (a >= b) S[b * LS + a] = v;
+                       } else {
+                         // A(a, b) with a <= b is upper; it is conj of lower-form (b, a)
+                         if (a <= b) S[a * LS + b] = conjx(v);
+                       }
+                     });
 }
 
 // Store the lower form back into the caller's triangle (diagonal included
@@ -314,16 +334,19 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
   const int np = (n + LEAF - 1) / LEAF, npad = np * LEAF;
   auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
   auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
-  // lower form, read along memory's contiguous index; identity padding
+  // identity padding (rows / columns n .. npad - 1 of the lower form)
   for (int idx = tid; idx < npad * npad; idx += BLK_THREADS) {
-    const int i = idx % npad, k = idx / npad;
-    const int r = lower ? i : k, c = lower ? k : i;  // lower-form coordinates, r >= c wanted
-    if (r < c) continue;
-    T v;
-    if (r < n) v = lower ? convert<T>(A[(long)c * lda + r]) : conjx(convert<T>(A[(long)r * lda + c]));
-    else v = from_real<T>(r == c ? 1.0 : 0.0);
-    elem(r, c) = v;
+    const int r = idx % npad, c = idx / npad;
+    if (r >= c && r >= n) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
   }
+  // lower form, read along memory's contiguous index (i), all loads of a
+  // thread in flight together
+  batched_gather<16>(n * n, A, [&](int idx) { return (long)(idx / n) * lda + idx % n; },
+                     [&](int idx, ST x) {
+                       const int i = idx % n, k = idx / n;
+                       const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
+                       if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
+                     });
   if (tid == 0) flag = -1;
   __syncthreads();
   for (int p = 0; p < np; p++) {
@@ -434,15 +457,16 @@ trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const ST *A, lo
   const int n = (int)((m - off) < nb ? (m - off) : nb);
   // lower form: S(r, c) = op(A)(r, c) when op(A) is lower-like, else
   // conj(op(A)(c, r)); op(A)(i, j) = A(i, j), A(j, i) or conj(A(j, i)).
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int a = idx % n, b = idx / n;  // memory element A(off + a, off + b)
-    const T v = convert<T>(A[(long)(off + b) * lda + off + a]);
-    // as an element of op(A): (a, b) for N, (b, a) for T/C
-    int r = (op == OP_N) ? a : b, c = (op == OP_N) ? b : a;
-    T e = (op == OP_C) ? conjx(v) : v;
-    if (!lowerlike) { const int t = r; r = c; c = t; e = conjx(e); }
-    if (r >= c) S[c * LS + r] = e;
-  }
+  batched_gather<16>(n * n, A + off * lda + off, [&](int idx) { return (long)(idx / n) * lda + idx % n; },
+                     [&](int idx, ST x) {
+                       const int a = idx % n, b = idx / n;  // memory element A(off + a, off + b)
+                       const T v = convert<T>(x);
+                       // as an element of op(A): (a, b) for N, (b, a) for T/C
+                       int r = (op == OP_N) ? a : b, c = (op == OP_N) ? b : a;
+                       T e = (op == OP_C) ? conjx(v) : v;
+                       if (!lowerlike) { const int t = r; r = c; c = t; e = conjx(e); }
+                       if (r >= c) S[c * LS + r] = e;
+                     });
   __syncthreads();
   if (!unit) diag_reciprocals<T, NMAX>(S, dinv, n);
   __syncthreads();

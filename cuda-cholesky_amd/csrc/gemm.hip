@@ -190,21 +190,38 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
 #pragma unroll
       for (int j = 0; j < BJ; j++) acc[q][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
-  // Real data with alpha = +-1 and beta != 0: during K step b < BI * BJ the
-  // wave also reads the C values of its accumulator block b and folds
-  // (beta / alpha) C into it (exact scaling), so reading C is spread over the
-  // K loop instead of landing on the epilogue in one burst; the epilogue then
-  // only stores alpha * acc for those blocks.  npre blocks are pre-added.
-  // Not with split K: there each split writes its raw partial sum and C
-  // enters once, in the reduction.
+  // Real data with alpha = +-1 and beta != 0: the accumulators start from
+  // (beta / alpha) C (exact scaling), read from clamped addresses right after
+  // the first operand loads, so the C read overlaps them and the first
+  // products; the epilogue then only stores alpha * acc.  npre = the number
+  // of accumulator blocks seeded this way (all of them, or none).  Not with
+  // split K: there each split writes its raw partial sum and C enters once,
+  // in the reduction.
   int npre = 0;
   double cfac = 0.0;
   if constexpr (!CPLX) {
     if (!is_zero(p.beta) && (p.alpha == 1.0 || p.alpha == -1.0) && p.part == nullptr) {
       cfac = p.beta / p.alpha;
-      npre = BI * BJ;  // reduced to the number of K steps below
+      npre = BI * BJ;
     }
   }
+  auto seed_c = [&]() __attribute__((always_inline)) {
+    if constexpr (!CPLX) {
+      if (npre > 0) {  // uniform
+#pragma unroll
+        for (int i = 0; i < BI; i++) {
+          const int gm = min(m0 + wm0 + 16 * i + lc, p.m - 1);
+#pragma unroll
+          for (int j = 0; j < BJ; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int gn = min(n0 + wn0 + 16 * j + lg + 4 * r, p.n - 1);
+              acc[0][i][j][r] = cfac * p.C[(long)gn * p.ldc + gm];
+            }
+        }
+      }
+    }
+  };
 
   if (kbeg < kend) {
     // Per-thread source pointers at k = kbeg, and their per-element /
@@ -276,8 +293,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     };
 
     const int ntiles = (kend - kbeg + KT - 1) / KT;
-    if (npre > ntiles) npre = ntiles;
     load(0, pa, pb);
+    seed_c();
     store(0);
     __syncthreads();
     for (int kt = 0; kt < ntiles; kt++) {
@@ -286,19 +303,6 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
         pa += a_kstride;
         pb += b_kstride;
         load((kt + 1) * KT, pa, pb);
-      }
-      double cpre[4] = {0.0, 0.0, 0.0, 0.0};
-      if constexpr (!CPLX) {
-        if (kt < npre) {  // C values of accumulator block kt (uniform branch)
-          const int gm = m0 + wm0 + 16 * (kt / BJ) + lc;
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int gn = n0 + wn0 + 16 * (kt % BJ) + lg + 4 * r;
-            const bool ok = gm < p.m && gn < p.n;
-            const double v = p.C[ok ? (long)gn * p.ldc + gm : 0];
-            cpre[r] = ok ? v : 0.0;
-          }
-        }
       }
       const double *cA = sA + (kt & 1) * BUFA, *cB = sB + (kt & 1) * BUFB;
 #pragma unroll
@@ -332,26 +336,13 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
             }
         }
       }
-      if constexpr (!CPLX) {
-        if (kt < npre) {  // select, not multiply: C may hold NaN outside the output triangle
-#pragma unroll
-          for (int i = 0; i < BI; i++)
-#pragma unroll
-            for (int j = 0; j < BJ; j++)
-#pragma unroll
-              for (int r = 0; r < 4; r++) {
-                const double a = acc[0][i][j][r];
-                acc[0][i][j][r] = (i * BJ + j == kt) ? a + cfac * cpre[r] : a;
-              }
-        }
-      }
       // The other buffer was last read in the previous step, which every
       // thread finished before the previous barrier.
       if (more) store((kt + 1) & 1);
       __syncthreads();
     }
   } else {
-    npre = 0;
+    seed_c();
   }
 
   // ---- epilogue: lane lc holds row m; register r of lane group lg holds
