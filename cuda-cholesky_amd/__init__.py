@@ -55,6 +55,8 @@ _res = ctypes.c_int        # CUresult
 _stream = ctypes.c_void_p  # CUstream
 _longp = ctypes.POINTER(ctypes.c_long)
 _h = ctypes.c_void_p
+_hp = ctypes.c_void_p      # host array (the cuMultiGPU* routines)
+_taskfn = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)   # CUresult (*)(const void *)
 
 # name -> argument types (restype CUresult unless noted); mirrors
 # include/blas.h and include/lapack.h
@@ -115,6 +117,42 @@ SIGNATURES = {
     "cuLAPACKSetBlockSize": [_h, ctypes.c_char, ctypes.c_long],
     "cuLAPACKGetTiming": [_h, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.c_long, _longp],
+    # include/cumultigpu.h and the host-pointer halves of blas.h / lapack.h
+    "cuTaskCreate": [ctypes.POINTER(_h), _taskfn, ctypes.c_void_p, _sz],
+    "cuTaskDestroy": [_h, ctypes.POINTER(_res)],
+    "cuTaskExecute": [_h],
+    "cuMultiGPUCreate": [ctypes.POINTER(_h), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
+    "cuMultiGPUDestroy": [_h],
+    "cuMultiGPUGetContextCount": [_h],
+    "cuMultiGPURunTask": [_h, ctypes.c_int, _h],
+    "cuMultiGPUSynchronize": [_h],
+    "cuMultiGPUBLASCreate": [ctypes.POINTER(_h), _h],
+    "cuMultiGPUBLASDestroy": [_h],
+    "cuMultiGPUBLASSynchronize": [_h],
+    "cuMultiGPUDsyrk": [_h, _enum, _enum, _sz, _sz, ctypes.c_double, _hp, _sz, ctypes.c_double, _hp,
+                        _sz],
+    "cuMultiGPUZherk": [_h, _enum, _enum, _sz, _sz, ctypes.c_double, _hp, _sz, ctypes.c_double, _hp,
+                        _sz],
+    "cuMultiGPUDgemm": [_h, _enum, _enum, _sz, _sz, _sz, ctypes.c_double, _hp, _sz, _hp, _sz,
+                        ctypes.c_double, _hp, _sz],
+    "cuMultiGPUZgemm": [_h, _enum, _enum, _sz, _sz, _sz, _Z, _hp, _sz, _hp, _sz, _Z, _hp, _sz],
+    "cuMultiGPUDtrmm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_double, _hp, _sz, _hp,
+                        _sz],
+    "cuMultiGPUZtrmm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _Z, _hp, _sz, _hp, _sz],
+    "cuMultiGPUDtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_double, _hp, _sz, _hp,
+                        _sz],
+    "cuMultiGPUZtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _Z, _hp, _sz, _hp, _sz],
+    "cuMultiGPULAPACKCreate": [ctypes.POINTER(_h), _h],
+    "cuMultiGPULAPACKDestroy": [_h],
+    "cuMultiGPULAPACKSynchronize": [_h],
+    "cuMultiGPUDpotrf": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUZpotrf": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUDtrtri": [_h, _enum, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUZtrtri": [_h, _enum, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUDlauum": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUZlauum": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUDpotri": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUZpotri": [_h, _enum, _sz, _hp, _sz, _longp],
 }
 
 XERBLA_T = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_long)
@@ -421,6 +459,181 @@ def cuStrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream
 def cuZtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
     return lib().cuZtrsm(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), A, lda, B, ldb,
                          stream)
+
+
+# ---- host-pointer API (include/cumultigpu.h; cuMultiGPU* of blas.h / lapack.h) ----
+
+TaskFunction = _taskfn
+
+
+class Task:
+    """cuTaskCreate / cuTaskExecute / cuTaskDestroy; `function` is a
+    TaskFunction (keep it alive while the task exists), `args` bytes."""
+
+    def __init__(self, function, args=b""):
+        self.h = _h()
+        self._fn = function
+        buf = ctypes.create_string_buffer(args, len(args)) if args else None
+        st = lib().cuTaskCreate(ctypes.byref(self.h), function, buf, len(args))
+        if st != CUDA_SUCCESS:
+            raise RuntimeError(f"cuTaskCreate failed: {st}")
+
+    def execute(self):
+        return lib().cuTaskExecute(self.h)
+
+    def destroy(self):
+        """(status, the task's own result)."""
+        r = _res(0)
+        st = lib().cuTaskDestroy(self.h, ctypes.byref(r))
+        self.h = _h()
+        return st, r.value
+
+
+class MultiGPU:
+    """A CUmultiGPU context over `devices` (HIP device ids); one device is
+    used."""
+
+    def __init__(self, devices=(0,)):
+        self.h = _h()
+        arr = (ctypes.c_int * len(devices))(*devices)
+        st = lib().cuMultiGPUCreate(ctypes.byref(self.h), arr, len(devices))
+        if st != CUDA_SUCCESS:
+            raise RuntimeError(f"cuMultiGPUCreate failed: {st}")
+
+    def context_count(self):
+        return lib().cuMultiGPUGetContextCount(self.h)
+
+    def run_task(self, i, task):
+        return lib().cuMultiGPURunTask(self.h, i, _hv(task))
+
+    def synchronize(self):
+        return lib().cuMultiGPUSynchronize(self.h)
+
+    def close(self):
+        if self.h:
+            lib().cuMultiGPUDestroy(self.h)
+            self.h = _h()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class _MultiGPUHandle:
+    _create = _destroy = _sync = None
+
+    def __init__(self, mgpu):
+        self.h = _h()
+        st = getattr(lib(), self._create)(ctypes.byref(self.h), _hv(mgpu))
+        if st != CUDA_SUCCESS:
+            raise RuntimeError(f"{self._create} failed: {st}")
+
+    def synchronize(self):
+        return getattr(lib(), self._sync)(self.h)
+
+    def close(self):
+        if self.h:
+            getattr(lib(), self._destroy)(self.h)
+            self.h = _h()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class MultiGPUBLASHandle(_MultiGPUHandle):
+    _create, _destroy, _sync = "cuMultiGPUBLASCreate", "cuMultiGPUBLASDestroy", "cuMultiGPUBLASSynchronize"
+
+
+class MultiGPULAPACKHandle(_MultiGPUHandle):
+    _create, _destroy, _sync = ("cuMultiGPULAPACKCreate", "cuMultiGPULAPACKDestroy",
+                                "cuMultiGPULAPACKSynchronize")
+
+
+def _host(M):
+    """Address of a host array: a column-major numpy array (its leading
+    dimension is M.shape[0]), a raw address, or None."""
+    if M is None or isinstance(M, int):
+        return M
+    if not (M.flags.f_contiguous or M.ndim == 1):
+        raise ValueError("host matrices must be Fortran-ordered")
+    return M.ctypes.data
+
+
+def cuMultiGPUDgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return lib().cuMultiGPUDgemm(_hv(handle), transA, transB, m, n, k, alpha, _host(A), lda, _host(B),
+                                 ldb, beta, _host(C), ldc)
+
+
+def cuMultiGPUZgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return lib().cuMultiGPUZgemm(_hv(handle), transA, transB, m, n, k, zscalar(alpha), _host(A), lda,
+                                 _host(B), ldb, zscalar(beta), _host(C), ldc)
+
+
+def cuMultiGPUDsyrk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    return lib().cuMultiGPUDsyrk(_hv(handle), uplo, trans, n, k, alpha, _host(A), lda, beta, _host(C),
+                                 ldc)
+
+
+def cuMultiGPUZherk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    return lib().cuMultiGPUZherk(_hv(handle), uplo, trans, n, k, alpha, _host(A), lda, beta, _host(C),
+                                 ldc)
+
+
+def cuMultiGPUDtrmm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUDtrmm(_hv(handle), side, uplo, trans, diag, m, n, alpha, _host(A), lda,
+                                 _host(B), ldb)
+
+
+def cuMultiGPUZtrmm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUZtrmm(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), _host(A),
+                                 lda, _host(B), ldb)
+
+
+def cuMultiGPUDtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUDtrsm(_hv(handle), side, uplo, trans, diag, m, n, alpha, _host(A), lda,
+                                 _host(B), ldb)
+
+
+def cuMultiGPUZtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUZtrsm(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), _host(A),
+                                 lda, _host(B), ldb)
+
+
+def cuMultiGPUDpotrf(handle, uplo, n, A, lda):
+    return _info_call(lib().cuMultiGPUDpotrf, _hv(handle), uplo, n, _host(A), lda)
+
+
+def cuMultiGPUZpotrf(handle, uplo, n, A, lda):
+    return _info_call(lib().cuMultiGPUZpotrf, _hv(handle), uplo, n, _host(A), lda)
+
+
+def cuMultiGPUDtrtri(handle, uplo, diag, n, A, lda):
+    return _info_call(lib().cuMultiGPUDtrtri, _hv(handle), uplo, diag, n, _host(A), lda)
+
+
+def cuMultiGPUZtrtri(handle, uplo, diag, n, A, lda):
+    return _info_call(lib().cuMultiGPUZtrtri, _hv(handle), uplo, diag, n, _host(A), lda)
+
+
+def cuMultiGPUDlauum(handle, uplo, n, A, lda):
+    return _info_call(lib().cuMultiGPUDlauum, _hv(handle), uplo, n, _host(A), lda)
+
+
+def cuMultiGPUZlauum(handle, uplo, n, A, lda):
+    return _info_call(lib().cuMultiGPUZlauum, _hv(handle), uplo, n, _host(A), lda)
+
+
+def cuMultiGPUDpotri(handle, uplo, n, A, lda):
+    return _info_call(lib().cuMultiGPUDpotri, _hv(handle), uplo, n, _host(A), lda)
+
+
+def cuMultiGPUZpotri(handle, uplo, n, A, lda):
+    return _info_call(lib().cuMultiGPUZpotri, _hv(handle), uplo, n, _host(A), lda)
 
 
 # ---- device matrices (torch is only the allocator here) ----

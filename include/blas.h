@@ -18,6 +18,7 @@
 #include <stddef.h>
 #include <stdbool.h>
 #include "cu_abi.h"
+#include "cumultigpu.h"
 
 #ifdef __cplusplus
 typedef _Complex double cuDoubleComplex_t;
@@ -121,6 +122,50 @@ CUresult cuStrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
                  size_t, size_t,
                  float, CUdeviceptr, size_t,
                  CUdeviceptr, size_t, CUstream);
+
+/*
+ * Host-pointer BLAS of the reference's multi-GPU layer
+ * (include/blas.h:257-360), collapsed onto the one device of the CUmultiGPU
+ * context: operands are column-major host arrays, copied to the device
+ * through double-buffered pinned staging, the device routine above runs, and
+ * the result is copied back before the call returns (the reference may still
+ * be running its tasks until cuMultiGPUBLASSynchronize; here that call only
+ * waits for the device).  Same argument order, in-place results and xerbla
+ * indices as the reference.  Double and double complex.
+ */
+typedef struct __cumultigpublashandle_st * CUmultiGPUBLAShandle;
+CUresult cuMultiGPUBLASCreate(CUmultiGPUBLAShandle *, CUmultiGPU);
+CUresult cuMultiGPUBLASDestroy(CUmultiGPUBLAShandle);
+CUresult cuMultiGPUBLASSynchronize(CUmultiGPUBLAShandle);
+
+/* (replaces cuMultiGPUDsyrk, reference blas/dsyrk.c:212) */
+CUresult cuMultiGPUDsyrk(CUmultiGPUBLAShandle, CBlasUplo, CBlasTranspose, size_t, size_t,
+                         double, const double *, size_t, double, double *, size_t);
+/* (replaces cuMultiGPUZherk, reference blas/zherk.c:241) */
+CUresult cuMultiGPUZherk(CUmultiGPUBLAShandle, CBlasUplo, CBlasTranspose, size_t, size_t,
+                         double, const cuDoubleComplex_t *, size_t, double, cuDoubleComplex_t *, size_t);
+/* (replaces cuMultiGPUDgemm, reference blas/dgemm.c:435) */
+CUresult cuMultiGPUDgemm(CUmultiGPUBLAShandle, CBlasTranspose, CBlasTranspose, size_t, size_t, size_t,
+                         double, const double *, size_t, const double *, size_t, double, double *, size_t);
+/* (replaces cuMultiGPUZgemm, reference blas/zgemm.c:519) */
+CUresult cuMultiGPUZgemm(CUmultiGPUBLAShandle, CBlasTranspose, CBlasTranspose, size_t, size_t, size_t,
+                         cuDoubleComplex_t, const cuDoubleComplex_t *, size_t, const cuDoubleComplex_t *,
+                         size_t, cuDoubleComplex_t, cuDoubleComplex_t *, size_t);
+/* B <- alpha op(A) B or alpha B op(A), in place (replaces cuMultiGPUDtrmm,
+ * reference blas/dtrmm.c:468) */
+CUresult cuMultiGPUDtrmm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, double, const double *, size_t, double *, size_t);
+/* (replaces cuMultiGPUZtrmm, reference blas/ztrmm.c:519) */
+CUresult cuMultiGPUZtrmm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, cuDoubleComplex_t, const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *,
+                         size_t);
+/* (replaces cuMultiGPUDtrsm, reference blas/dtrsm.c:268) */
+CUresult cuMultiGPUDtrsm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, double, const double *, size_t, double *, size_t);
+/* (replaces cuMultiGPUZtrsm, reference blas/ztrsm.c:297) */
+CUresult cuMultiGPUZtrsm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, cuDoubleComplex_t, const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *,
+                         size_t);
 
 #ifdef __cplusplus
 }

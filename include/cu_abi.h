@@ -33,6 +33,9 @@ typedef uintptr_t CUdeviceptr;
 
 typedef struct ihipStream_t * CUstream;
 
+/* Device ordinal (the HIP device id). */
+typedef int CUdevice;
+
 typedef enum {
   CUDA_SUCCESS                              = 0,
   CUDA_ERROR_INVALID_VALUE                  = 1,

@@ -78,6 +78,34 @@ CUresult cuDlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, double *, CUstre
  * single precision). */
 CUresult cuZlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, double *, CUstream);
 
+/*
+ * Host-pointer LAPACK of the reference's multi-GPU layer
+ * (include/lapack.h:117-158), collapsed onto the one device of the
+ * CUmultiGPU context: the matrix (column-major host array, in place) is
+ * copied to the device through double-buffered pinned staging, factored /
+ * inverted by the device routines above and copied back.  As in the
+ * reference, a numerical failure (info > 0) returns
+ * CUDA_ERROR_INVALID_VALUE; an illegal argument calls xerbla with the
+ * reference's index.  Double and double complex.
+ */
+typedef struct __cumultigpulapackhandle_st * CUmultiGPULAPACKhandle;
+CUresult cuMultiGPULAPACKCreate(CUmultiGPULAPACKhandle *, CUmultiGPU);
+CUresult cuMultiGPULAPACKDestroy(CUmultiGPULAPACKhandle);
+CUresult cuMultiGPULAPACKSynchronize(CUmultiGPULAPACKhandle);
+/* (replaces cuMultiGPUDpotrf / cuMultiGPUZpotrf, reference lapack/dpotrf.c:298, lapack/zpotrf.c:300) */
+CUresult cuMultiGPUDpotrf(CUmultiGPULAPACKhandle, CBlasUplo, size_t, double *, size_t, long *);
+CUresult cuMultiGPUZpotrf(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+/* (replaces cuMultiGPUDtrtri / cuMultiGPUZtrtri, reference lapack/dtrtri.c:470, lapack/ztrtri.c:470) */
+CUresult cuMultiGPUDtrtri(CUmultiGPULAPACKhandle, CBlasUplo, CBlasDiag, size_t, double *, size_t, long *);
+CUresult cuMultiGPUZtrtri(CUmultiGPULAPACKhandle, CBlasUplo, CBlasDiag, size_t, cuDoubleComplex_t *, size_t,
+                          long *);
+/* (replaces cuMultiGPUDlauum / cuMultiGPUZlauum, reference lapack/dlauum.c:294, lapack/zlauum.c:293) */
+CUresult cuMultiGPUDlauum(CUmultiGPULAPACKhandle, CBlasUplo, size_t, double *, size_t, long *);
+CUresult cuMultiGPUZlauum(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+/* (replaces cuMultiGPUDpotri / cuMultiGPUZpotri, reference lapack/dpotri.c:48, lapack/zpotri.c:48) */
+CUresult cuMultiGPUDpotri(CUmultiGPULAPACKhandle, CBlasUplo, size_t, double *, size_t, long *);
+CUresult cuMultiGPUZpotri(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+
 #ifdef __cplusplus
 }
 #endif

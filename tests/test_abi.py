@@ -57,7 +57,14 @@ def test_headers_compile_as_c_and_cpp(tmp_path):
         "  CUresult r = cuDpotrf(h, CBlasLower, 4, A, 4, &info);\n"
         "  double complex one = 1.0;\n"
         "  return (int)r + (int)cuZgemm(0, CBlasNoTrans, CBlasConjTrans, 1, 1, 1, one, A, 1, A, 1,"
-        " one, A, 1, (CUstream)0); }\n")
+        " one, A, 1, (CUstream)0); }\n"
+        "int h(double *A, double complex *Z) { CUdevice dev = 0; CUmultiGPU mg; long info;\n"
+        "  CUmultiGPULAPACKhandle l; CUmultiGPUBLAShandle b; double complex one = 1.0;\n"
+        "  CUresult r = cuMultiGPUCreate(&mg, &dev, 1);\n"
+        "  r = cuMultiGPULAPACKCreate(&l, mg); r = cuMultiGPUBLASCreate(&b, mg);\n"
+        "  r = cuMultiGPUDpotrf(l, CBlasLower, 4, A, 4, &info);\n"
+        "  r = cuMultiGPUZgemm(b, CBlasNoTrans, CBlasConjTrans, 1, 1, 1, one, Z, 1, Z, 1, one, Z, 1);\n"
+        "  return (int)r + cuMultiGPUGetContextCount(mg); }\n")
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{INCLUDE}", str(src)],
                    check=True)
     cpp = tmp_path / "use.cpp"
@@ -188,3 +195,86 @@ def test_solver_argument_indices(cc):
 def test_solver_quick_returns(cc):
     assert cc.cuDpotrs(FAKE, cc.Lower, 0, 3, 0, 1, 0, 1) == (0, 0)
     assert cc.cuDsposv(FAKE, cc.Lower, 5, 0, 0, 5, 0, 5, 0, 5) == (0, 0, 0)
+
+
+# ---- host-pointer API (include/cumultigpu.h, cuMultiGPU* of blas.h / lapack.h)
+
+def test_task_runs_on_calling_thread_with_private_args(cc):
+    """reference multigpu/multigpu_seq.c:28-95: the task keeps a copy of its
+    arguments and hands its function's result back through cuTaskDestroy."""
+    seen = []
+
+    def body(args):
+        v = ctypes.cast(args, ctypes.POINTER(ctypes.c_int * 2)).contents
+        seen.append((v[0], v[1]))
+        return v[0] + v[1]
+
+    fn = cc.TaskFunction(body)
+    args = bytearray(ctypes.c_int(40)) + bytearray(ctypes.c_int(2))
+    t = cc.Task(fn, bytes(args))
+    args[0] = 99                      # the caller's buffer is not the task's
+    assert t.execute() == cc.CUDA_SUCCESS
+    st, result = t.destroy()
+    assert st == cc.CUDA_SUCCESS and result == 42 and seen == [(40, 2)]
+    t2 = cc.Task(cc.TaskFunction(lambda a: 7))   # no arguments
+    assert t2.execute() == 0 and t2.destroy() == (0, 7)
+
+
+def test_task_and_context_argument_errors(cc):
+    L = cc.lib()
+    h = ctypes.c_void_p()
+    fn = cc.TaskFunction(lambda a: 0)
+    assert L.cuTaskCreate(ctypes.byref(h), cc.TaskFunction(), None, 0) == cc.CUDA_ERROR_INVALID_VALUE
+    assert L.cuTaskCreate(ctypes.byref(h), fn, None, 8) == cc.CUDA_ERROR_INVALID_VALUE
+    dev = (ctypes.c_int * 1)(0)
+    assert L.cuMultiGPUCreate(ctypes.byref(h), dev, 0) == cc.CUDA_ERROR_INVALID_VALUE
+    assert L.cuMultiGPURunTask(None, 0, FAKE) == cc.CUDA_ERROR_INVALID_VALUE
+    assert L.cuMultiGPUGetContextCount(None) == 0
+    assert L.cuMultiGPUBLASCreate(ctypes.byref(h), None) == cc.CUDA_ERROR_INVALID_VALUE
+    assert L.cuMultiGPULAPACKCreate(ctypes.byref(h), None) == cc.CUDA_ERROR_INVALID_VALUE
+
+
+def test_multigpu_blas_argument_indices(cc):
+    """reference blas/dgemm.c:441-451 (8, 10, 13), blas/dsyrk.c:219-226 (7, 10),
+    blas/zherk.c:248-255 (trans = T -> 2), blas/dtrmm.c:474-481 and
+    blas/dtrsm.c:274-281 (9, 11): checked before anything is read."""
+    with cc.Xerbla() as x:
+        assert cc.cuMultiGPUDgemm(FAKE, cc.NoTrans, cc.NoTrans, 5, 4, 3, 1.0, 0, 4, 0, 3, 1.0, 0, 5) == 1
+        assert cc.cuMultiGPUDgemm(FAKE, cc.NoTrans, cc.Trans, 5, 4, 3, 1.0, 0, 5, 0, 3, 1.0, 0, 5) == 1
+        assert cc.cuMultiGPUZgemm(FAKE, cc.Trans, cc.NoTrans, 5, 4, 3, 1.0, 0, 3, 0, 3, 1.0, 0, 4) == 1
+        assert cc.cuMultiGPUDsyrk(FAKE, cc.Lower, cc.Trans, 5, 3, 1.0, 0, 2, 1.0, 0, 5) == 1
+        assert cc.cuMultiGPUDsyrk(FAKE, cc.Upper, cc.NoTrans, 5, 3, 1.0, 0, 5, 1.0, 0, 4) == 1
+        assert cc.cuMultiGPUZherk(FAKE, cc.Lower, cc.Trans, 5, 3, 1.0, 0, 5, 1.0, 0, 5) == 1
+        assert cc.cuMultiGPUDtrmm(FAKE, cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, 6, 4, 1.0, 0, 5, 0,
+                                  6) == 1
+        assert cc.cuMultiGPUZtrmm(FAKE, cc.Right, cc.Upper, cc.ConjTrans, cc.Unit, 6, 4, 1.0, 0, 4, 0,
+                                  5) == 1
+        assert cc.cuMultiGPUDtrsm(FAKE, cc.Right, cc.Lower, cc.Trans, cc.NonUnit, 6, 4, 1.0, 0, 3, 0,
+                                  6) == 1
+        assert cc.cuMultiGPUZtrsm(FAKE, cc.Left, cc.Upper, cc.NoTrans, cc.Unit, 6, 4, 1.0, 0, 6, 0,
+                                  5) == 1
+    assert x.calls == [("cuMultiGPUDgemm", 8), ("cuMultiGPUDgemm", 10), ("cuMultiGPUZgemm", 13),
+                       ("cuMultiGPUDsyrk", 7), ("cuMultiGPUDsyrk", 10), ("cuMultiGPUZherk", 2),
+                       ("cuMultiGPUDtrmm", 9), ("cuMultiGPUZtrmm", 11), ("cuMultiGPUDtrsm", 9),
+                       ("cuMultiGPUZtrsm", 11)]
+
+
+def test_multigpu_lapack_argument_indices(cc):
+    """reference lapack/dpotrf.c:302-308 (lda -> -4), lapack/dtrtri.c:475-481
+    (lda -> -5), lapack/dlauum.c:299-305, lapack/dpotri.c:53-59."""
+    with cc.Xerbla() as x:
+        for fn in ("Dpotrf", "Zpotrf", "Dlauum", "Zlauum", "Dpotri", "Zpotri"):
+            assert getattr(cc, "cuMultiGPU" + fn)(FAKE, cc.Lower, 10, 0, 9) == (1, -4)
+        for fn in ("Dtrtri", "Ztrtri"):
+            assert getattr(cc, "cuMultiGPU" + fn)(FAKE, cc.Upper, cc.NonUnit, 10, 0, 9) == (1, -5)
+    assert [i for _, i in x.calls] == [4] * 6 + [5] * 2
+
+
+def test_multigpu_quick_returns(cc):
+    # nothing to do: no device, no host array touched
+    assert cc.cuMultiGPUDgemm(FAKE, cc.NoTrans, cc.NoTrans, 0, 4, 3, 1.0, 0, 1, 0, 3, 1.0, 0, 1) == 0
+    assert cc.cuMultiGPUDgemm(FAKE, cc.NoTrans, cc.NoTrans, 4, 4, 3, 0.0, 0, 4, 0, 3, 1.0, 0, 4) == 0
+    assert cc.cuMultiGPUDsyrk(FAKE, cc.Lower, cc.NoTrans, 4, 0, 1.0, 0, 4, 1.0, 0, 4) == 0
+    assert cc.cuMultiGPUDtrsm(FAKE, cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, 0, 4, 1.0, 0, 1, 0, 1) == 0
+    assert cc.cuMultiGPUDpotrf(FAKE, cc.Lower, 0, 0, 1) == (0, 0)
+    assert cc.cuMultiGPUZtrtri(FAKE, cc.Lower, cc.Unit, 0, 0, 1) == (0, 0)
