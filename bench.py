@@ -280,6 +280,28 @@ def cpu_baseline(torch, n_cpu):
     }
 
 
+def host_api(torch, pkg, A0, n, reps=2):
+    """cuMultiGPUDpotrf on a host copy of A: the PCIe-inclusive rate of the
+    host-pointer API (upload, factor, download), after one call that sizes the
+    pinned staging and device buffers."""
+    import numpy as np
+    host = A0.cpu().numpy().T                  # symmetric: any storage order is A
+    out = {}
+    with pkg.MultiGPU((torch.cuda.current_device(),)) as ctx, pkg.MultiGPULAPACKHandle(ctx) as hl:
+        for rep in range(reps):
+            M = np.array(host, order="F")
+            t = time.perf_counter()
+            st, info = pkg.cuMultiGPUDpotrf(hl, pkg.Lower, n, M, n)
+            dt = time.perf_counter() - t
+            assert st == 0 and info == 0, (st, info)
+        L = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+        out["ms"] = round(dt * 1e3, 2)
+        out["tflops_pcie_inclusive"] = round(n ** 3 / 3 / dt / 1e12, 3)
+        out["bytes_moved"] = 2 * n * n * 8
+        out["backward_error"] = backward_error(torch, A0, L, "L")
+    return out
+
+
 def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs):
     """Batched mixed-precision solve + logdet + inverse (cuDsposvxBatched) of
     `count` FP64 SPD matrices A_i = G_i G_i^T / n_i + alpha_i I with n_i
@@ -371,6 +393,8 @@ def main():
     ap.add_argument("--no-batched-mixed", action="store_true",
                     help="skip the batched mixed-precision solve + logdet + inverse (512 matrices, "
                          "n log-uniform in [128, 2048])")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-pointer cuMultiGPUDpotrf leg (PCIe-inclusive)")
     ap.add_argument("--counter-calibration", action="store_true",
                     help="run one elementwise copy of A (n^2 * 8 bytes read and written) before the "
                          "steps, as the known-traffic reference kernel of a rocprofv3 --pmc run")
@@ -461,6 +485,10 @@ def main():
         extra["zpotri_upper_8192_tflops_real_n3"] = round(4 * nz ** 3 / dtz / 1e12, 3)
         del Z, Z0, Zc
         log("[bench] complex done")
+
+    if not args.no_host:
+        extra["host_pointer_dpotrf"] = host_api(torch, pkg, A0, n)
+        log(f"[bench] cuMultiGPUDpotrf (host arrays) n={n}: {extra['host_pointer_dpotrf']['ms']} ms")
 
     if not args.no_mixed:
         del A
