@@ -43,22 +43,41 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
-// Gather of `total` global elements: use(idx, A[addr(idx)]) for idx in
-// [0, total), U loads per thread in flight together (addresses clamped into
-// range, so every load is valid and none waits on the previous one; a plain
-// strided loop would pay one memory round trip per element).
-template <int U, typename T, typename Addr, typename Use>
-__device__ __forceinline__ void batched_gather(int total, const T *A, Addr addr, Use use) {
-  for (int base = threadIdx.x; base < total; base += U * (int)blockDim.x) {
+// Visits the n x n block at A (column-major, 1 <= n <= NMAX) as
+// use(i, k, A(i, k)), i the contiguous index: thread t takes row t % NMAX of
+// columns t / NMAX + j * (THREADS / NMAX), U loads in flight together from
+// clamped (always valid) addresses.  Nothing divides by the runtime n: a
+// 32-bit division is a ~40-instruction sequence here, and per-element
+// divisions used to cost the diagonal-block kernels more than their loads.
+template <int NMAX, int THREADS, int U, typename T, typename Use>
+__device__ __forceinline__ void gather_block(int n, const T *A, long lda, Use use) {
+  static_assert(THREADS % NMAX == 0, "whole rows per sweep");
+  constexpr int STEP = THREADS / NMAX, PER = (NMAX + STEP - 1) / STEP;
+  const int i = (int)threadIdx.x % NMAX, k0 = (int)threadIdx.x / NMAX;
+  const int ic = min(i, n - 1);
+#pragma unroll
+  for (int j0 = 0; j0 < PER; j0 += U) {
+    if (k0 + j0 * STEP >= n) break;
     T v[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) v[u] = A[addr(min(base + u * (int)blockDim.x, total - 1))];
+    for (int u = 0; u < U; u++) v[u] = A[(long)min(k0 + (j0 + u) * STEP, n - 1) * lda + ic];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int idx = base + u * (int)blockDim.x;
-      if (idx < total) use(idx, v[u]);
+      const int k = k0 + (j0 + u) * STEP;
+      if (i < n && k < n) use(i, k, v[u]);
     }
   }
+}
+
+// fn(i, k) for every element of an n x n block, with the thread mapping of
+// gather_block (stores to column-major memory stay contiguous across lanes).
+template <int NMAX, int THREADS, typename Fn>
+__device__ __forceinline__ void for_block(int n, Fn fn) {
+  static_assert(THREADS % NMAX == 0, "whole rows per sweep");
+  constexpr int STEP = THREADS / NMAX;
+  const int i = (int)threadIdx.x % NMAX, k0 = (int)threadIdx.x / NMAX;
+  if (i < n)
+    for (int k = k0; k < n; k += STEP) fn(i, k);
 }
 
 // S is column-major with leading dimension NMAX + 1: S(r, c) = S[c * LS + r].
@@ -70,51 +89,47 @@ struct Lds {
 
 // Load the lower form of the n x n block at A: S(r, c) = A(r, c) (lower) or
 // conj(A(c, r)) (upper), r >= c.  Memory is read along its contiguous index.
-template <typename T, int NMAX>
+template <typename T, int NMAX, int THREADS>
 __device__ __forceinline__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
   constexpr int LS = Lds<T, NMAX>::LS;
   // a: contiguous index in memory; the unreferenced triangle is read but not used
-  batched_gather<16>(n * n, A, [&](int idx) { return (long)(idx / n) * lda + idx % n; },
-                     [&](int idx, T v) {
-                       const int a = idx % n, b = idx / n;
-                       if (lower) {
-                         if (a >= b) S[b * LS + a] = v;
-                       } else {
-                         // A(a, b) with a <= b is upper; it is conj of lower-form (b, a)
-                         if (a <= b) S[a * LS + b] = conjx(v);
-                       }
-                     });
+  gather_block<NMAX, THREADS, 16>(n, A, lda, [&](int a, int b, T v) {
+    if (lower) {
+      if (a >= b) S[b * LS + a] = v;
+    } else {
+      // A(a, b) with a <= b is upper; it is conj of lower-form (b, a)
+      if (a <= b) S[a * LS + b] = conjx(v);
+    }
+  });
 }
 
 // Store the lower form back into the caller's triangle (diagonal included
 // unless skip_diag).
-template <typename T, int NMAX>
+template <typename T, int NMAX, int THREADS>
 __device__ __forceinline__ void store_lower_form(const T *S, bool lower, int n, T *A, long lda, bool skip_diag) {
   constexpr int LS = Lds<T, NMAX>::LS;
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int a = idx % n, b = idx / n;
-    if (skip_diag && a == b) continue;
+  for_block<NMAX, THREADS>(n, [&](int a, int b) {
+    if (skip_diag && a == b) return;
     if (lower) {
       if (a >= b) A[(long)b * lda + a] = S[b * LS + a];
     } else {
       if (a <= b) A[(long)b * lda + a] = conjx(S[a * LS + b]);
     }
-  }
+  });
 }
 
 // W(r, c) = S(r, c) for r >= c, zero above; unit -> ones on the diagonal.
 // With upper, W is the conjugate transpose instead (upper triangular).
-template <typename T, int NMAX, typename ST = T>
+template <typename T, int NMAX, int THREADS, typename ST = T>
 __device__ __forceinline__ void store_dense(const T *S, bool upper, int n, bool unit, ST *W, long ldw) {
   constexpr int LS = Lds<T, NMAX>::LS;
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int r = idx % n, c = idx / n;
+  for_block<NMAX, THREADS>(n, [&](int r, int c) {
     T v = from_real<T>(0.0);
     if (r == c) v = unit ? from_real<T>(1.0) : (upper ? conjx(S[c * LS + r]) : S[c * LS + r]);
     else if (!upper && r > c) v = S[c * LS + r];
     else if (upper && r < c) v = conjx(S[r * LS + c]);
     W[(long)c * ldw + r] = convert<ST>(v);
-  }
+  });
 }
 
 // In-place R = L^H L of the lower form (lower part), columns left to right:
@@ -294,24 +309,29 @@ template <typename T, typename ST>
 __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int n, int g0, ST *A,
                                                 long lda) {
   const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-  T a[4];
+  T a[LEAF];  // row r (row layout: the four lane groups hold the same row)
 #pragma unroll
-  for (int j = 0; j < 4; j++) a[j] = S[(q + 4 * j) * ls + r];
+  for (int c = 0; c < LEAF; c++) a[c] = S[c * ls + r];
   double unused = 0.0;
-  const int bad = t16::leaf_factor<T>(a, 0, &unused);
+  const int bad = t16::leaf_factor_rows<T>(a, 0, &unused);
   if (bad >= 0) return bad;
+  // lane group q writes columns q + 4 j of its row
+  auto pick = [&](const T *v, int j) {
+    return sel(q == 0, v[4 * j], sel(q == 1, v[4 * j + 1], sel(q == 2, v[4 * j + 2], v[4 * j + 3])));
+  };
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int c = q + 4 * j, gr = g0 + r, gc = g0 + c;
+    const T v = pick(a, j);
     if (r >= c && gr < n) {
-      if (lower) A[(long)gc * lda + gr] = convert<ST>(a[j]);
-      else A[(long)gr * lda + gc] = convert<ST>(conjx(a[j]));
+      if (lower) A[(long)gc * lda + gr] = convert<ST>(v);
+      else A[(long)gr * lda + gc] = convert<ST>(conjx(v));
     }
   }
-  T x[4];
-  t16::leaf_inverse<T>(a, x);
+  T x[LEAF];
+  t16::leaf_inverse_rows<T>(a, x);
 #pragma unroll
-  for (int j = 0; j < 4; j++) S[(q + 4 * j) * ls + r] = x[j];
+  for (int j = 0; j < 4; j++) S[(q + 4 * j) * ls + r] = pick(x, j);
   return -1;
 }
 
@@ -335,18 +355,14 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
   auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
   auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
   // identity padding (rows / columns n .. npad - 1 of the lower form)
-  for (int idx = tid; idx < npad * npad; idx += BLK_THREADS) {
-    const int r = idx % npad, c = idx / npad;
-    if (r >= c && r >= n) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
-  }
-  // lower form, read along memory's contiguous index (i), all loads of a
+  for (int c = tid; c < npad; c += BLK_THREADS)
+    for (int r = max(n, c); r < npad; r++) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
+  // lower form, read along memory's contiguous index (i), the loads of a
   // thread in flight together
-  batched_gather<16>(n * n, A, [&](int idx) { return (long)(idx / n) * lda + idx % n; },
-                     [&](int idx, ST x) {
-                       const int i = idx % n, k = idx / n;
-                       const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
-                       if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
-                     });
+  gather_block<NMAX, BLK_THREADS, 16>(n, A, lda, [&](int i, int k, ST x) {
+    const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
+    if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
+  });
   if (tid == 0) flag = -1;
   __syncthreads();
   for (int p = 0; p < np; p++) {
@@ -378,14 +394,13 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
     __syncthreads();
   }
   // off-diagonal tiles of the factor (the diagonal ones are already out)
-  for (int idx = tid; idx < n * n; idx += BLK_THREADS) {
-    const int i = idx % n, k = idx / n;
+  for_block<NMAX, BLK_THREADS>(n, [&](int i, int k) {
     const int r = lower ? i : k, c = lower ? k : i;
-    if (r / LEAF <= c / LEAF) continue;
+    if (r / LEAF <= c / LEAF) return;
     const T v = elem(r, c);
     if (lower) A[(long)c * lda + r] = convert<ST>(v);
     else A[(long)r * lda + c] = convert<ST>(conjx(v));
-  }
+  });
   __syncthreads();
   for (int p = np - 2; p >= 0; p--) {
     // column block p below the diagonal: B_I <- sum_{J = p+1..I} X_IJ L_Jp,
@@ -408,10 +423,9 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
     __syncthreads();
   }
   // W = inv(L) dense, zeros above the diagonal
-  for (int idx = tid; idx < n * n; idx += BLK_THREADS) {
-    const int r = idx % n, c = idx / n;
+  for_block<NMAX, BLK_THREADS>(n, [&](int r, int c) {
     W[(long)c * ldw + r] = convert<ST>(r >= c ? elem(r, c) : from_real<T>(0.0));
-  }
+  });
 }
 
 template <typename T, int NMAX>
@@ -423,7 +437,7 @@ diag_trtri_kernel(int lower, int unit, int n, const T *A, long lda, T *B, long l
   __shared__ T S[NMAX * (NMAX + 1)];
   __shared__ T vec[2 * NMAX];
   __shared__ T dinv[NMAX];
-  load_lower_form<T, NMAX>(S, lower, n, A, lda);
+  load_lower_form<T, NMAX, DIAG_THREADS>(S, lower, n, A, lda);
   __syncthreads();
   if (!unit) diag_reciprocals<T, NMAX>(S, dinv, n);
   __syncthreads();
@@ -432,17 +446,17 @@ diag_trtri_kernel(int lower, int unit, int n, const T *A, long lda, T *B, long l
   __syncthreads();
   reg_to_lds<T, NMAX>(x, S, n);
   __syncthreads();
-  store_lower_form<T, NMAX>(S, lower, n, B, ldb, unit != 0);
-  if (W != nullptr) store_dense<T, NMAX>(S, false, n, unit != 0, W, ldw);
+  store_lower_form<T, NMAX, DIAG_THREADS>(S, lower, n, B, ldb, unit != 0);
+  if (W != nullptr) store_dense<T, NMAX, DIAG_THREADS>(S, false, n, unit != 0, W, ldw);
 }
 
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(LAUUM_THREADS) diag_lauum_kernel(int lower, int n, T *A, long lda) {
   __shared__ T S[NMAX * (NMAX + 1)];
-  load_lower_form<T, NMAX>(S, lower, n, A, lda);
+  load_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, A, lda);
   __syncthreads();
   lds_lauum<T, NMAX>(S, n);
-  store_lower_form<T, NMAX>(S, lower, n, A, lda, false);
+  store_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, A, lda, false);
 }
 
 // Block b inverts the nb x nb diagonal block of op(A) starting at b * nb.
@@ -457,16 +471,15 @@ trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const ST *A, lo
   const int n = (int)((m - off) < nb ? (m - off) : nb);
   // lower form: S(r, c) = op(A)(r, c) when op(A) is lower-like, else
   // conj(op(A)(c, r)); op(A)(i, j) = A(i, j), A(j, i) or conj(A(j, i)).
-  batched_gather<16>(n * n, A + off * lda + off, [&](int idx) { return (long)(idx / n) * lda + idx % n; },
-                     [&](int idx, ST x) {
-                       const int a = idx % n, b = idx / n;  // memory element A(off + a, off + b)
-                       const T v = convert<T>(x);
-                       // as an element of op(A): (a, b) for N, (b, a) for T/C
-                       int r = (op == OP_N) ? a : b, c = (op == OP_N) ? b : a;
-                       T e = (op == OP_C) ? conjx(v) : v;
-                       if (!lowerlike) { const int t = r; r = c; c = t; e = conjx(e); }
-                       if (r >= c) S[c * LS + r] = e;
-                     });
+  gather_block<NMAX, DIAG_THREADS, 16>(n, A + off * lda + off, lda, [&](int a, int b, ST x) {
+    // memory element A(off + a, off + b)
+    const T v = convert<T>(x);
+    // as an element of op(A): (a, b) for N, (b, a) for T/C
+    int r = (op == OP_N) ? a : b, c = (op == OP_N) ? b : a;
+    T e = (op == OP_C) ? conjx(v) : v;
+    if (!lowerlike) { const int t = r; r = c; c = t; e = conjx(e); }
+    if (r >= c) S[c * LS + r] = e;
+  });
   __syncthreads();
   if (!unit) diag_reciprocals<T, NMAX>(S, dinv, n);
   __syncthreads();
@@ -475,7 +488,7 @@ trsm_inv_kernel(int lowerlike, int op, int unit, long m, int nb, const ST *A, lo
   __syncthreads();
   reg_to_lds<T, NMAX>(x, S, n);
   __syncthreads();
-  store_dense<T, NMAX, ST>(S, !lowerlike, n, unit != 0, W + (long)blockIdx.x * nb * nb, nb);
+  store_dense<T, NMAX, DIAG_THREADS, ST>(S, !lowerlike, n, unit != 0, W + (long)blockIdx.x * nb * nb, nb);
 }
 
 // One workgroup scans the diagonal; info = first (upper) or last (lower)
