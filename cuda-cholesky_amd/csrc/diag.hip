@@ -308,7 +308,9 @@ constexpr int BLK_THREADS = 512;
 template <typename T, typename ST>
 __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int n, int g0, ST *A,
                                                 long lda) {
-  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  int l = threadIdx.x & 63;
+  asm volatile("" : "+v"(l));  // lane-derived values computed here, not kept live across the kernel's loop
+  const int r = l & 15, q = l >> 4;
   T a[LEAF];  // row r (row layout: the four lane groups hold the same row)
 #pragma unroll
   for (int c = 0; c < LEAF; c++) a[c] = S[c * ls + r];
@@ -328,10 +330,10 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
       else A[(long)gr * lda + gc] = convert<ST>(conjx(v));
     }
   }
-  T x[LEAF];
-  t16::leaf_inverse_rows<T>(a, x);
+  T x[4];
+  t16::leaf_inverse_cols<T>(a, x);
 #pragma unroll
-  for (int j = 0; j < 4; j++) S[(q + 4 * j) * ls + r] = pick(x, j);
+  for (int j = 0; j < 4; j++) S[(q + 4 * j) * ls + r] = x[j];
   return -1;
 }
 
@@ -343,8 +345,10 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
 // T is the compute type (the LDS copy), ST the storage type of A and W
 // (float for the mixed-precision factorisation, which computes the diagonal
 // blocks in double).
+// One workgroup per launch: occupancy does not matter, so the compiler may use
+// every register (two waves per SIMD) rather than spill.
 template <typename T, typename ST, int NMAX>
-__global__ void __launch_bounds__(BLK_THREADS)
+__global__ void __launch_bounds__(BLK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
 diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info, long info_off) {
   if (*info != 0) return;
   constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL, NW = BLK_THREADS / 64;

@@ -11,6 +11,8 @@
 //   no workgroup barrier.
 #pragma once
 
+#include <type_traits>
+
 #include "internal.h"
 
 namespace hc {
@@ -147,21 +149,42 @@ __device__ __forceinline__ void leaf_inverse(const T a[4], T x[4]) {
 }
 
 // ---------------------------------------------------------------------------
-// Row layout: lane r + 16 q holds row r of the leaf in a[0..15] (the four lane
-// groups q hold the same row; entries above the diagonal are not used).  A
-// pivot's own row element is then in the lane's registers, and everything
-// else a pivot needs comes from v_readlane at a compile-time lane, with no LDS
-// round trip: one wave factors and inverts a 16 x 16 leaf in about a fifth of
-// the time of the shuffle-based column layout above.
+// Row layout: lane r + 16 q holds row r of the leaf in a[0..15] (the four
+// 16-lane rows of the wave, q = 0..3, hold the same matrix row; entries above
+// the diagonal are not used).  A pivot's own row element is then in the lane's
+// registers, and the other elements it needs come from DPP row_newbcast (lane
+// m of each 16-lane row to the whole row: a VGPR move, no LDS round trip and
+// no scalar register).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double readlane(double v, int lane) {
+// f(integral_constant<int, I>) for I = B .. E - 1: loop indices that are
+// compile-time constants without relying on the unroller (which declines
+// large bodies, leaving register arrays indexed at run time, i.e. in scratch).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// v from lane M of this lane's 16-lane row.
+template <int M>
+__device__ __forceinline__ int row_bcast_b32(int v) {
+  // every lane reads a valid source lane, so no "old" value is needed
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + M, 0xf, 0xf, true);
+}
+template <int M>
+__device__ __forceinline__ double row_bcast(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  const unsigned lo = (unsigned)row_bcast_b32<M>((int)(unsigned)b);
+  const unsigned hi = (unsigned)row_bcast_b32<M>((int)(unsigned)(b >> 32));
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
-__device__ __forceinline__ zd readlane(zd v, int lane) { return make_zd(readlane(v.x, lane), readlane(v.y, lane)); }
+template <int M>
+__device__ __forceinline__ zd row_bcast(zd v) {
+  return make_zd(row_bcast<M>(v.x), row_bcast<M>(v.y));
+}
 
 // 1 / sqrt(d) for a normal d > 0: the hardware estimate (v_rsq_f64) refined by
 // two Newton-Raphson steps y <- y + y (1 - d y^2) / 2, each of which squares
@@ -179,55 +202,69 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 
 // Factor the leaf (row layout) in place: on return a[c], c <= r, is L(r, c).
 // Returns the first non-positive / NaN pivot (0-based, the same in every
-// lane) or -1; *logsum accumulates log(d_k) of the pivots k < kmax.
+// lane) or -1; *logsum accumulates log(d_k) of the pivots k < kmax.  The
+// rank-1 update uses A(r, k) / d and conj(A(c, k)) unscaled, so the pivot's
+// square root is off the update chain.  There is no branch on the pivot: a
+// failed pivot is recorded and replaced by 1, and the caller discards the
+// leaf (a data-dependent exit per pivot costs nested exec-mask regions).
 template <typename T>
 __device__ __forceinline__ int leaf_factor_rows(T a[LEAF], int kmax, double *logsum) {
-  const int r = threadIdx.x & 15;
-#pragma unroll
-  for (int k = 0; k < LEAF; k++) {
-    const double d = realx(readlane(a[k], k));  // A(k, k) after k rank-1 updates
-    if (!(d > 0.0)) return k;                   // uniform
+  int r = threadIdx.x & 15;
+  asm volatile("" : "+v"(r));
+  int bad = -1;
+  static_for<0, LEAF>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    const double d0 = realx(row_bcast<k>(a[k]));  // A(k, k) after k rank-1 updates
+    const bool ok = d0 > 0.0;                     // false for NaN
+    bad = (bad < 0 && !ok) ? k : bad;
+    const double d = ok ? d0 : 1.0;
     if (k < kmax) *logsum += log(d);
     const double rs = rsqrt_nr(d);
-    const T lr = rs * a[k];                     // L(r, k) for r > k
-#pragma unroll
-    for (int c = k + 1; c < LEAF; c++) {
-      // A(r, c) -= L(r, k) conj(L(c, k)); rows r < c only touch unused entries
-      const T lc = conjx(rs * readlane(a[k], c));
-      a[c] = a[c] - lr * lc;
-    }
-    a[k] = sel(r == k, from_real<T>(d * rs), sel(r > k, lr, a[k]));
-  }
-  return -1;
+    const T ld = (rs * rs) * a[k];                // A(r, k) / d
+    static_for<k + 1, LEAF>([&](auto C) {         // rows r < c only touch unused entries
+      constexpr int c = decltype(C)::value;
+      a[c] = a[c] - ld * conjx(row_bcast<c>(a[k]));
+    });
+    a[k] = sel(r == k, from_real<T>(d * rs), sel(r > k, rs * a[k], a[k]));
+  });
+  return bad;
 }
 
-// x = inv(L) (row layout: x[c], c <= r, is X(r, c); zeros above) of a leaf
-// factored by leaf_factor_rows, by forward substitution over rows:
-// X(k, :) = (e_k - sum_{j < k} L(k, j) X(j, :)) / L(k, k).
+// Columns q, q + 4, q + 8, q + 12 of X = inv(L) for a leaf factored by
+// leaf_factor_rows (lane r + 16 q: x[j] = X(r, q + 4 j), zeros above the
+// diagonal; the wave's four rows share the sixteen columns).  Forward
+// substitution X(k, c) = (delta_kc - sum_{j < k} L(k, j) X(j, c)) / L(k, k):
+// step k broadcasts row k of X inside each 16-lane row and every lane adds
+// its L(r, k) X(k, c); with the strictly lower part of L the sums of rows
+// already finished stop growing, so X is formed once after the loop.
 template <typename T>
-__device__ __forceinline__ void leaf_inverse_rows(const T a[LEAF], T x[LEAF]) {
-  const int r = threadIdx.x & 15;
+__device__ __forceinline__ void leaf_inverse_cols(const T a[LEAF], T x[4]) {
+  int r = threadIdx.x & 15, q = (threadIdx.x >> 4) & 3;
+  // opaque here, so that what is derived from the lane index is computed
+  // here rather than hoisted out of the caller's loop and kept live (and
+  // spilled) through the factorisation
+  asm volatile("" : "+v"(r), "+v"(q));
   T dg = a[0];
 #pragma unroll
   for (int c = 1; c < LEAF; c++) dg = sel(r == c, a[c], dg);
   const T dinv = from_real<T>(1.0) / dg;  // every lane its own row's pivot, in parallel
-  T acc[LEAF];
+  T e[4], acc[4];
 #pragma unroll
-  for (int c = 0; c < LEAF; c++) {
-    acc[c] = from_real<T>(0.0);
-    x[c] = from_real<T>(0.0);
+  for (int j = 0; j < 4; j++) {
+    e[j] = sel(r == q + 4 * j, from_real<T>(1.0), from_real<T>(0.0));
+    acc[j] = from_real<T>(0.0);
   }
+  static_for<0, LEAF>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    const T lk = sel(k < r, a[k], from_real<T>(0.0));  // strictly lower part of L
 #pragma unroll
-  for (int k = 0; k < LEAF; k++) {
-    // row k of X, finished by lane k (other lanes compute and discard it)
-#pragma unroll
-    for (int c = 0; c <= k; c++) {
-      const T t = (sel(r == c, from_real<T>(1.0), from_real<T>(0.0)) - acc[c]) * dinv;
-      x[c] = sel(r == k, t, x[c]);
-      const T xk = readlane(t, k);
-      acc[c] = acc[c] + a[k] * xk;  // rows r > k: sum_{j <= k} L(r, j) X(j, c)
+    for (int j = 0; j < 4; j++) {
+      const T xk = row_bcast<k>((e[j] - acc[j]) * dinv);  // X(k, q + 4 j), final in lane k
+      acc[j] = acc[j] + lk * xk;
     }
-  }
+  });
+#pragma unroll
+  for (int j = 0; j < 4; j++) x[j] = (e[j] - acc[j]) * dinv;
 }
 
 }  // namespace t16
