@@ -363,39 +363,54 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
     for (int r = max(n, c); r < npad; r++) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
   // lower form, read along memory's contiguous index (i), the loads of a
   // thread in flight together
-  gather_block<NMAX, BLK_THREADS, 16>(n, A, lda, [&](int i, int k, ST x) {
+  // every load of a thread in flight at once (NMAX^2 / BLK_THREADS of them)
+  gather_block<NMAX, BLK_THREADS, NMAX * NMAX / BLK_THREADS>(n, A, lda, [&](int i, int k, ST x) {
     const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
     if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
   });
   if (tid == 0) flag = -1;
   __syncthreads();
-  for (int p = 0; p < np; p++) {
-    if (wave == 0) {
-      const int bad = leaf_factor_inverse<T, ST>(tile(p, p), LL, lower, n, p * LEAF, A, lda);
-      if (bad >= 0 && (tid & 63) == 0) flag = p * LEAF + bad;
+  // One leaf of look-ahead: while wave 0 updates tile (p+1, p+1) and factors
+  // leaf p + 1, waves 1 .. NW-1 apply the rest of step p's trailing update.
+  // Step p = -1 only factors leaf 0 (one call site keeps the leaf inlined once).
+  for (int p = -1; p < np; p++) {
+    if (p >= 0) {
+      for (int I = p + 1 + wave; I < np; I += NW) {  // L_Ip = A_Ip inv(L_pp)^H
+        TileAcc<T> acc;
+        acc_zero(acc);
+        tile_mma<true>(acc, tile(I, p), LL, tile(p, p), LL, 1.0);
+        acc_store(acc, tile(I, p), LL);
+      }
+      __syncthreads();
+    }
+    if (p + 1 < np) {
+      if (wave == 0) {  // A_{p+1,p+1} -= L_{p+1,p} L_{p+1,p}^H, then leaf p + 1
+        if (p >= 0) {
+          TileAcc<T> acc;
+          acc_load(acc, tile(p + 1, p + 1), LL);
+          tile_mma<true>(acc, tile(p + 1, p), LL, tile(p + 1, p), LL, -1.0);
+          acc_store(acc, tile(p + 1, p + 1), LL);
+          wave_lds_fence();
+        }
+        const int bad = leaf_factor_inverse<T, ST>(tile(p + 1, p + 1), LL, lower, n, (p + 1) * LEAF, A, lda);
+        if (bad >= 0 && (tid & 63) == 0) flag = (p + 1) * LEAF + bad;
+      } else if (p >= 0) {
+        int t = 0;
+        for (int J = p + 1; J < np; J++)
+          for (int I = (J == p + 1) ? J + 1 : J; I < np; I++, t++)
+            if (t % (NW - 1) == wave - 1) {  // A_IJ -= L_Ip L_Jp^H
+              TileAcc<T> acc;
+              acc_load(acc, tile(I, J), LL);
+              tile_mma<true>(acc, tile(I, p), LL, tile(J, p), LL, -1.0);
+              acc_store(acc, tile(I, J), LL);
+            }
+      }
     }
     __syncthreads();
-    if (flag >= 0) {  // padding never fails, so flag < n
+    if (flag >= 0) {  // leaf p + 1 failed (padding never fails, so flag < n)
       if (tid == 0) *info = (int)(info_off + flag + 1);
       return;
     }
-    for (int I = p + 1 + wave; I < np; I += NW) {  // L_Ip = A_Ip inv(L_pp)^H
-      TileAcc<T> acc;
-      acc_zero(acc);
-      tile_mma<true>(acc, tile(I, p), LL, tile(p, p), LL, 1.0);
-      acc_store(acc, tile(I, p), LL);
-    }
-    __syncthreads();
-    int t = 0;
-    for (int J = p + 1; J < np; J++)
-      for (int I = J; I < np; I++, t++)
-        if (t % NW == wave) {  // A_IJ -= L_Ip L_Jp^H
-          TileAcc<T> acc;
-          acc_load(acc, tile(I, J), LL);
-          tile_mma<true>(acc, tile(I, p), LL, tile(J, p), LL, -1.0);
-          acc_store(acc, tile(I, J), LL);
-        }
-    __syncthreads();
   }
   // off-diagonal tiles of the factor (the diagonal ones are already out)
   for_block<NMAX, BLK_THREADS>(n, [&](int i, int k) {
