@@ -345,12 +345,21 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
 // T is the compute type (the LDS copy), ST the storage type of A and W
 // (float for the mixed-precision factorisation, which computes the diagonal
 // blocks in double).
-// One workgroup per launch: occupancy does not matter, so the compiler may use
-// every register (two waves per SIMD) rather than spill.
+// One workgroup per launch, run beside the trailing update's workgroups
+// (256 registers per wave, up to two per SIMD): with at most 128 registers per
+// wave (WPE = 4 waves per SIMD) the real-type kernel fits next to one of them
+// and starts at once; the complex one needs more registers and is left free.
+template <typename T>
+constexpr int diag_waves_per_eu() { return is_complex<T>::value ? 2 : 4; }
+
 template <typename T, typename ST, int NMAX>
-__global__ void __launch_bounds__(BLK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(BLK_THREADS)
+__attribute__((amdgpu_waves_per_eu(diag_waves_per_eu<T>(), diag_waves_per_eu<T>())))
 diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info, long info_off) {
   if (*info != 0) return;
+  // This block is on the critical path of the factorisation while trailing-
+  // update waves share its SIMDs: take instruction issue ahead of them.
+  __builtin_amdgcn_s_setprio(3);
   constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL, NW = BLK_THREADS / 64;
   __shared__ T S[NPMAX * (NPMAX + 1) / 2 * TS];
   __shared__ int flag;
