@@ -6,7 +6,7 @@
 // ... (G = grid size; `order` lists the matrices by decreasing n, so the
 // round-robin deal is a longest-first schedule).  Each matrix is factored in
 // place, right-looking over 16-column leaves:
-//   * wave 0 factors leaf p in registers (cross-lane shuffles), writes L_pp,
+//   * wave 0 factors leaf p in registers (DPP row broadcasts), writes L_pp,
 //     accumulates log of the pivots and inverts L_pp into LDS;
 //   * all waves form the panel L_Ip = A_Ip inv(L_pp)^T on the matrix cores
 //     (written to the matrix and kept in LDS);
@@ -45,10 +45,13 @@ struct View {
   ST *M;
   int n;
   __device__ long off(int r, int c) const { return LOWER ? (long)c * n + r : (long)r * n + c; }
-  // value of lower-form (r, c): zero above the diagonal, identity beyond n
+  // value of lower-form (r, c): zero above the diagonal, identity beyond n.
+  // The load is unconditional, from a clamped (always valid) address, and the
+  // value is selected afterwards: no branch per element, so loads stay in
+  // flight together and register arrays filled from them stay in registers.
   __device__ double get(int r, int c) const {
-    if (r < n && c < n) return r >= c ? (double)M[off(r, c)] : 0.0;
-    return r == c ? 1.0 : 0.0;
+    const double x = (double)M[off(min(r, n - 1), min(c, n - 1))];
+    return (r < n && c < n) ? (r >= c ? x : 0.0) : (r == c ? 1.0 : 0.0);
   }
   __device__ void put(int r, int c, double v) const {
     if (r < n && c < n && r >= c) M[off(r, c)] = (ST)v;
@@ -69,21 +72,31 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
     const int n = nvec[idx];
     const View<ST, LOWER> v{A + offs[idx], n};
     const int np = (n + LEAF - 1) / LEAF;
-    double logsum = 0.0;
+    double logpart = 0.0;  // wave 0, lanes 0..15: sum of log(L_rr^2) of row r of every leaf
     int bad = -1;
     for (int p = 0; p < np; p++) {
       const int g0 = p * LEAF;
       if (wave == 0) {
         const int r = li, q = lk;
-        double a[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) a[j] = v.get(g0 + r, g0 + q + 4 * j);
-        const int b = t16::leaf_factor<double>(a, n - g0, &logsum);
+        double a[LEAF];  // row r of the leaf (row layout, t16::leaf_factor_rows)
+        t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = v.get(g0 + r, g0 + decltype(C)::value); });
+        double unused = 0.0;
+        const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
         if (b < 0) {
+          // log of this lane's own pivot L_rr^2, all rows at once (off the
+          // pivot chain); summed over the 16 rows once per matrix
+          double dg = a[0];
+          t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(r == decltype(C)::value, a[decltype(C)::value], dg); });
+          if (q == 0 && g0 + r < n) logpart += 2.0 * log(dg);
 #pragma unroll
-          for (int j = 0; j < 4; j++) v.put(g0 + r, g0 + q + 4 * j, a[j]);
+          for (int j = 0; j < 4; j++) {  // lane group q writes columns q + 4 j of its row
+            // a value select (a conditional over the array elements themselves
+            // would select their addresses and keep the array in memory)
+            const double f = t16::sel(q == 0, a[4 * j], t16::sel(q == 1, a[4 * j + 1], t16::sel(q == 2, a[4 * j + 2], a[4 * j + 3])));
+            v.put(g0 + r, g0 + q + 4 * j, f);
+          }
           double x[4];
-          t16::leaf_inverse<double>(a, x);
+          t16::leaf_inverse_cols<double>(a, x);
 #pragma unroll
           for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
         }
@@ -157,10 +170,9 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
       for (int p = np - 1; p >= 0; p--) {
         const int g0 = p * LEAF;
         if (wave == 0) {
-          double a[4], x[4];
-#pragma unroll
-          for (int j = 0; j < 4; j++) a[j] = v.get(g0 + li, g0 + lk + 4 * j);
-          t16::leaf_inverse<double>(a, x);
+          double a[LEAF], x[4];
+          t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = v.get(g0 + li, g0 + decltype(C)::value); });
+          t16::leaf_inverse_cols<double>(a, x);
 #pragma unroll
           for (int j = 0; j < 4; j++) Li[(lk + 4 * j) * LL + li] = x[j];
         }
@@ -227,9 +239,14 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
         __syncthreads();
       }
     }
-    if (tid == 0) {
-      info[idx] = bad >= 0 ? bad + 1 : 0;
-      if (logdet != nullptr) logdet[idx] = bad >= 0 ? __builtin_nan("") : logsum;
+    if (wave == 0) {
+      double logsum = logpart;  // lanes 16..63 hold 0
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) logsum += __shfl_xor(logsum, off, 64);
+      if (lane == 0) {
+        info[idx] = bad >= 0 ? bad + 1 : 0;
+        if (logdet != nullptr) logdet[idx] = bad >= 0 ? __builtin_nan("") : logsum;
+      }
     }
     __syncthreads();  // flag and LDS are reused by the next matrix
   }
