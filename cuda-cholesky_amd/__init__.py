@@ -48,6 +48,16 @@ def zscalar(v):
     return _Z(v.real, v.imag)
 
 
+class _C(ctypes.Structure):
+    """C99 `float complex` passed by value."""
+    _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
+
+
+def cscalar(v):
+    v = complex(v)
+    return _C(v.real, v.imag)
+
+
 _sz = ctypes.c_size_t
 _ptr = ctypes.c_uint64     # CUdeviceptr: device byte address
 _enum = ctypes.c_int       # enums are int-sized
@@ -100,6 +110,26 @@ SIGNATURES = {
     "cuStrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_float, _ptr, _sz, _ptr, _sz,
                 _stream],
     "cuSpotrf": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuStrmm2": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_float, _ptr, _sz, _ptr, _sz,
+                 _ptr, _sz, _stream],
+    "cuCherk": [_h, _enum, _enum, _sz, _sz, ctypes.c_float, _ptr, _sz, ctypes.c_float, _ptr, _sz,
+                _stream],
+    "cuCgemm2": [_h, _enum, _enum, _sz, _sz, _sz, _C, _ptr, _sz, _ptr, _sz, _C, _ptr, _sz, _ptr,
+                 _sz, _stream],
+    "cuCtrmm2": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _C, _ptr, _sz, _ptr, _sz, _ptr, _sz,
+                 _stream],
+    "cuCtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _C, _ptr, _sz, _ptr, _sz, _stream],
+    "cuCpotrf": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuStrtri": [_h, _enum, _enum, _sz, _ptr, _sz, _longp],
+    "cuCtrtri": [_h, _enum, _enum, _sz, _ptr, _sz, _longp],
+    "cuStrti22": [_h, _enum, _enum, _sz, _ptr, _sz, _ptr, _sz, _ptr, _stream],
+    "cuCtrti22": [_h, _enum, _enum, _sz, _ptr, _sz, _ptr, _sz, _ptr, _stream],
+    "cuSlauum": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuClauum": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuSpotri": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuCpotri": [_h, _enum, _sz, _ptr, _sz, _longp],
+    "cuSlogdet": [_h, _ptr, _sz, _sz, ctypes.POINTER(ctypes.c_float), _stream],
+    "cuClogdet": [_h, _ptr, _sz, _sz, ctypes.POINTER(ctypes.c_float), _stream],
     # include/cucholesky.h (extensions)
     "cuDpotrs": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _longp],
     "cuDsposv": [_h, _enum, _sz, _sz, _ptr, _sz, _ptr, _sz, _ptr, _sz, _longp, _longp],
@@ -459,6 +489,81 @@ def cuStrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream
 def cuZtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
     return lib().cuZtrsm(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), A, lda, B, ldb,
                          stream)
+
+
+# ---- single precision and single complex (widened onto the double paths) ----
+
+def cuStrmm2(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx, stream=None):
+    return lib().cuStrmm2(_hv(handle), side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx,
+                          stream)
+
+
+def cuCherk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream=None):
+    return lib().cuCherk(_hv(handle), uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream)
+
+
+def cuCgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, D, ldd,
+             stream=None):
+    return lib().cuCgemm2(_hv(handle), transA, transB, m, n, k, cscalar(alpha), A, lda, B, ldb,
+                          cscalar(beta), C, ldc, D, ldd, stream)
+
+
+def cuCtrmm2(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx, stream=None):
+    return lib().cuCtrmm2(_hv(handle), side, uplo, trans, diag, m, n, cscalar(alpha), A, lda, B,
+                          ldb, X, ldx, stream)
+
+
+def cuCtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, stream=None):
+    return lib().cuCtrsm(_hv(handle), side, uplo, trans, diag, m, n, cscalar(alpha), A, lda, B, ldb,
+                         stream)
+
+
+def cuCpotrf(handle, uplo, n, A, lda):
+    return _info_call(lib().cuCpotrf, _hv(handle), uplo, n, A, lda)
+
+
+def cuStrtri(handle, uplo, diag, n, A, lda):
+    return _info_call(lib().cuStrtri, _hv(handle), uplo, diag, n, A, lda)
+
+
+def cuCtrtri(handle, uplo, diag, n, A, lda):
+    return _info_call(lib().cuCtrtri, _hv(handle), uplo, diag, n, A, lda)
+
+
+def cuStrti22(handle, uplo, diag, n, A, lda, B, ldb, info_ptr, stream=None):
+    return lib().cuStrti22(_hv(handle), uplo, diag, n, A, lda, B, ldb, info_ptr, stream)
+
+
+def cuCtrti22(handle, uplo, diag, n, A, lda, B, ldb, info_ptr, stream=None):
+    return lib().cuCtrti22(_hv(handle), uplo, diag, n, A, lda, B, ldb, info_ptr, stream)
+
+
+def cuSlauum(handle, uplo, n, A, lda):
+    return _info_call(lib().cuSlauum, _hv(handle), uplo, n, A, lda)
+
+
+def cuClauum(handle, uplo, n, A, lda):
+    return _info_call(lib().cuClauum, _hv(handle), uplo, n, A, lda)
+
+
+def cuSpotri(handle, uplo, n, A, lda):
+    return _info_call(lib().cuSpotri, _hv(handle), uplo, n, A, lda)
+
+
+def cuCpotri(handle, uplo, n, A, lda):
+    return _info_call(lib().cuCpotri, _hv(handle), uplo, n, A, lda)
+
+
+def cuSlogdet(handle, x, incx, n, stream=None):
+    r = ctypes.c_float(0.0)
+    st = lib().cuSlogdet(_hv(handle), x, incx, n, ctypes.byref(r), stream)
+    return st, r.value
+
+
+def cuClogdet(handle, x, incx, n, stream=None):
+    r = ctypes.c_float(0.0)
+    st = lib().cuClogdet(_hv(handle), x, incx, n, ctypes.byref(r), stream)
+    return st, r.value
 
 
 # ---- host-pointer API (include/cumultigpu.h; cuMultiGPU* of blas.h / lapack.h) ----

@@ -677,6 +677,9 @@ hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials
 HC_INST(double)
 HC_INST(zd)
 #undef HC_INST
+// log-determinants of single / single-complex data (logs taken in FP64)
+template hipError_t logdet<float>(hipStream_t, const float *, long, long, double *, double *);
+template hipError_t logdet<zf>(hipStream_t, const zf *, long, long, double *, double *);
 // FP32 storage, FP64 computation (mixed-precision factorisation and solves)
 template hipError_t diag_potrf<float>(hipStream_t, char, int, float *, long, float *, long, int *,
                                       long);

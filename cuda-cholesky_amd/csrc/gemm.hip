@@ -495,7 +495,37 @@ __global__ void copy2d_kernel(int m, int n, const T *S, long lds, T *D, long ldd
   if (i < m && j < n) D[(long)j * ldd + i] = S[(long)j * lds + i];
 }
 
+__device__ __forceinline__ double cvt(float v, double *) { return (double)v; }
+__device__ __forceinline__ float cvt(double v, float *) { return (float)v; }
+__device__ __forceinline__ zd cvt(zf v, zd *) { return make_zd((double)v.x, (double)v.y); }
+__device__ __forceinline__ zf cvt(zd v, zf *) {
+  zf r;
+  r.x = (float)v.x;
+  r.y = (float)v.y;
+  return r;
+}
+
+template <typename S, typename D>
+__global__ void convert2d_kernel(int m, long n, const S *src, long lds, D *dst, long ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (long j = blockIdx.y; j < n; j += gridDim.y)
+    dst[j * ldd + i] = cvt(src[j * lds + i], static_cast<D *>(nullptr));
+}
+
 }  // namespace
+
+template <typename S, typename D>
+hipError_t convert2d(hipStream_t s, long m, long n, const S *src, long lds, D *dst, long ldd) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((m + 255) / 256), (unsigned)(n < 8192 ? n : 8192));
+  hipLaunchKernelGGL((convert2d_kernel<S, D>), grid, dim3(256), 0, s, (int)m, n, src, lds, dst, ldd);
+  return hipGetLastError();
+}
+template hipError_t convert2d<float, double>(hipStream_t, long, long, const float *, long, double *, long);
+template hipError_t convert2d<double, float>(hipStream_t, long, long, const double *, long, float *, long);
+template hipError_t convert2d<zf, zd>(hipStream_t, long, long, const zf *, long, zd *, long);
+template hipError_t convert2d<zd, zf>(hipStream_t, long, long, const zd *, long, zf *, long);
 
 template <typename T>
 hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,

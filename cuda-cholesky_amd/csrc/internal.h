@@ -52,6 +52,13 @@ __host__ __device__ inline zd operator/(zd a, zd b) {
   const double r = b.x / b.y, d = b.y + b.x * r;
   return make_zd((a.x * r + a.y) / d, (a.y * r - a.x) / d);
 }
+// Single complex, layout-compatible with C99 `float complex`: storage only
+// (the single-complex entry points compute in double complex).
+struct __attribute__((aligned(8))) zf {
+  float x, y;
+};
+__host__ __device__ inline double realx(zf a) { return a.x; }
+
 __host__ __device__ inline float conjx(float a) { return a; }
 __host__ __device__ inline double realx(float a) { return a; }
 __host__ __device__ inline bool is_zero(float a) { return a == 0.0f; }
@@ -115,6 +122,12 @@ hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, fl
 // 2-D copy D <- S (m x n)
 template <typename T>
 hipError_t copy2d(hipStream_t s, long m, long n, const T *S, long lds, T *D, long ldd);
+
+// D = S element-wise with a change of precision, m x n column-major: float <->
+// double and single <-> double complex (the single and single-complex entry
+// points run the double / double-complex paths on a widened copy).
+template <typename S, typename D>
+hipError_t convert2d(hipStream_t s, long m, long n, const S *src, long lds, D *dst, long ldd);
 
 // ---------------------------------------------------------------------------
 // level-3 building blocks (blas_ops.cpp)

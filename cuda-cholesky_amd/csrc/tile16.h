@@ -4,11 +4,11 @@
 // * TileAcc / tile_mma: a 16 x 16 accumulator in the v_mfma_f64_16x16x4_f64
 //   result layout (lane l, register r holds element (l / 16 + 4 r, l % 16))
 //   and C += s X op(Y) over a 16-deep k range, complex as four real MFMAs.
-// * leaf_factor / leaf_inverse: one wave factors a 16 x 16 Hermitian
-//   positive definite tile held in registers (lane r + 16 q holds row r,
-//   columns q + 4 j, j = 0..3) and inverts the factor, moving pivots and
-//   column values with cross-lane shuffles, so the 16 sequential pivots need
-//   no workgroup barrier.
+// * leaf_factor_rows / leaf_inverse_cols: one wave factors a 16 x 16
+//   Hermitian positive definite tile held in registers (lane r + 16 q holds
+//   row r) and inverts the factor, moving values between lanes with DPP row
+//   broadcasts, so the 16 sequential pivots need no workgroup barrier and no
+//   LDS round trip.
 #pragma once
 
 #include <type_traits>
@@ -86,67 +86,8 @@ __device__ __forceinline__ void tile_mma(TileAcc<zd> &c, const zd *X, int ldx, c
   }
 }
 
-__device__ __forceinline__ double lane_shfl(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ zd lane_shfl(zd v, int src) {
-  return make_zd(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
-
 // Orders this wave's LDS writes before its following LDS reads by other lanes.
 __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
-
-// Factor the leaf in registers in place: on return the lower part of `a` is
-// L.  Returns the first non-positive / NaN pivot (0-based, the same value in
-// every lane) or -1.  *logsum accumulates log(d_k) of the pivots k < kmax
-// (d_k = L_kk^2), in pivot order.
-template <typename T>
-__device__ __forceinline__ int leaf_factor(T a[4], int kmax, double *logsum) {
-  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-#pragma unroll
-  for (int k = 0; k < LEAF; k++) {
-    const int jk = k >> 2, qk = k & 3;  // column k: lanes (*, qk), register jk
-    const T ak = a[jk];                 // column k before this step's update
-    const double d = realx(lane_shfl(ak, k + 16 * qk));
-    if (!(d > 0.0)) return k;           // uniform: every lane holds the same d
-    if (k < kmax) *logsum += log(d);
-    const double sd = sqrt(d), rs = 1.0 / sd;
-    const T lrk = rs * lane_shfl(ak, r + 16 * qk);
-    const T fin = sel(r == k, from_real<T>(sd), lrk);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int c = q + 4 * j;
-      const T lck = conjx(rs * lane_shfl(ak, c + 16 * qk));
-      const T upd = a[j] - lrk * lck;
-      a[j] = sel(c > k, upd, sel(c == k, fin, a[j]));
-    }
-  }
-  return -1;
-}
-
-// x = inv(L) of the factored leaf (lower, exact zeros above), by forward
-// substitution over rows.
-template <typename T>
-__device__ __forceinline__ void leaf_inverse(const T a[4], T x[4]) {
-  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-#pragma unroll
-  for (int j = 0; j < 4; j++) x[j] = from_real<T>(0.0);
-#pragma unroll
-  for (int k = 0; k < LEAF; k++) {
-    const int jk = k >> 2, qk = k & 3;
-    const T dk = from_real<T>(1.0) / lane_shfl(a[jk], k + 16 * qk);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int c = q + 4 * j;
-      const T fin = (sel(c == k, from_real<T>(1.0), from_real<T>(0.0)) - x[j]) * dk;
-      x[j] = sel(r == k, fin, x[j]);
-    }
-    const T lrk = lane_shfl(a[jk], r + 16 * qk);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const T xkc = lane_shfl(x[j], k + 16 * q);
-      x[j] = sel(r > k, x[j] + lrk * xkc, x[j]);
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Row layout: lane r + 16 q holds row r of the leaf in a[0..15] (the four

@@ -22,10 +22,12 @@
 
 #ifdef __cplusplus
 typedef _Complex double cuDoubleComplex_t;
+typedef _Complex float cuComplex_t;
 extern "C" {
 #else
 #include <complex.h>
 typedef double complex cuDoubleComplex_t;
+typedef float complex cuComplex_t;
 #endif
 
 /** BLAS enums, castable to char (reference include/blas.h:24-27). */
@@ -121,6 +123,43 @@ CUresult cuSgemm2(CUBLAShandle, CBlasTranspose, CBlasTranspose,
 CUresult cuStrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
                  size_t, size_t,
                  float, CUdeviceptr, size_t,
+                 CUdeviceptr, size_t, CUstream);
+
+/*
+ * Single precision triangular multiply and the single complex BLAS: the
+ * operands are widened on the stream into stream-ordered double / double
+ * complex copies, the double / double complex routine above runs on them,
+ * and the result is narrowed back (at least single-precision accuracy; the
+ * unreferenced triangle and padding are not written).  Same argument order,
+ * xerbla indices and quick returns as the double twins.
+ */
+/* (replaces cuStrmm2, reference blas/strmm.c:374) */
+CUresult cuStrmm2(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                  size_t, size_t,
+                  float, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  CUdeviceptr, size_t, CUstream);
+/* C <- alpha op(A) op(A)^H + beta C, trans N or C (replaces cuCherk,
+ * reference blas/cherk.c:190) */
+CUresult cuCherk(CUBLAShandle, CBlasUplo, CBlasTranspose,
+                 size_t, size_t,
+                 float, CUdeviceptr, size_t,
+                 float, CUdeviceptr, size_t, CUstream);
+#define cuCgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream) \
+        cuCgemm2(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc, stream)
+/* (replaces cuCgemm2, reference blas/cgemm.c:226) */
+CUresult cuCgemm2(CUBLAShandle, CBlasTranspose, CBlasTranspose,
+                  size_t, size_t, size_t,
+                  cuComplex_t, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  cuComplex_t, CUdeviceptr, size_t, CUdeviceptr, size_t, CUstream);
+/* (replaces cuCtrmm2, reference blas/ctrmm.c:422) */
+CUresult cuCtrmm2(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                  size_t, size_t,
+                  cuComplex_t, CUdeviceptr, size_t, CUdeviceptr, size_t,
+                  CUdeviceptr, size_t, CUstream);
+/* (replaces cuCtrsm, reference blas/ctrsm.c:248) */
+CUresult cuCtrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
+                 size_t, size_t,
+                 cuComplex_t, CUdeviceptr, size_t,
                  CUdeviceptr, size_t, CUstream);
 
 /*

@@ -79,6 +79,36 @@ CUresult cuDlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, double *, CUstre
 CUresult cuZlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, double *, CUstream);
 
 /*
+ * Single precision and single complex: the matrix is widened on the handle's
+ * stream into a stream-ordered double / double complex copy, the routine
+ * above runs on it, and the result is narrowed back (the whole n x n block
+ * round-trips, exactly for the unreferenced triangle).  Same argument order,
+ * info conventions and xerbla indices as the double twins.  cuSpotrf is the
+ * native FP32 path declared above.
+ */
+/* (replace cuCpotrf, reference lapack/cpotrf.c:174) */
+CUresult cuCpotrf(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* (replace cuStrtri / cuCtrtri, reference lapack/strtri.c:369, lapack/ctrtri.c:326) */
+CUresult cuStrtri(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, long *);
+CUresult cuCtrtri(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, long *);
+/* (replace cuStrti22 / cuCtrti22, reference lapack/strtri.c:301, lapack/ctrtri.c:301) */
+#define cuStrti2(handle, uplo, diag, n, A, lda, info, stream) cuStrti22(handle, uplo, diag, n, A, lda, A, lda, info, stream)
+#define cuCtrti2(handle, uplo, diag, n, A, lda, info, stream) cuCtrti22(handle, uplo, diag, n, A, lda, A, lda, info, stream)
+CUresult cuStrti22(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, CUdeviceptr, size_t, CUdeviceptr, CUstream);
+CUresult cuCtrti22(CULAPACKhandle, CBlasUplo, CBlasDiag, size_t, CUdeviceptr, size_t, CUdeviceptr, size_t, CUdeviceptr, CUstream);
+/* (replace cuSlauum / cuClauum, reference lapack/slauum.c:197, lapack/clauum.c:154) */
+CUresult cuSlauum(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+CUresult cuClauum(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* (replace cuSpotri / cuCpotri, reference lapack/spotri.c:25, lapack/cpotri.c:25) */
+CUresult cuSpotri(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+CUresult cuCpotri(CULAPACKhandle, CBlasUplo, size_t, CUdeviceptr, size_t, long *);
+/* 2 * sum(log(Re x[i * incx])) of single / single complex data, logs and sum
+ * in double precision, rounded to float (replace cuSlogdet / cuClogdet,
+ * reference lapack/slogdet.c:37, lapack/clogdet.c:38). */
+CUresult cuSlogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, float *, CUstream);
+CUresult cuClogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, float *, CUstream);
+
+/*
  * Host-pointer LAPACK of the reference's multi-GPU layer
  * (include/lapack.h:117-158), collapsed onto the one device of the
  * CUmultiGPU context: the matrix (column-major host array, in place) is

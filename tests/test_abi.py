@@ -278,3 +278,29 @@ def test_multigpu_quick_returns(cc):
     assert cc.cuMultiGPUDtrsm(FAKE, cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, 0, 4, 1.0, 0, 1, 0, 1) == 0
     assert cc.cuMultiGPUDpotrf(FAKE, cc.Lower, 0, 0, 1) == (0, 0)
     assert cc.cuMultiGPUZtrtri(FAKE, cc.Lower, cc.Unit, 0, 0, 1) == (0, 0)
+
+
+def test_single_and_single_complex_argument_indices(cc):
+    """The single / single-complex entry points check their arguments like
+    their double twins, before anything is widened."""
+    L, U, N, T, C = cc.Lower, cc.Upper, cc.NoTrans, cc.Trans, cc.ConjTrans
+    with cc.Xerbla() as x:
+        assert cc.cuCgemm2(FAKE, N, N, 5, 4, 3, 1.0, 0, 4, 0, 3, 1.0, 0, 5, 0, 5) == 1
+        assert cc.cuCherk(FAKE, L, T, 5, 3, 1.0, 0, 5, 1.0, 0, 5) == 1
+        assert cc.cuStrmm2(FAKE, cc.Left, U, T, cc.Unit, 6, 4, 1.0, 0, 6, 0, 6, 0, 5) == 1
+        assert cc.cuCtrmm2(FAKE, cc.Right, L, C, cc.NonUnit, 6, 4, 1.0, 0, 3, 0, 6, 0, 6) == 1
+        assert cc.cuCtrsm(FAKE, cc.Right, U, C, cc.Unit, 6, 4, 1.0, 0, 4, 0, 5) == 1
+        assert cc.cuCpotrf(FAKE, L, 10, 0, 9) == (1, -4)
+        assert cc.cuStrtri(FAKE, U, ord("X"), 10, 0, 10) == (1, -3)
+        assert cc.cuCtrtri(FAKE, U, cc.NonUnit, 10, 0, 9) == (1, -5)
+        assert cc.cuClauum(FAKE, L, 10, 0, 9) == (1, -4)
+        assert cc.cuSpotri(FAKE, ord("X"), 10, 0, 10) == (1, -2)
+        assert cc.cuStrti22(FAKE, L, cc.NonUnit, 4, 0, 3, 0, 4, 0) == 1
+    assert x.calls == [("cuCgemm2", 8), ("cuCherk", 3), ("cuStrmm2", 13), ("cuCtrmm2", 9),
+                       ("cuCtrsm", 11), ("cuCpotrf", 4), ("cuStrtri", 3), ("cuCtrtri", 5),
+                       ("cuClauum", 4), ("cuSpotri", 2), ("cuStrti22", 6)]
+    # quick returns need no device
+    assert cc.cuCgemm2(FAKE, N, N, 0, 4, 3, 1.0, 0, 1, 0, 3, 1.0, 0, 1, 0, 1) == 0
+    assert cc.cuCpotrf(FAKE, L, 0, 0, 1) == (0, 0)
+    assert cc.cuSlogdet(FAKE, 0, 1, 0) == (0, 0.0)
+    assert cc.cuClogdet(FAKE, 0, 1, 0) == (0, 0.0)
