@@ -172,6 +172,27 @@ SIGNATURES = {
     "cuMultiGPUDtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_double, _hp, _sz, _hp,
                         _sz],
     "cuMultiGPUZtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _Z, _hp, _sz, _hp, _sz],
+    "cuMultiGPUSsyrk": [_h, _enum, _enum, _sz, _sz, ctypes.c_float, _hp, _sz, ctypes.c_float, _hp,
+                        _sz],
+    "cuMultiGPUCherk": [_h, _enum, _enum, _sz, _sz, ctypes.c_float, _hp, _sz, ctypes.c_float, _hp,
+                        _sz],
+    "cuMultiGPUSgemm": [_h, _enum, _enum, _sz, _sz, _sz, ctypes.c_float, _hp, _sz, _hp, _sz,
+                        ctypes.c_float, _hp, _sz],
+    "cuMultiGPUCgemm": [_h, _enum, _enum, _sz, _sz, _sz, _C, _hp, _sz, _hp, _sz, _C, _hp, _sz],
+    "cuMultiGPUStrmm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_float, _hp, _sz, _hp,
+                        _sz],
+    "cuMultiGPUCtrmm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _C, _hp, _sz, _hp, _sz],
+    "cuMultiGPUStrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, ctypes.c_float, _hp, _sz, _hp,
+                        _sz],
+    "cuMultiGPUCtrsm": [_h, _enum, _enum, _enum, _enum, _sz, _sz, _C, _hp, _sz, _hp, _sz],
+    "cuMultiGPUSpotrf": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUCpotrf": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUStrtri": [_h, _enum, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUCtrtri": [_h, _enum, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUSlauum": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUClauum": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUSpotri": [_h, _enum, _sz, _hp, _sz, _longp],
+    "cuMultiGPUCpotri": [_h, _enum, _sz, _hp, _sz, _longp],
     "cuMultiGPULAPACKCreate": [ctypes.POINTER(_h), _h],
     "cuMultiGPULAPACKDestroy": [_h],
     "cuMultiGPULAPACKSynchronize": [_h],
@@ -707,6 +728,64 @@ def cuMultiGPUDtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb
 def cuMultiGPUZtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
     return lib().cuMultiGPUZtrsm(_hv(handle), side, uplo, trans, diag, m, n, zscalar(alpha), _host(A),
                                  lda, _host(B), ldb)
+
+
+def cuMultiGPUSgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return lib().cuMultiGPUSgemm(_hv(handle), transA, transB, m, n, k, alpha, _host(A), lda, _host(B),
+                                 ldb, beta, _host(C), ldc)
+
+
+def cuMultiGPUCgemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return lib().cuMultiGPUCgemm(_hv(handle), transA, transB, m, n, k, cscalar(alpha), _host(A), lda,
+                                 _host(B), ldb, cscalar(beta), _host(C), ldc)
+
+
+def cuMultiGPUSsyrk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    return lib().cuMultiGPUSsyrk(_hv(handle), uplo, trans, n, k, alpha, _host(A), lda, beta, _host(C),
+                                 ldc)
+
+
+def cuMultiGPUCherk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    return lib().cuMultiGPUCherk(_hv(handle), uplo, trans, n, k, alpha, _host(A), lda, beta, _host(C),
+                                 ldc)
+
+
+def cuMultiGPUStrmm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUStrmm(_hv(handle), side, uplo, trans, diag, m, n, alpha, _host(A), lda,
+                                 _host(B), ldb)
+
+
+def cuMultiGPUCtrmm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUCtrmm(_hv(handle), side, uplo, trans, diag, m, n, cscalar(alpha), _host(A),
+                                 lda, _host(B), ldb)
+
+
+def cuMultiGPUStrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUStrsm(_hv(handle), side, uplo, trans, diag, m, n, alpha, _host(A), lda,
+                                 _host(B), ldb)
+
+
+def cuMultiGPUCtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return lib().cuMultiGPUCtrsm(_hv(handle), side, uplo, trans, diag, m, n, cscalar(alpha), _host(A),
+                                 lda, _host(B), ldb)
+
+
+def _mlapack(name):
+    def call(handle, uplo, *args):
+        *rest, A, lda = args
+        return _info_call(getattr(lib(), name), _hv(handle), uplo, *rest, _host(A), lda)
+    call.__name__ = name
+    return call
+
+
+cuMultiGPUSpotrf = _mlapack("cuMultiGPUSpotrf")
+cuMultiGPUCpotrf = _mlapack("cuMultiGPUCpotrf")
+cuMultiGPUStrtri = _mlapack("cuMultiGPUStrtri")   # (handle, uplo, diag, n, A, lda)
+cuMultiGPUCtrtri = _mlapack("cuMultiGPUCtrtri")
+cuMultiGPUSlauum = _mlapack("cuMultiGPUSlauum")
+cuMultiGPUClauum = _mlapack("cuMultiGPUClauum")
+cuMultiGPUSpotri = _mlapack("cuMultiGPUSpotri")
+cuMultiGPUCpotri = _mlapack("cuMultiGPUCpotri")
 
 
 def cuMultiGPUDpotrf(handle, uplo, n, A, lda):

@@ -243,11 +243,17 @@ struct __cumultigpulapackhandle_st {
 namespace {
 
 using zc = cuDoubleComplex_t;
+using cc = cuComplex_t;
 
 bool is_zero(double v) { return v == 0.0; }
+bool is_zero(float v) { return v == 0.0f; }
 bool is_zero(zc v) { return __real__ v == 0.0 && __imag__ v == 0.0; }
+bool is_zero(cc v) { return __real__ v == 0.0f && __imag__ v == 0.0f; }
 bool is_one(double v) { return v == 1.0; }
+bool is_one(float v) { return v == 1.0f; }
 bool is_one(zc v) { return __real__ v == 1.0 && __imag__ v == 0.0; }
+bool is_one(cc v) { return __real__ v == 1.0f && __imag__ v == 0.0f; }
+template <typename T> constexpr bool complex_v() { return std::is_same<T, zc>::value || std::is_same<T, cc>::value; }
 template <typename T>
 T zero_of() {
   T z{};
@@ -297,6 +303,12 @@ CUresult mgemm(const char *name, CUmultiGPUBLAShandle h, CBlasTranspose ta, CBla
   if constexpr (std::is_same<T, double>::value)
     r = cuDgemm2(h->blas, ta, tb, m, n, kk, alpha, dp(dA), ldA, dp(dB), ldB, beta, dp(dC), ldC, dp(dC), ldC,
                  s.stream());
+  else if constexpr (std::is_same<T, float>::value)
+    r = cuSgemm2(h->blas, ta, tb, m, n, kk, alpha, dp(dA), ldA, dp(dB), ldB, beta, dp(dC), ldC, dp(dC), ldC,
+                 s.stream());
+  else if constexpr (std::is_same<T, cc>::value)
+    r = cuCgemm2(h->blas, ta, tb, m, n, kk, alpha, dp(dA), ldA, dp(dB), ldB, beta, dp(dC), ldC, dp(dC), ldC,
+                 s.stream());
   else
     r = cuZgemm2(h->blas, ta, tb, m, n, kk, alpha, dp(dA), ldA, dp(dB), ldB, beta, dp(dC), ldC, dp(dC), ldC,
                  s.stream());
@@ -308,7 +320,7 @@ CUresult mgemm(const char *name, CUmultiGPUBLAShandle h, CBlasTranspose ta, CBla
 template <typename T>
 CUresult mherk(const char *name, CUmultiGPUBLAShandle h, CBlasUplo uplo, CBlasTranspose trans, size_t n,
                size_t k, double alpha, const T *A, size_t lda, double beta, T *C, size_t ldc) {
-  constexpr bool cplx = !std::is_same<T, double>::value;
+  constexpr bool cplx = complex_v<T>();
   // reference blas/dsyrk.c:216-226, blas/zherk.c:245-257 (ZHERK rejects
   // trans = T with index 2)
   if (h == nullptr) return bad_arg(name, 1);
@@ -330,8 +342,12 @@ CUresult mherk(const char *name, CUmultiGPUBLAShandle h, CBlasUplo uplo, CBlasTr
   // C whole, also when beta = 0 (unread then): its other triangle comes back as it was
   HC_CHECK(stage_in(s, 2, C, ldc, n, n, ldC, &dC));
   CUresult r;
-  if constexpr (cplx)
+  if constexpr (std::is_same<T, zc>::value)
     r = cuZherk(h->blas, uplo, trans, n, kk, alpha, dp(dA), ldA, beta, dp(dC), ldC, s.stream());
+  else if constexpr (std::is_same<T, cc>::value)
+    r = cuCherk(h->blas, uplo, trans, n, kk, (float)alpha, dp(dA), ldA, (float)beta, dp(dC), ldC, s.stream());
+  else if constexpr (std::is_same<T, float>::value)
+    r = cuSsyrk(h->blas, uplo, trans, n, kk, (float)alpha, dp(dA), ldA, (float)beta, dp(dC), ldC, s.stream());
   else
     r = cuDsyrk(h->blas, uplo, trans, n, kk, alpha, dp(dA), ldA, beta, dp(dC), ldC, s.stream());
   if (r != CUDA_SUCCESS) return r;
@@ -363,10 +379,16 @@ CUresult mtri(const char *name, CUmultiGPUBLAShandle h, CBlasSide side, CBlasUpl
   HC_CHECK(stage_in(s, 1, zero ? nullptr : B, ldb, m, n, ldB, &dB));
   CUresult r;
   void *out = dB;
-  if (zero) {
+  if (zero) {  // B <- 0 on the device (a GEMM with k = 0 and beta = 0 reads nothing)
     const T z = zero_of<T>();
     if constexpr (std::is_same<T, double>::value)
       r = cuDgemm2(h->blas, CBlasNoTrans, CBlasNoTrans, m, n, 0, z, dp(dB), ldB, dp(dB), ldB, z, dp(dB), ldB,
+                   dp(dB), ldB, s.stream());
+    else if constexpr (std::is_same<T, float>::value)
+      r = cuSgemm2(h->blas, CBlasNoTrans, CBlasNoTrans, m, n, 0, z, dp(dB), ldB, dp(dB), ldB, z, dp(dB), ldB,
+                   dp(dB), ldB, s.stream());
+    else if constexpr (std::is_same<T, cc>::value)
+      r = cuCgemm2(h->blas, CBlasNoTrans, CBlasNoTrans, m, n, 0, z, dp(dB), ldB, dp(dB), ldB, z, dp(dB), ldB,
                    dp(dB), ldB, s.stream());
     else
       r = cuZgemm2(h->blas, CBlasNoTrans, CBlasNoTrans, m, n, 0, z, dp(dB), ldB, dp(dB), ldB, z, dp(dB), ldB,
@@ -374,12 +396,22 @@ CUresult mtri(const char *name, CUmultiGPUBLAShandle h, CBlasSide side, CBlasUpl
   } else if constexpr (TRSM) {
     if constexpr (std::is_same<T, double>::value)
       r = cuDtrsm(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, s.stream());
+    else if constexpr (std::is_same<T, float>::value)
+      r = cuStrsm(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, s.stream());
+    else if constexpr (std::is_same<T, cc>::value)
+      r = cuCtrsm(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, s.stream());
     else
       r = cuZtrsm(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, s.stream());
   } else {
     HC_CHECK(stage_in<T>(s, 3, nullptr, 0, m, n, ldB, &dX));  // the device multiply is out of place
     if constexpr (std::is_same<T, double>::value)
       r = cuDtrmm2(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, dp(dX), ldB,
+                   s.stream());
+    else if constexpr (std::is_same<T, float>::value)
+      r = cuStrmm2(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, dp(dX), ldB,
+                   s.stream());
+    else if constexpr (std::is_same<T, cc>::value)
+      r = cuCtrmm2(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, dp(dX), ldB,
                    s.stream());
     else
       r = cuZtrmm2(h->blas, side, uplo, trans, diag, m, n, alpha, dp(dA), ldA, dp(dB), ldB, dp(dX), ldB,
@@ -418,17 +450,28 @@ CUresult mlapack(const char *name, Lop op, CUmultiGPULAPACKhandle h, CBlasUplo u
   void *dA;
   HC_CHECK(stage_in(s, 0, A, lda, n, n, ldA, &dA));
   HC_CHECK(hipStreamSynchronize(s.stream()));  // the routines below run on the LAPACK handle's streams
-  constexpr bool cplx = !std::is_same<T, double>::value;
   CULAPACKhandle L = h->lapack;
   const CUdeviceptr a = dp(dA);
+  // the device routine of this precision: 0 double, 1 single, 2 double complex, 3 single complex
+  constexpr int P = std::is_same<T, double>::value ? 0 : std::is_same<T, float>::value ? 1 : std::is_same<T, zc>::value ? 2 : 3;
   CUresult r = CUDA_SUCCESS;
   switch (op) {
-    case Lop::potrf: r = cplx ? cuZpotrf(L, uplo, n, a, ldA, info) : cuDpotrf(L, uplo, n, a, ldA, info); break;
-    case Lop::trtri:
-      r = cplx ? cuZtrtri(L, uplo, diag, n, a, ldA, info) : cuDtrtri(L, uplo, diag, n, a, ldA, info);
+    case Lop::potrf:
+      r = P == 0 ? cuDpotrf(L, uplo, n, a, ldA, info) : P == 1 ? cuSpotrf(L, uplo, n, a, ldA, info)
+          : P == 2 ? cuZpotrf(L, uplo, n, a, ldA, info) : cuCpotrf(L, uplo, n, a, ldA, info);
       break;
-    case Lop::lauum: r = cplx ? cuZlauum(L, uplo, n, a, ldA, info) : cuDlauum(L, uplo, n, a, ldA, info); break;
-    case Lop::potri: r = cplx ? cuZpotri(L, uplo, n, a, ldA, info) : cuDpotri(L, uplo, n, a, ldA, info); break;
+    case Lop::trtri:
+      r = P == 0 ? cuDtrtri(L, uplo, diag, n, a, ldA, info) : P == 1 ? cuStrtri(L, uplo, diag, n, a, ldA, info)
+          : P == 2 ? cuZtrtri(L, uplo, diag, n, a, ldA, info) : cuCtrtri(L, uplo, diag, n, a, ldA, info);
+      break;
+    case Lop::lauum:
+      r = P == 0 ? cuDlauum(L, uplo, n, a, ldA, info) : P == 1 ? cuSlauum(L, uplo, n, a, ldA, info)
+          : P == 2 ? cuZlauum(L, uplo, n, a, ldA, info) : cuClauum(L, uplo, n, a, ldA, info);
+      break;
+    case Lop::potri:
+      r = P == 0 ? cuDpotri(L, uplo, n, a, ldA, info) : P == 1 ? cuSpotri(L, uplo, n, a, ldA, info)
+          : P == 2 ? cuZpotri(L, uplo, n, a, ldA, info) : cuCpotri(L, uplo, n, a, ldA, info);
+      break;
   }
   if (r != CUDA_SUCCESS) return r;
   HC_CHECK(hipStreamSynchronize(L->stream));
@@ -610,6 +653,55 @@ CUresult cuMultiGPUZtrsm(CUmultiGPUBLAShandle h, CBlasSide side, CBlasUplo uplo,
   return mtri<zc, true>("cuMultiGPUZtrsm", h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
 }
 
+// single and single complex (reference blas/ssyrk.c:214, blas/cherk.c:240,
+// blas/sgemm.c:432, blas/cgemm.c:509, blas/strmm.c:467, blas/ctrmm.c:518,
+// blas/strsm.c:267, blas/ctrsm.c:296)
+CUresult cuMultiGPUSsyrk(CUmultiGPUBLAShandle h, CBlasUplo uplo, CBlasTranspose trans, size_t n, size_t k,
+                         float alpha, const float *A, size_t lda, float beta, float *C, size_t ldc) {
+  return mherk<float>("cuMultiGPUSsyrk", h, uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+CUresult cuMultiGPUCherk(CUmultiGPUBLAShandle h, CBlasUplo uplo, CBlasTranspose trans, size_t n, size_t k,
+                         float alpha, const cc *A, size_t lda, float beta, cc *C, size_t ldc) {
+  return mherk<cc>("cuMultiGPUCherk", h, uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+CUresult cuMultiGPUSgemm(CUmultiGPUBLAShandle h, CBlasTranspose ta, CBlasTranspose tb, size_t m, size_t n,
+                         size_t k, float alpha, const float *A, size_t lda, const float *B, size_t ldb,
+                         float beta, float *C, size_t ldc) {
+  return mgemm<float>("cuMultiGPUSgemm", h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+CUresult cuMultiGPUCgemm(CUmultiGPUBLAShandle h, CBlasTranspose ta, CBlasTranspose tb, size_t m, size_t n,
+                         size_t k, cc alpha, const cc *A, size_t lda, const cc *B, size_t ldb, cc beta, cc *C,
+                         size_t ldc) {
+  return mgemm<cc>("cuMultiGPUCgemm", h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+CUresult cuMultiGPUStrmm(CUmultiGPUBLAShandle h, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
+                         CBlasDiag diag, size_t m, size_t n, float alpha, const float *A, size_t lda, float *B,
+                         size_t ldb) {
+  return mtri<float, false>("cuMultiGPUStrmm", h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+CUresult cuMultiGPUCtrmm(CUmultiGPUBLAShandle h, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
+                         CBlasDiag diag, size_t m, size_t n, cc alpha, const cc *A, size_t lda, cc *B,
+                         size_t ldb) {
+  return mtri<cc, false>("cuMultiGPUCtrmm", h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+CUresult cuMultiGPUStrsm(CUmultiGPUBLAShandle h, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
+                         CBlasDiag diag, size_t m, size_t n, float alpha, const float *A, size_t lda, float *B,
+                         size_t ldb) {
+  return mtri<float, true>("cuMultiGPUStrsm", h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+CUresult cuMultiGPUCtrsm(CUmultiGPUBLAShandle h, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
+                         CBlasDiag diag, size_t m, size_t n, cc alpha, const cc *A, size_t lda, cc *B,
+                         size_t ldb) {
+  return mtri<cc, true>("cuMultiGPUCtrsm", h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
+}
+
 // ---------------------------------------------------------------- LAPACK handle
 
 CUresult cuMultiGPULAPACKCreate(CUmultiGPULAPACKhandle *handle, CUmultiGPU mGPU) {
@@ -669,6 +761,36 @@ CUresult cuMultiGPUDpotri(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, do
 }
 CUresult cuMultiGPUZpotri(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, zc *A, size_t lda, long *info) {
   return mlapack<zc>("cuMultiGPUZpotri", Lop::potri, h, uplo, CBlasNonUnit, n, A, lda, info);
+}
+
+// single and single complex (reference lapack/spotrf.c:400, lapack/cpotrf.c:300,
+// lapack/strtri.c:474, lapack/ctrtri.c:469, lapack/slauum.c:308,
+// lapack/clauum.c:293, lapack/spotri.c:48, lapack/cpotri.c:47)
+CUresult cuMultiGPUSpotrf(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, float *A, size_t lda, long *info) {
+  return mlapack<float>("cuMultiGPUSpotrf", Lop::potrf, h, uplo, CBlasNonUnit, n, A, lda, info);
+}
+CUresult cuMultiGPUCpotrf(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, cc *A, size_t lda, long *info) {
+  return mlapack<cc>("cuMultiGPUCpotrf", Lop::potrf, h, uplo, CBlasNonUnit, n, A, lda, info);
+}
+CUresult cuMultiGPUStrtri(CUmultiGPULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, float *A, size_t lda,
+                          long *info) {
+  return mlapack<float>("cuMultiGPUStrtri", Lop::trtri, h, uplo, diag, n, A, lda, info);
+}
+CUresult cuMultiGPUCtrtri(CUmultiGPULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, cc *A, size_t lda,
+                          long *info) {
+  return mlapack<cc>("cuMultiGPUCtrtri", Lop::trtri, h, uplo, diag, n, A, lda, info);
+}
+CUresult cuMultiGPUSlauum(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, float *A, size_t lda, long *info) {
+  return mlapack<float>("cuMultiGPUSlauum", Lop::lauum, h, uplo, CBlasNonUnit, n, A, lda, info);
+}
+CUresult cuMultiGPUClauum(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, cc *A, size_t lda, long *info) {
+  return mlapack<cc>("cuMultiGPUClauum", Lop::lauum, h, uplo, CBlasNonUnit, n, A, lda, info);
+}
+CUresult cuMultiGPUSpotri(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, float *A, size_t lda, long *info) {
+  return mlapack<float>("cuMultiGPUSpotri", Lop::potri, h, uplo, CBlasNonUnit, n, A, lda, info);
+}
+CUresult cuMultiGPUCpotri(CUmultiGPULAPACKhandle h, CBlasUplo uplo, size_t n, cc *A, size_t lda, long *info) {
+  return mlapack<cc>("cuMultiGPUCpotri", Lop::potri, h, uplo, CBlasNonUnit, n, A, lda, info);
 }
 
 }  // extern "C"

@@ -170,7 +170,8 @@ CUresult cuCtrsm(CUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag,
  * the result is copied back before the call returns (the reference may still
  * be running its tasks until cuMultiGPUBLASSynchronize; here that call only
  * waits for the device).  Same argument order, in-place results and xerbla
- * indices as the reference.  Double and double complex.
+ * indices as the reference.  All four precisions (single and single complex
+ * run on the device routines above of the same precision).
  */
 typedef struct __cumultigpublashandle_st * CUmultiGPUBLAShandle;
 CUresult cuMultiGPUBLASCreate(CUmultiGPUBLAShandle *, CUmultiGPU);
@@ -205,6 +206,27 @@ CUresult cuMultiGPUDtrsm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTransp
 CUresult cuMultiGPUZtrsm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
                          size_t, cuDoubleComplex_t, const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *,
                          size_t);
+/* single and single complex (replace cuMultiGPUSsyrk / Cherk / Sgemm / Cgemm /
+ * Strmm / Ctrmm / Strsm / Ctrsm, reference blas/ssyrk.c:214, blas/cherk.c:240,
+ * blas/sgemm.c:432, blas/cgemm.c:509, blas/strmm.c:467, blas/ctrmm.c:518,
+ * blas/strsm.c:267, blas/ctrsm.c:296) */
+CUresult cuMultiGPUSsyrk(CUmultiGPUBLAShandle, CBlasUplo, CBlasTranspose, size_t, size_t,
+                         float, const float *, size_t, float, float *, size_t);
+CUresult cuMultiGPUCherk(CUmultiGPUBLAShandle, CBlasUplo, CBlasTranspose, size_t, size_t,
+                         float, const cuComplex_t *, size_t, float, cuComplex_t *, size_t);
+CUresult cuMultiGPUSgemm(CUmultiGPUBLAShandle, CBlasTranspose, CBlasTranspose, size_t, size_t, size_t,
+                         float, const float *, size_t, const float *, size_t, float, float *, size_t);
+CUresult cuMultiGPUCgemm(CUmultiGPUBLAShandle, CBlasTranspose, CBlasTranspose, size_t, size_t, size_t,
+                         cuComplex_t, const cuComplex_t *, size_t, const cuComplex_t *, size_t, cuComplex_t,
+                         cuComplex_t *, size_t);
+CUresult cuMultiGPUStrmm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, float, const float *, size_t, float *, size_t);
+CUresult cuMultiGPUCtrmm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, cuComplex_t, const cuComplex_t *, size_t, cuComplex_t *, size_t);
+CUresult cuMultiGPUStrsm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, float, const float *, size_t, float *, size_t);
+CUresult cuMultiGPUCtrsm(CUmultiGPUBLAShandle, CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t,
+                         size_t, cuComplex_t, const cuComplex_t *, size_t, cuComplex_t *, size_t);
 
 #ifdef __cplusplus
 }

@@ -116,7 +116,7 @@ CUresult cuClogdet(CULAPACKhandle, CUdeviceptr, size_t, size_t, float *, CUstrea
  * inverted by the device routines above and copied back.  As in the
  * reference, a numerical failure (info > 0) returns
  * CUDA_ERROR_INVALID_VALUE; an illegal argument calls xerbla with the
- * reference's index.  Double and double complex.
+ * reference's index.  All four precisions.
  */
 typedef struct __cumultigpulapackhandle_st * CUmultiGPULAPACKhandle;
 CUresult cuMultiGPULAPACKCreate(CUmultiGPULAPACKhandle *, CUmultiGPU);
@@ -135,6 +135,18 @@ CUresult cuMultiGPUZlauum(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuDoubleCom
 /* (replaces cuMultiGPUDpotri / cuMultiGPUZpotri, reference lapack/dpotri.c:48, lapack/zpotri.c:48) */
 CUresult cuMultiGPUDpotri(CUmultiGPULAPACKhandle, CBlasUplo, size_t, double *, size_t, long *);
 CUresult cuMultiGPUZpotri(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+/* single and single complex (replace cuMultiGPUSpotrf ... cuMultiGPUCpotri,
+ * reference lapack/spotrf.c:400, lapack/cpotrf.c:300, lapack/strtri.c:474,
+ * lapack/ctrtri.c:469, lapack/slauum.c:308, lapack/clauum.c:293,
+ * lapack/spotri.c:48, lapack/cpotri.c:47) */
+CUresult cuMultiGPUSpotrf(CUmultiGPULAPACKhandle, CBlasUplo, size_t, float *, size_t, long *);
+CUresult cuMultiGPUCpotrf(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuComplex_t *, size_t, long *);
+CUresult cuMultiGPUStrtri(CUmultiGPULAPACKhandle, CBlasUplo, CBlasDiag, size_t, float *, size_t, long *);
+CUresult cuMultiGPUCtrtri(CUmultiGPULAPACKhandle, CBlasUplo, CBlasDiag, size_t, cuComplex_t *, size_t, long *);
+CUresult cuMultiGPUSlauum(CUmultiGPULAPACKhandle, CBlasUplo, size_t, float *, size_t, long *);
+CUresult cuMultiGPUClauum(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuComplex_t *, size_t, long *);
+CUresult cuMultiGPUSpotri(CUmultiGPULAPACKhandle, CBlasUplo, size_t, float *, size_t, long *);
+CUresult cuMultiGPUCpotri(CUmultiGPULAPACKhandle, CBlasUplo, size_t, cuComplex_t *, size_t, long *);
 
 #ifdef __cplusplus
 }

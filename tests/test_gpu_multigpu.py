@@ -432,3 +432,104 @@ def test_lauum(gpu, mlapack, cplx, uplo):
 
 def test_handles_synchronize(gpu, mblas, mlapack):
     assert mblas.synchronize() == 0 and mlapack.synchronize() == 0
+
+
+# ---------------------------------------------------------------- single and single complex
+
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def f32(M):
+    return np.asfortranarray(M.astype(np.complex64 if np.iscomplexobj(M) else np.float32))
+
+
+def w64(M):
+    return np.asfortranarray(M.astype(np.complex128 if np.iscomplexobj(M) else np.float64))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_sc_gemm(gpu, mblas, cplx, ta, tb):
+    m, n, k = 70, 45, 33
+    r = rng(m, n, k, ord(ta), ord(tb), int(cplx), 9)
+    A = f32(unif(r, (m, k) if ta == "N" else (k, m), cplx))
+    B = f32(unif(r, (k, n) if tb == "N" else (n, k), cplx))
+    C = f32(unif(r, (m, n), cplx))
+    al, be = (0.75 - 0.5j, 1.25 + 0.25j) if cplx else (0.75, -1.25)
+    Cp = padded(C, 2, 5.0)
+    fn = gpu.cuMultiGPUCgemm if cplx else gpu.cuMultiGPUSgemm
+    assert fn(mblas, ord(ta), ord(tb), m, n, k, al, A, A.shape[0], B, B.shape[0], be, Cp, m + 2) == 0
+    ref = w64(C)
+    (O.zgemm if cplx else O.dgemm)(ta, tb, m, n, k, al, w64(A), w64(B), be, ref)
+    bound = 2 * (k + 2) * EPS32 * (abs(al) * np.abs(opm(w64(A), ta)) @ np.abs(opm(w64(B), tb)) + abs(be) * np.abs(w64(C)))
+    assert np.all(np.abs(Cp[:m] - ref) <= bound)
+    assert np.all(Cp[m:] == 5.0)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("uplo", "LU")
+def test_sc_syrk_herk(gpu, mblas, cplx, uplo):
+    n, k = 80, 40
+    r = rng(n, k, ord(uplo), int(cplx), 10)
+    A = f32(unif(r, (n, k), cplx))
+    C = f32(unif(r, (n, n), cplx))
+    Cg = C.copy(order="F")
+    fn = gpu.cuMultiGPUCherk if cplx else gpu.cuMultiGPUSsyrk
+    assert fn(mblas, ord(uplo), gpu.NoTrans, n, k, 0.5, A, n, 2.0, Cg, n) == 0
+    ref = w64(C)
+    (O.zherk if cplx else O.dsyrk)(uplo, "N", n, k, 0.5, w64(A), 2.0, ref)
+    mask = tri(np.ones((n, n)), uplo).astype(bool)
+    bound = 2 * (k + 2) * EPS32 * (0.5 * np.abs(w64(A)) @ np.abs(w64(A)).T + 2.0 * np.abs(w64(C)))
+    assert np.all(np.abs(Cg - ref)[mask] <= bound[mask])
+    np.testing.assert_array_equal(Cg[other_mask(n, uplo)], C[other_mask(n, uplo)])
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("solve", [False, True])
+def test_sc_trmm_trsm(gpu, mblas, cplx, solve):
+    m, n = 60, 35
+    r = rng(m, n, int(cplx), int(solve), 11)
+    T = tri_rand(r, m, cplx)
+    T = f32(T)
+    B = f32(unif(r, (m, n), cplx))
+    Bg = B.copy(order="F")
+    al = (0.5 + 0.5j) if cplx else 0.75
+    name = ("cuMultiGPUCtr" if cplx else "cuMultiGPUStr") + ("sm" if solve else "mm")
+    assert getattr(gpu, name)(mblas, gpu.Left, gpu.Lower, gpu.NoTrans, gpu.NonUnit, m, n, al,
+                              poison(T, "L", "N"), m, Bg, m) == 0
+    ref = w64(B)
+    if solve:
+        (O.ztrsm if cplx else O.dtrsm)("L", "L", "N", "N", m, n, al, w64(T), ref)
+        assert rel(w64(Bg), ref) <= 20 * m * EPS32
+    else:
+        out = O.fmat(m, n, m, np.complex128 if cplx else np.float64)
+        (O.ztrmm2 if cplx else O.dtrmm2)("L", "L", "N", "N", m, n, al, w64(T), w64(B), out)
+        assert rel(w64(Bg), out) <= 4 * m * EPS32
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("uplo", "LU")
+def test_sc_potrf_potri_trtri_lauum(gpu, mlapack, cplx, uplo):
+    n = 200
+    A = f32(O.hpd(n, n, 1.0) if cplx else O.spd(n, n, 1.0))
+    F = A.copy(order="F")
+    p = "C" if cplx else "S"
+    st, info = getattr(gpu, f"cuMultiGPU{p}potrf")(mlapack, ord(uplo), n, F, n)
+    assert st == 0 and info == 0
+    assert O.potrf_backward_error(w64(A), w64(F), uplo) <= 20 * n * EPS32
+    np.testing.assert_array_equal(F[other_mask(n, uplo)], A[other_mask(n, uplo)])
+    X = F.copy(order="F")
+    st, info = getattr(gpu, f"cuMultiGPU{p}potri")(mlapack, ord(uplo), n, X, n)
+    assert st == 0 and info == 0
+    assert O.inverse_residual(w64(A), w64(X), uplo) <= 100 * EPS32
+    T = F.copy(order="F")
+    st, info = getattr(gpu, f"cuMultiGPU{p}trtri")(mlapack, ord(uplo), gpu.NonUnit, n, T, n)
+    assert st == 0 and info == 0
+    ref, _ = (O.ztrtri if cplx else O.dtrtri)(uplo, "N", w64(F))
+    mask = tri(np.ones((n, n)), uplo).astype(bool)
+    assert rel(w64(T)[mask], ref[mask]) <= 4 * EPS32
+    Lm = F.copy(order="F")
+    st, info = getattr(gpu, f"cuMultiGPU{p}lauum")(mlapack, ord(uplo), n, Lm, n)
+    assert st == 0 and info == 0
+    refl, _ = (O.zlauum if cplx else O.dlauum)(uplo, w64(F))
+    assert rel(tri(w64(Lm), uplo), tri(refl, uplo)) <= 4 * EPS32
