@@ -1,16 +1,24 @@
 #!/usr/bin/env python3
-"""Measure the outer block size of the blocked factorisations on this GPU.
+"""Measure the tunable schedule parameters of the library on this GPU.
 
-For every precision (D: cuDpotrf, Z: cuZpotrf, S: cuSpotrf, B: the BF16
-factor of cuDsposvx) and every order in the sweep, each candidate block size
-is set with cuLAPACKSetBlockSize and the factorisation timed (best of a few
-runs, fresh input each time, HIP events on the device clock).  The winner
-for each order covers all n up to it; the table goes to tuning.txt beside
-the library (or --out), where the library reads it at first use:
+* Outer block size of the blocked factorisations: for every precision (D:
+  cuDpotrf, Z: cuZpotrf, S: cuSpotrf, B: the BF16 factor of cuDsposvx) and
+  every order in the sweep, each candidate block size is set with
+  cuLAPACKSetBlockSize and the factorisation timed (best of a few runs, fresh
+  input each time).  The winner for each order covers all n up to it.
+* Persistent schedule of the batched factorisation (cuPotrfBatchedRun /
+  cuPotriBatchedRun): workgroups per CU, from 1 up to the kernel's occupancy,
+  for FP64 / FP32 storage with and without the inverse, timed on a batch of
+  uneven sizes (median of a few graph launches, HIP events on the launch
+  stream).
+
+The table goes to tuning.txt beside the library (or --out), where the library
+reads it at first use:
 
     potrf_nb <precision> <n_max> <nb>
+    batched_per_cu <precision> <inverse 0|1> <workgroups per CU>
 
-    python cuda-cholesky_amd/autotune.py [--quick] [--out PATH]
+    python cuda-cholesky_amd/autotune.py [--quick] [--only nb|batched] [--out PATH]
 """
 import argparse
 import importlib.util
@@ -83,19 +91,76 @@ def time_one(torch, pkg, h, prec, n, A0, B, reps):
     return best
 
 
+def tune_batched(torch, pkg, h, count, reps, per_cu_max=6):
+    """batched_per_cu lines: a batch of `count` SPD matrices, n_i ~ U[8, 256]."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    sizes = rng.integers(8, 257, size=count)
+    offs = np.zeros(count, dtype=np.int64)
+    offs[1:] = np.cumsum(sizes.astype(np.int64) ** 2)[:-1]
+    A0 = torch.empty(int((sizes.astype(np.int64) ** 2).sum()), dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(6)
+    for n in np.unique(sizes):
+        idx = np.nonzero(sizes == n)[0]
+        G = torch.randn(len(idx), n, n, generator=g, device="cuda", dtype=torch.float64)
+        M = torch.bmm(G, G.transpose(1, 2).contiguous()) / n + 0.5 * torch.eye(n, device="cuda",
+                                                                                 dtype=torch.float64)
+        for j, i in enumerate(idx):
+            A0[offs[i]:offs[i] + n * n] = M[j].reshape(-1)
+        del G, M
+    s = torch.cuda.Stream()
+    logd = torch.empty(count, dtype=torch.float64, device="cuda")
+    info = torch.empty(count, dtype=torch.int32, device="cuda")
+    lines = []
+    for prec, dtype in (("D", torch.float64), ("S", torch.float32)):
+        src = A0.to(dtype)
+        work = torch.empty_like(src)
+        for inverse in (False, True):
+            times = {}
+            for per_cu in range(1, per_cu_max + 1):
+                os.environ["CUCHOLESKY_BATCHED_PER_CU"] = str(per_cu)
+                with pkg.BatchedPlan(h, pkg.Lower, prec, sizes.tolist(), offs.tolist()) as plan:
+                    assert plan.status == 0
+                    fn = plan.run_inverse if inverse else plan.run
+                    ts = []
+                    for r in range(reps + 1):
+                        work.copy_(src)
+                        torch.cuda.synchronize()
+                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        e0.record(s)
+                        assert fn(pkg.ptr(work), pkg.ptr(logd), pkg.ptr(info), s.cuda_stream) == 0
+                        e1.record(s)
+                        s.synchronize()
+                        if r > 0:  # the first launch captures the graph
+                            ts.append(e0.elapsed_time(e1))
+                    assert int(info.abs().sum().item()) == 0
+                times[per_cu] = sorted(ts)[len(ts) // 2]
+            os.environ.pop("CUCHOLESKY_BATCHED_PER_CU", None)
+            best = min(times, key=times.get)
+            detail = " ".join(f"{k}:{v:.3f}ms" for k, v in times.items())
+            lines.append(f"batched_per_cu {prec} {int(inverse)} {best}   # {count} matrices n in [8, 256] {detail}")
+            print(lines[-1], flush=True)
+        del src, work
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--quick", action="store_true", help="a reduced sweep (a minute or two)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.txt"))
+    ap.add_argument("--only", choices=["nb", "batched"], help="run one of the two sweeps")
     args = ap.parse_args()
     import torch
     pkg = package()
     pkg.lib()
     h = pkg.LAPACKHandle()
     sweep = QUICK if args.quick else SWEEP
-    lines = ["# outer block sizes measured by autotune.py on " + torch.cuda.get_device_name(0),
-             "# potrf_nb <precision> <n_max> <nb>   (seconds per candidate in the trailing comment)"]
+    lines = ["# schedule parameters measured by autotune.py on " + torch.cuda.get_device_name(0),
+             "# potrf_nb <precision> <n_max> <nb>   (seconds per candidate in the trailing comment)",
+             "# batched_per_cu <precision> <inverse> <workgroups per CU>"]
+    if args.only == "batched":
+        sweep = {}
     for prec, (orders, cands) in sweep.items():
         dtype = {"D": torch.float64, "Z": torch.complex128, "S": torch.float32, "B": torch.float64}[prec]
         for i, n in enumerate(orders):
@@ -114,7 +179,14 @@ def main():
             print(lines[-1], flush=True)
             del A0, B
             torch.cuda.empty_cache()
+    if args.only != "nb":
+        lines += tune_batched(torch, pkg, h, 4000 if args.quick else 20000, args.reps + 2)
     h.close()
+    if args.only is not None and os.path.exists(args.out):
+        # keep the entries of the sweep that did not run
+        keep = "batched_per_cu" if args.only == "nb" else "potrf_nb"
+        with open(args.out) as f:
+            lines += [ln.rstrip("\n") for ln in f if ln.startswith(keep)]
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {args.out}", flush=True)

@@ -117,13 +117,19 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
   }
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nv[a] > nv[b]; });
-  // persistent grid: every CU filled to the kernel's occupancy, no more (a
-  // workgroup that waits for a slot would start its share late)
+  // persistent grid: at most every CU filled to the kernel's occupancy (a
+  // workgroup that waits for a slot would start its share late); fewer per CU
+  // when the autotuner measured that faster ($CUCHOLESKY_BATCHED_PER_CU
+  // overrides both)
   int cus = 0;
   HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+  const char *env = std::getenv("CUCHOLESKY_BATCHED_PER_CU");
+  const int forced = env != nullptr ? std::atoi(env) : 0;
   auto per_cu = [&](bool inverse) {
-    return precision == 'D' ? hc::potrf_batched_blocks_per_cu<double>((char)uplo, inverse)
-                            : hc::potrf_batched_blocks_per_cu<float>((char)uplo, inverse);
+    const int occ = precision == 'D' ? hc::potrf_batched_blocks_per_cu<double>((char)uplo, inverse)
+                                     : hc::potrf_batched_blocks_per_cu<float>((char)uplo, inverse);
+    const int want = forced > 0 ? forced : hc::tuned_batched_per_cu(precision, inverse);
+    return want > 0 ? std::min(want, occ) : occ;
   };
   p->grid = std::max(1, std::min<int>((int)count, cus * per_cu(false)));
   p->grid_inverse = std::max(1, std::min<int>((int)count, cus * per_cu(true)));

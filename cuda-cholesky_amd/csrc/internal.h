@@ -277,6 +277,9 @@ hipError_t basis_combine(hipStream_t s, long n, long nrhs, int nv, int nvec, con
 // Measured outer block size for precision D / Z / S / B at order n, 0 when
 // the tuning table has no entry (tuning.cpp).
 long tuned_potrf_nb(char prec, long n);
+// Workgroups per CU of the batched factorisation's persistent grid measured
+// by the autotuner (prec D or S), or 0.
+int tuned_batched_per_cu(char prec, bool inverse);
 
 // LAPACK internals shared by the solvers (lapack_api.cpp).
 hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out);

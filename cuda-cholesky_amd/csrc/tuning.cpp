@@ -1,12 +1,14 @@
 // Block sizes chosen by measurement.
 //
 // cuda-cholesky_amd/autotune.py times the factorisations over a grid of
-// outer block sizes on the GPU at hand and writes the winners to a text
-// table, one line per entry:
+// outer block sizes, and the persistent schedule of the batched
+// factorisation, on the GPU at hand and writes the winners to a text table,
+// one line per entry:
 //     potrf_nb <precision> <n_max> <nb>
-// precision: D (FP64), Z (complex FP64), S (FP32 products), B (BF16
-// products).  For an order n the first line of that precision with
-// n <= n_max applies.  The table is read once, from $CUCHOLESKY_TUNING or
+//     batched_per_cu <precision> <inverse> <workgroups per CU>
+// precision: D (FP64), Z (complex FP64), S (FP32 products / storage), B
+// (BF16 products).  For an order n the first potrf_nb line of that precision
+// with n <= n_max applies.  The table is read once, from $CUCHOLESKY_TUNING or
 // else tuning.txt beside the library; without it the built-in defaults hold.
 #include "internal.h"
 
@@ -33,6 +35,9 @@ std::vector<Entry> &table() {
   return t;
 }
 
+// batched_per_cu entries, indexed [precision D/S][inverse]; 0 = none
+int g_batched_per_cu[2][2] = {{0, 0}, {0, 0}};
+
 void load() {
   std::string path;
   if (const char *e = std::getenv("CUCHOLESKY_TUNING")) {
@@ -53,21 +58,30 @@ void load() {
   char line[256];
   while (std::fgets(line, sizeof line, f) != nullptr) {
     if (line[0] == '#') continue;
-    if (std::sscanf(line, "%63s %7s %ld %ld", key, prec, &n_max, &nb) == 4 &&
-        std::strcmp(key, "potrf_nb") == 0 && nb > 0 && n_max > 0)
+    if (std::sscanf(line, "%63s %7s %ld %ld", key, prec, &n_max, &nb) != 4) continue;
+    if (std::strcmp(key, "potrf_nb") == 0 && nb > 0 && n_max > 0)
       table().push_back(Entry{prec[0], n_max, nb});
+    else if (std::strcmp(key, "batched_per_cu") == 0 && (prec[0] == 'D' || prec[0] == 'S') &&
+             (n_max == 0 || n_max == 1) && nb > 0 && nb <= 64)
+      g_batched_per_cu[prec[0] == 'S' ? 1 : 0][n_max] = (int)nb;
   }
   std::fclose(f);
 }
 
+std::once_flag g_once;
+
 }  // namespace
 
 long tuned_potrf_nb(char prec, long n) {
-  static std::once_flag once;
-  std::call_once(once, load);
+  std::call_once(g_once, load);
   for (const Entry &e : table())
     if (e.prec == prec && n <= e.n_max) return e.nb;
   return 0;
+}
+
+int tuned_batched_per_cu(char prec, bool inverse) {
+  std::call_once(g_once, load);
+  return g_batched_per_cu[prec == 'S' ? 1 : 0][inverse ? 1 : 0];
 }
 
 }  // namespace hc
