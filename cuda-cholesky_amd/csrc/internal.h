@@ -215,11 +215,12 @@ hipError_t s2d(hipStream_t s, long m, long n, const float *Y, long ldy, double *
 hipError_t add_s2d(hipStream_t s, long m, long n, const float *Y, long ldy, double *X, long ldx);
 // out[j] = max_i |X(i, j)|
 hipError_t colmax(hipStream_t s, long m, long n, const double *X, long ldx, double *out);
-// ||A||_inf of the symmetric matrix stored in triangle uplo (device scalar).
-constexpr long kNormChunk = 1024;
-long sym_norm_ws_doubles(long n);
-hipError_t sym_norm_inf(hipStream_t s, char uplo, long n, const double *A, long lda, double *ws,
-                        double *out);
+// The FP32 lower-form copy of to_float_lower and the infinity norm of the
+// symmetric matrix from one read of A (64 x 64 tiles; deterministic partial
+// sums in ws of to_float_norm_ws_doubles(n) doubles); *anorm on the device.
+long to_float_norm_ws_doubles(long n);
+hipError_t to_float_norm(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
+                         int *overflow, double *ws, double *anorm);
 
 // ---------------------------------------------------------------------------
 // symmetric matrix times a few right-hand sides (symm.hip)

@@ -107,45 +107,110 @@ __global__ void __launch_bounds__(256) colmax_kernel(int m, const double *X, lon
   if (threadIdx.x == 0) out[j] = r;
 }
 
-// Stored part of column j, diagonal included: rows j.. (lower) or ..j (upper).
-__global__ void __launch_bounds__(256) abs_colsum_kernel(int lower, int n, const double *A, long lda,
-                                                         double *colpart) {
-  const int j = blockIdx.x;
-  const int i0 = lower ? j : 0, i1 = lower ? n : j + 1;
-  double s = 0.0;
-  for (int i = i0 + threadIdx.x; i < i1; i += 256) s += fabs(A[(long)j * lda + i]);
-  const double r = block_sum<256>(s);
-  if (threadIdx.x == 0) colpart[j] = r;
-}
-
-// Stored part of row i without the diagonal, restricted to one chunk of
-// columns: columns < i (lower) or > i (upper).  Threads walk rows, so every
-// column step is one coalesced read.
-__global__ void __launch_bounds__(256) abs_rowsum_kernel(int lower, int n, const double *A, long lda,
-                                                         int chunk, double *part) {
-  const int i = blockIdx.x * 256 + threadIdx.x, cb = blockIdx.y;
-  if (i >= n) return;
-  const int c0 = cb * chunk, c1 = min(n, c0 + chunk);
-  const int lo = lower ? c0 : max(c0, i + 1), hi = lower ? min(c1, i) : c1;
-  double s = 0.0;
-  for (int c = lo; c < hi; c++) s += fabs(A[(long)c * lda + i]);
-  part[(long)cb * n + i] = s;
-}
-
-// ||A||_inf = max_i (colpart[i] + sum_cb part[cb][i]).
-__global__ void __launch_bounds__(1024) anorm_final_kernel(int n, int nchunk, const double *colpart,
-                                                           const double *part, double *out) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    double s = colpart[i];
-    for (int cb = 0; cb < nchunk; cb++) s += part[(long)cb * n + i];
-    v = (s > v || s != s) ? s : v;
+// One 64 x 64 lower-form tile per workgroup (a 1-D grid over the lower
+// tiles): the FP32 copy as in to_float_lower_kernel, and the tile's share of
+// the infinity norm of the symmetric matrix -- row sums of |a| over the tile
+// (diagonal included) to rowpart[bj][row], column sums of |a| strictly below
+// the diagonal to colpart[bi][column] -- so the matrix is read once for both.
+constexpr int NT = 64;
+__global__ void __launch_bounds__(256)
+to_float_norm_kernel(int lower, int n, const double *A, long lda, float *S, long lds, int *overflow,
+                     double *rowpart, double *colpart) {
+  __shared__ double t[NT][NT + 1];  // t[c][r]: lower-form element (r0 + r, c0 + c), 0 outside
+  const long tix = blockIdx.x;
+  int bi = (int)((sqrt(8.0 * (double)tix + 1.0) - 1.0) * 0.5);
+  while ((long)bi * (bi + 1) / 2 > tix) bi--;
+  while ((long)(bi + 1) * (bi + 2) / 2 <= tix) bi++;
+  const int bj = (int)(tix - (long)bi * (bi + 1) / 2);
+  const int r0 = bi * NT, c0 = bj * NT;
+  const int li = threadIdx.x & (NT - 1), g = threadIdx.x >> 6;
+  // lower: lanes over rows of A (lower-form rows); upper: lanes over rows of A,
+  // which are lower-form columns.  Loads from clamped addresses, all in flight.
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int k = g + 4 * u;
+    const int R = lower ? r0 + li : r0 + k, C = lower ? c0 + k : c0 + li;
+    const int Rc = min(R, n - 1), Cc = min(C, n - 1);
+    v[u] = lower ? A[(long)Cc * lda + Rc] : A[(long)Rc * lda + Cc];
   }
-  const double r = block_max<1024>(v);
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int k = g + 4 * u;
+    const int r = lower ? li : k, c = lower ? k : li;
+    const int R = r0 + r, C = c0 + c;
+    t[c][r] = (R < n && C < n && R >= C) ? v[u] : 0.0;
+  }
+  __syncthreads();
+  bool ovf = false;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {  // S column-major, lanes over rows
+    const int c = g + 4 * u, R = r0 + li, C = c0 + c;
+    if (R < n && C < n && R >= C) {
+      const double x = t[c][li];
+      ovf |= !(fabs(x) <= (double)FLT_MAX);
+      S[(long)C * lds + R] = (float)x;
+    }
+  }
+  if (threadIdx.x < NT) {  // row sums of the tile
+    const int R = r0 + threadIdx.x;
+    double sum = 0.0;
+    for (int c = 0; c < NT; c++) sum += fabs(t[c][threadIdx.x]);
+    if (R < n) rowpart[(long)bj * n + R] = sum;
+  } else if (threadIdx.x < 2 * NT) {  // column sums strictly below the diagonal
+    const int c = threadIdx.x - NT, C = c0 + c;
+    double sum = 0.0;
+    for (int r = 0; r < NT; r++)
+      if (r0 + r > C) sum += fabs(t[c][r]);
+    if (C < n) colpart[(long)bi * n + C] = sum;
+  }
+  if (ovf) *overflow = 1;
+}
+
+// Row i of the symmetric matrix: sum_{bj <= i / 64} rowpart[bj][i] +
+// sum_{bi >= i / 64} colpart[bi][i]; the block's maximum to bmax.
+__global__ void __launch_bounds__(256)
+tile_norm_rows_kernel(int n, int nt, const double *rowpart, const double *colpart, double *bmax) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const int b = i / NT;
+    for (int j = 0; j <= b; j++) v += rowpart[(long)j * n + i];
+    for (int j = b; j < nt; j++) v += colpart[(long)j * n + i];
+  }
+  const double r = block_max<256>(v);
+  if (threadIdx.x == 0) bmax[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(256) max_final_kernel(int count, const double *bmax, double *out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) v = (bmax[i] > v || bmax[i] != bmax[i]) ? bmax[i] : v;
+  const double r = block_max<256>(v);
   if (threadIdx.x == 0) out[0] = r;
 }
 
 }  // namespace
+
+long to_float_norm_ws_doubles(long n) {
+  const long nt = (n + NT - 1) / NT;
+  return 2 * nt * n + (n + 255) / 256 + 1;
+}
+
+hipError_t to_float_norm(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
+                         int *overflow, double *ws, double *anorm) {
+  if (n <= 0) return hipSuccess;
+  const long nt = (n + NT - 1) / NT;
+  double *rowpart = ws, *colpart = ws + nt * n, *bmax = ws + 2 * nt * n;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(to_float_norm_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s,
+                     uplo == 'L' ? 1 : 0, (int)n, A, lda, S, lds, overflow, rowpart, colpart);
+  HC_TRY(hipGetLastError());
+  hipLaunchKernelGGL(tile_norm_rows_kernel, dim3(nb), dim3(256), 0, s, (int)n, (int)nt, rowpart, colpart,
+                     bmax);
+  HC_TRY(hipGetLastError());
+  hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, s, (int)nb, bmax, anorm);
+  return hipGetLastError();
+}
 
 hipError_t to_float_lower(hipStream_t s, char uplo, long n, const double *A, long lda, float *S,
                           long lds, int *overflow) {
@@ -181,28 +246,6 @@ hipError_t add_s2d(hipStream_t s, long m, long n, const float *Y, long ldy, doub
 hipError_t colmax(hipStream_t s, long m, long n, const double *X, long ldx, double *out) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)n), dim3(256), 0, s, (int)m, X, ldx, out);
-  return hipGetLastError();
-}
-
-long sym_norm_ws_doubles(long n) {
-  const long nchunk = (n + kNormChunk - 1) / kNormChunk;
-  return n + nchunk * n + 1;
-}
-
-hipError_t sym_norm_inf(hipStream_t s, char uplo, long n, const double *A, long lda, double *ws,
-                        double *out) {
-  if (n <= 0) return hipSuccess;
-  const int lower = uplo == 'L' ? 1 : 0;
-  const long nchunk = (n + kNormChunk - 1) / kNormChunk;
-  double *colpart = ws, *part = ws + n;
-  hipLaunchKernelGGL(abs_colsum_kernel, dim3((unsigned)n), dim3(256), 0, s, lower, (int)n, A, lda,
-                     colpart);
-  HC_TRY(hipGetLastError());
-  hipLaunchKernelGGL(abs_rowsum_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nchunk),
-                     dim3(256), 0, s, lower, (int)n, A, lda, (int)kNormChunk, part);
-  HC_TRY(hipGetLastError());
-  hipLaunchKernelGGL(anorm_final_kernel, dim3(1), dim3(1024), 0, s, (int)n, (int)nchunk, colpart,
-                     part, out);
   return hipGetLastError();
 }
 

@@ -97,7 +97,7 @@ SposvLayout sposv_layout(CULAPACKhandle h, long n, long nrhs) {
   L.R = off;    off = align256(off + (size_t)n * nrhs * sizeof(double));          // residual
   L.RS = off;   off = align256(off + (size_t)n * nrhs * sizeof(float));           // FP32 rhs
   L.PART = off; off = align256(off + hc::symm_tall_ws_bytes(n, nrhs));  // symmetric products
-  L.NORM = off; off = align256(off + (size_t)hc::sym_norm_ws_doubles(n) * sizeof(double));
+  L.NORM = off; off = align256(off + (size_t)hc::to_float_norm_ws_doubles(n) * sizeof(double));
   L.DEV = off;  off = align256(off + (3 * (size_t)nrhs + 8) * sizeof(double));    // norms, flags
   L.P16 = off;  off = align256(off + hc::potrf_bf16_ws_bytes(h, n, true));          // bf16 panels
   L.V = off;    off = align256(off + (size_t)nrhs * (kGmresM + 1) * n * sizeof(double));  // Krylov bases
@@ -193,8 +193,12 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   hipStream_t s = h->stream;
 
   // ||A||_inf; refinement stops when every column has
-  // ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n)   (LAPACK dsposv, eps = 2^-53)
-  HC_CHECK(hc::sym_norm_inf(s, ul, N, Ad, (long)lda, NW, dnorm));
+  // ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n)   (LAPACK dsposv, eps = 2^-53).
+  // One read of A gives both the norm and the FP32 copy the first attempt
+  // factors (a later attempt converts again: the factor overwrites it).
+  HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+  HC_CHECK(hc::to_float_norm(s, ul, N, Ad, (long)lda, S, N, ovf, NW, dnorm));
+  bool converted = true;
   double anorm = 0.0;
   HC_CHECK(hipMemcpyAsync(&anorm, dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
   HC_CHECK(hipStreamSynchronize(s));
@@ -225,8 +229,11 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
 
   // One attempt with a low-precision factor; returns the LAPACK ITER code.
   auto attempt = [&](bool bf16, bool split, long *code) -> CUresult {
-    HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
-    HC_CHECK(hc::to_float_lower(s, ul, N, Ad, (long)lda, S, N, ovf));
+    if (!converted) {
+      HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+      HC_CHECK(hc::to_float_lower(s, ul, N, Ad, (long)lda, S, N, ovf));
+    }
+    converted = false;
     int flag = 0;
     CU_TRY(read_flag(&flag));
     if (flag) { *code = -2; return CUDA_SUCCESS; }       // narrowing overflowed
