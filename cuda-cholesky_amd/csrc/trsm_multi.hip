@@ -135,27 +135,72 @@ __device__ __forceinline__ int wait_ready(const int *ready, bool lowerlike, int 
   }
 }
 
-template <typename T> struct ChainLds {
+// Row-split accumulation (one 16-row strip per wave over all of k) or, for
+// the k-split (KS) form, partial sums of the four k quarters.
+template <typename T, bool KS> struct ChainLds {
   static constexpr int WLD = sizeof(T) == 4 ? SB + 16 : SB + 4;  // inv(M_ii), column stride
   T W[SB * WLD];
   T Y[16 * (SB + 1)];  // B_i - sum, 128 x 16 column-major
 };
+template <typename T> struct ChainLds<T, true> {
+  static constexpr int WLD = SB + 16;
+  T W[SB * WLD];
+  T Y[16 * (SB + 1)];
+  T P[4 * 16 * (SB + 1)];  // k-quarter partial sums, [quarter][column][row]
+};
 
-// staging copy of block (slice y, block j): element (k, c) of the 128 x 16
-// block at ((q * 4 + g) * 16 + c) * VE + e for k = 4 VE q + VE g + e
+// staging copy of block (slice y, block j), laid out for the consumers'
+// lanes: element (k, c) of the 128 x 16 block at
+//   row split: ((q * 4 + g) * 16 + c) * VE + e   for k = 4 VE q + VE g + e
+//   k split:   ((h * 4 + g) * 16 + c) * 8 + u    for k = 32 h + 4 u + g
 template <typename T>
 __device__ __forceinline__ T *stage_block(int *sync, int gy, int nblk, int y, int j) {
   T *base = reinterpret_cast<T *>(reinterpret_cast<char *>(sync) + chain_flags_bytes(gy, nblk));
   return base + ((long)y * nblk + j) * SB * 16;
+}
+template <typename T, bool KS>
+__device__ __forceinline__ int stage_index(int k, int c) {
+  if (KS) return (((k >> 5) * 4 + (k & 3)) * 16 + c) * 8 + ((k & 31) >> 2);
+  constexpr int VE = Slice<T>::VE;
+  const int q = k / (4 * VE), g = (k % (4 * VE)) / VE, e = k % VE;
+  return ((q * 4 + g) * 16 + c) * VE + e;
+}
+
+// k-split operand of the non-transposed FP32 product: wave (h, kq) takes rows
+// 64 h .. 64 h + 63 and k = 32 kq .. 32 kq + 31 of M_ij.  At step u lane
+// (i = l % 16, g = l / 16) loads rows 64 h + 4 i .. + 3 of column
+// k = 32 kq + 4 u + g as one 16-byte vector; element t feeds the product of
+// accumulator t, whose row i stands for block row 64 h + 4 i + t.
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void load_mk(v4f_t (&d)[8], const float *A, long lda, long ri, long rb, int h,
+                                        int kq, int i, int g, int ni, int nb) {
+  if (ni == SB && nb == SB) {
+    const float *p = A + (rb + 32 * kq + g) * lda + ri + 64 * h + 4 * i;
+#pragma unroll
+    for (int u = 0; u < 8; u++) d[u] = *reinterpret_cast<const v4f_t *>(p + 4 * u * lda);
+    return;
+  }
+  // partial block: element loads from clamped (always valid) addresses
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int k = 32 * kq + 4 * u + g, r = 64 * h + 4 * i + t;
+      const float x = A[(rb + min(k, nb - 1)) * lda + ri + min(r, ni - 1)];
+      d[u][t] = (k < nb && r < ni) ? x : 0.0f;
+    }
 }
 
 template <typename T, bool TRANS, bool VEC>
 __global__ void __launch_bounds__(CH_THREADS)
 trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T *Winv, const T *B,
                   long ldb, T *X, long ldx, int *sync) {
-  __shared__ ChainLds<T> sm;
+  // non-transposed FP32 with 16-byte loads: k-split accumulation
+  constexpr bool KS = !TRANS && VEC;
+  static_assert(!KS || sizeof(T) == 4, "k-split form is FP32 only");
+  __shared__ ChainLds<T, KS> sm;
   __shared__ int ticket, known_sh;
-  constexpr int WLD = ChainLds<T>::WLD;
+  constexpr int WLD = ChainLds<T, KS>::WLD;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lc = lane & 15, lg = lane >> 4;
   const int nblk = (n + SB - 1) / SB;
@@ -181,6 +226,7 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
   }
   // B_i in accumulator layout
   T bv[4];
+  if constexpr (!KS)
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int rr = 16 * wave + Acc16<T>::row(lane, r), col = c0 + lc;
@@ -190,7 +236,68 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
   Acc16<T> acc;
   acc.zero();
   int known = 0;  // positions below are ready and acquired; -1: waits abandoned
-  if (s > 0) {
+  if constexpr (KS) {
+    const int h = wave >> 2, kq = wave & 3;
+    Acc16<float> a4[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) a4[t].zero();
+    if (s > 0) {
+      v4f_t mv[8], mvn[8];
+      auto blk = [&](int sp) { return lowerlike ? sp : nblk - 1 - sp; };
+      auto rows_of = [&](int j) { return (int)min((long)SB, n - (long)j * SB); };
+      load_mk(mv, reinterpret_cast<const float *>(A), lda, ri, (long)blk(0) * SB, h, kq, lc, lg, ni,
+              rows_of(blk(0)));
+      for (int sp = 0; sp < s; sp++) {
+        const int j = blk(sp);
+        if (known >= 0 && sp >= known) {
+          if (wave == 0) {
+            const int kn = wait_ready(ready, lowerlike, nblk, sp, s, lane, err);
+            if (lane == 0) known_sh = kn;
+          }
+          __syncthreads();
+          known = known_sh;
+        }
+        // X_j rows 32 kq + 4 u + g, column lc: 8 values, two 16-byte vectors
+        v4f_t x0, x1;
+        {
+          const v4f_t *xp = reinterpret_cast<const v4f_t *>(
+              reinterpret_cast<const float *>(stage_block<T>(sync, gridDim.y, nblk, y, j)) +
+              ((kq * 4 + lg) * 16 + lc) * 8);
+          x0 = xp[0];
+          x1 = xp[1];
+        }
+        if (sp + 1 < s) {
+          const int jn = blk(sp + 1);
+          load_mk(mvn, reinterpret_cast<const float *>(A), lda, ri, (long)jn * SB, h, kq, lc, lg, ni,
+                  rows_of(jn));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const float xb = u < 4 ? x0[u & 3] : x1[u & 3];
+#pragma unroll
+          for (int t = 0; t < 4; t++) a4[t].mma(mv[u][t], xb);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) mv[u] = mvn[u];
+      }
+    }
+    // partial sums: accumulator t, lane l, register r is block row
+    // 64 h + 16 (l / 16) + 4 r + t, column l % 16
+    float *P = reinterpret_cast<float *>(sm.P) + kq * 16 * (SB + 1);
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) P[lc * (SB + 1) + 64 * h + 16 * lg + 4 * r + t] = a4[t].v[r];
+    __syncthreads();
+    // Y = B_i - (sum of the four quarters, in a fixed order)
+    for (int e = tid; e < 16 * SB; e += CH_THREADS) {
+      const int c = e / SB, rr = e % SB, col = c0 + c;
+      const T b = (rr < ni && col < nrhs) ? B[(long)col * ldb + ri + rr] : T(0);
+      const float *Q = reinterpret_cast<const float *>(sm.P) + c * (SB + 1) + rr;
+      const float sum = ((Q[0] + Q[16 * (SB + 1)]) + Q[32 * (SB + 1)]) + Q[48 * (SB + 1)];
+      sm.Y[c * (SB + 1) + rr] = b - sum;
+    }
+  } else if (s > 0) {
     Slice<T> m, mn, xs;
     auto blk = [&](int sp) { return lowerlike ? sp : nblk - 1 - sp; };
     auto rows_of = [&](int j) { return (int)min((long)SB, n - (long)j * SB); };
@@ -229,10 +336,12 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
     }
   }
   // Y = B_i - sum  ->  X_i = inv(M_ii) Y
+  if constexpr (!KS) {
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int rr = 16 * wave + Acc16<T>::row(lane, r);
-    sm.Y[lc * (SB + 1) + rr] = bv[r] - acc.v[r];
+    for (int r = 0; r < 4; r++) {
+      const int rr = 16 * wave + Acc16<T>::row(lane, r);
+      sm.Y[lc * (SB + 1) + rr] = bv[r] - acc.v[r];
+    }
   }
   __syncthreads();
   acc.zero();
@@ -245,12 +354,10 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
   // exact zeros: B, the sum and inv(M_ii) are zero there)
   {
     T *xt = stage_block<T>(sync, gridDim.y, nblk, y, i);
-    constexpr int VE = Slice<T>::VE;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int k = 16 * wave + Acc16<T>::row(lane, r), col = c0 + lc;
-      const int q = k / (4 * VE), g = (k % (4 * VE)) / VE, e = k % VE;
-      xt[((q * 4 + g) * 16 + lc) * VE + e] = acc.v[r];
+      xt[stage_index<T, KS>(k, lc)] = acc.v[r];
       if (k < ni && col < nrhs) X[(long)col * ldx + ri + k] = acc.v[r];
     }
   }
@@ -309,8 +416,12 @@ hipError_t trsm_multi(hipStream_t s, char uplo, char trans, long n, long nrhs, c
       hipLaunchKernelGGL((trsm_chain_kernel<T, true, false>), grid, block, 0, s, ll, nn, nr, A, lda,
                          Winv, B, ldb, X, ldx, sync);
   } else {
-    hipLaunchKernelGGL((trsm_chain_kernel<T, false, false>), grid, block, 0, s, ll, nn, nr, A, lda, Winv,
-                       B, ldb, X, ldx, sync);
+    if (vec)
+      hipLaunchKernelGGL((trsm_chain_kernel<T, false, sizeof(T) == 4>), grid, block, 0, s, ll, nn, nr, A,
+                         lda, Winv, B, ldb, X, ldx, sync);
+    else
+      hipLaunchKernelGGL((trsm_chain_kernel<T, false, false>), grid, block, 0, s, ll, nn, nr, A, lda,
+                         Winv, B, ldb, X, ldx, sync);
   }
   return hipGetLastError();
 }

@@ -234,6 +234,21 @@ def test_dsposv_fp32(gpu, lapack, uplo, n, nrhs, alpha):
     np.testing.assert_array_equal(np.nan_to_num(Aafter, nan=1e300), np.nan_to_num(Astore, nan=1e300))
 
 
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("n,nrhs", [(1024, 16), (1000, 7), (1001, 16), (2100, 33)])
+def test_dsposv_fp32_refinement_steps(gpu, lapack, uplo, n, nrhs):
+    """A well-conditioned system (cond ~ 5) refined from the FP32 factor gains
+    about seven digits per step, so it meets the stopping test within three
+    steps only if both FP32 triangular solves are exact to FP32 rounding: a
+    block read from the wrong rows or columns costs convergence.  n = 1024
+    (whole 128-row blocks) and 1000 (a partial last block) take the 16-byte
+    loads of the FP32 copy (ld = n, a multiple of 4); n = 1001 and 2100 the
+    element-wise loads; 33 right-hand sides span three 16-column slices."""
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, uplo, n, nrhs, 1.0, "S", seed=7)
+    assert used == "S" and 0 <= it <= 3, (it, used)
+    assert dsposv_converged(A, X, B)
+
+
 @pytest.mark.parametrize("n,nrhs", [(300, 3), (1200, 16)])
 def test_dsposv_bf16(gpu, lapack, n, nrhs):
     A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", n, nrhs, 1.0, "B", seed=1)
