@@ -32,8 +32,8 @@ SWEEP = {
     # precision: (orders, candidate block sizes)
     "D": ([2048, 4096, 8192, 16384], [256, 384, 512, 768]),
     "Z": ([2048, 4096, 8192], [128, 192, 256, 384]),
-    "S": ([4096, 8192, 16384], [384, 512, 768, 1024]),
-    "B": ([4096, 8192, 16384], [384, 512, 768, 1024]),
+    "S": ([4096, 8192, 16384, 32768], [384, 512, 768, 1024]),
+    "B": ([4096, 8192, 16384, 32768], [384, 512, 768, 1024]),
 }
 QUICK = {
     "D": ([4096, 16384], [256, 384, 512]),
