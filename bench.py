@@ -114,11 +114,11 @@ def roofline_live(records):
 
 
 def mixed_precision(torch, pkg, h, n, nrhs, alphas):
-    """SPD solve A X = B (n x n, nrhs right-hand sides) four ways: FP64
-    cuDpotrf + cuDpotrs, and the mixed-precision cuDsposvx with FP32 or BF16
-    trailing-update products (FP64 diagonal blocks, FP64 residuals) refined to
-    LAPACK dsposv's stopping test by classical iterative refinement (FP32,
-    BF16) or GMRES-IR (BF16).  Wall time of each whole
+    """SPD solve A X = B (n x n, nrhs right-hand sides): FP64 cuDpotrf +
+    cuDpotrs, and the mixed-precision cuDsposvx with FP32, BF16, split-BF16 or
+    FP16 trailing-update products (FP64 diagonal blocks, FP64 residuals)
+    refined to LAPACK dsposv's stopping test by classical iterative refinement
+    or GMRES-IR (BF16, FP16).  Wall time of each whole
     solve, refinement steps, and the normwise backward error
     max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)."""
     out = {"n": n, "nrhs": nrhs}
@@ -148,7 +148,7 @@ def mixed_precision(torch, pkg, h, n, nrhs, alphas):
         rec["fp64_ms"] = round(t64 * 1e3, 2)
         rec["fp64_backward_error"] = berr(X)
         del F
-        for factor, refine in (("S", "I"), ("B", "I"), ("X", "I"), ("B", "G")):
+        for factor, refine in (("S", "I"), ("B", "I"), ("X", "I"), ("H", "I"), ("B", "G"), ("H", "G")):
             X = torch.zeros_like(B)
             for rep in range(2):       # the first call also grows the workspace
                 torch.cuda.synchronize()
@@ -160,7 +160,7 @@ def mixed_precision(torch, pkg, h, n, nrhs, alphas):
                 assert st == 0 and info == 0, (st, info)
                 if it < 0:             # A now holds its FP64 factor: regenerate
                     A = gpu_spd(torch, n, alpha, seed=3)
-            key = {"S": "fp32", "B": "bf16", "X": "bf16x3"}[factor] + ("_gmres" if refine == "G" else "")
+            key = {"S": "fp32", "B": "bf16", "X": "bf16x3", "H": "fp16"}[factor] + ("_gmres" if refine == "G" else "")
             rec[key] = {"ms": round(dt * 1e3, 2), "iter": it, "used": used,
                         "backward_error": berr(X), "speedup_vs_fp64": round(t64 / dt, 2)}
         out[f"alpha={alpha:g}"] = rec

@@ -344,7 +344,7 @@ def cuDsposv(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx):
 
 
 def cuDsposvx(handle, uplo, n, nrhs, A, lda, B, ldb, X, ldx, factor="S", refine="I"):
-    """(status, iter, used precision 'B' / 'S' / 'D', info)"""
+    """(status, iter, used precision 'B' / 'X' / 'H' / 'S' / 'D', info)"""
     it, info = ctypes.c_long(0), ctypes.c_long(0)
     used = ctypes.create_string_buffer(1)
     enc = (lambda c: c.encode() if isinstance(c, str) else c)

@@ -322,7 +322,11 @@ struct Bf16Params {
   const int *abort_flag;
 };
 
-template <int MT, int NT, bool TG>
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+// F16: the 16-bit copies are IEEE binary16 (v_mfma_f32_16x16x32_f16) instead
+// of bfloat16; the two formats share the layout and the sign bit.
+template <int MT, int NT, bool TG, bool F16>
 __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p) {
   if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
   constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
@@ -370,10 +374,13 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   const int lc = lane & 15, lg = lane >> 4;
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
 
-  // With alpha = +-1 the accumulators start from beta C (loaded at the start,
-  // so the C read overlaps the whole K loop) and the sign of alpha goes into
-  // the A fragments; otherwise C is combined in the epilogue.
-  const bool seed = (p.alpha == 1.0f || p.alpha == -1.0f);
+  // With |alpha| a power of two the accumulators start from (beta / |alpha|) C
+  // (loaded at the start, so the C read overlaps the whole K loop), the sign
+  // of alpha goes into the A fragments and the sums are scaled by |alpha| at
+  // the end: both scalings are exact.  Otherwise C is combined in the epilogue.
+  const float aabs = fabsf(p.alpha);
+  const unsigned abits = __float_as_uint(aabs);
+  const bool seed = (abits & 0x007fffffu) == 0u && abits >= 0x00800000u && abits < 0x7f800000u;
   const bool use_beta = p.beta != 0.0f;
   v4f acc[BI][BJ];
   // C loads from clamped (valid) addresses with the condition outside the
@@ -395,8 +402,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   // issued after the first operand loads, consumed by the first products
   auto seed_acc = [&]() __attribute__((always_inline)) {
     if (seed && use_beta) {
-      if (p.beta == 1.0f) load_c(true, 1.0f);
-      else load_c(false, p.beta);
+      const float cs = p.beta / aabs;
+      if (cs == 1.0f) load_c(true, 1.0f);
+      else load_c(false, cs);
     } else {
 #pragma unroll
       for (int i = 0; i < BI; i++)
@@ -459,8 +467,14 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
 #pragma unroll
         for (int i = 0; i < BI; i++)
 #pragma unroll
-          for (int j = 0; j < BJ; j++)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < BJ; j++) {
+            if constexpr (F16)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, fb[j]),
+                                                                 __builtin_bit_cast(v8h, fa[i]), acc[i][j],
+                                                                 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+          }
       }
     };
     const int ntiles = kend / BK;
@@ -487,6 +501,12 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
     seed_acc();
   }
 
+  if (seed && aabs != 1.0f) {
+#pragma unroll
+    for (int i = 0; i < BI; i++)
+#pragma unroll
+      for (int j = 0; j < BJ; j++) acc[i][j] *= aabs;
+  }
   if (!seed) {  // D = alpha acc + beta C: scale, then add C loaded in one batch
 #pragma unroll
     for (int i = 0; i < BI; i++)
@@ -543,6 +563,34 @@ __global__ void panel_to_bf16_kernel(int m, int kp, const float *P, long ldp, u1
   }
 }
 
+// T (binary16, row r at T + r * ldt, k contiguous) <- scale P for rows 0..m-1
+// and k < kp, zeros for kp <= k < ldt.  P starts at a diagonal element: its
+// strictly upper part (r < k) is not referenced and is stored as zero, so
+// that whatever that triangle holds cannot be taken for an overflow.  scale
+// is a power of two chosen so that every entry of an SPD factor is below
+// 2^15 in magnitude; an entry that is not (or not finite) records column
+// col0 + k + 1 in *info, which stops the factorisation as a failed pivot would.
+__global__ void panel_to_f16_kernel(int m, int kp, const float *P, long ldp, u16 *T, long ldt, float scale,
+                                    int *info, long col0) {
+  __shared__ float t[32][33];
+  const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + tx, k = k0 + j;
+    t[j][tx] = (r < m && k < kp) ? scale * P[(long)k * ldp + r] : 0.0f;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, k = k0 + tx;
+    if (r < m && k < ldt) {
+      const float x = r >= k ? t[tx][j] : 0.0f;
+      if (!(fabsf(x) <= 65504.0f)) atomicCAS(info, 0, (int)(col0 + k + 1));
+      const _Float16 h = (_Float16)x;
+      T[(long)r * ldt + k] = __builtin_bit_cast(u16, h);
+    }
+  }
+}
+
 // Split copy for three-product BF16 updates: x = hi + lo (hi = bf16(x),
 // lo = bf16(x - hi)), stored as rows of 3 ldt values
 //   TA row r: [hi | hi | lo],  TB row r: [hi | lo | hi]
@@ -595,9 +643,44 @@ hipError_t panel_to_bf16(hipStream_t s, long m, long kp, const float *P, long ld
   return hipGetLastError();
 }
 
+hipError_t panel_to_f16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt,
+                        float scale, int *info, long col0) {
+  if (m <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((m + 31) / 32), (unsigned)((ldt + 31) / 32));
+  hipLaunchKernelGGL(panel_to_f16_kernel, grid, dim3(256), 0, s, (int)m, (int)kp, P, ldp,
+                     static_cast<u16 *>(T), ldt, scale, info, col0);
+  return hipGetLastError();
+}
+
+template <bool F16>
+hipError_t launch_nt16(hipStream_t s, const Bf16Params &p, long m, long n, bool tri_grid) {
+  const long tm = (m + 127) / 128, tn = (n + 127) / 128;
+  const long tiles = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
+  const bool big = tiles >= 240;
+  if (tri_grid) {
+    if (m != n) return hipErrorInvalidValue;
+    if (big) {
+      hipLaunchKernelGGL((gemm_bf16_nt_kernel<128, 128, true, F16>), dim3((unsigned)tiles),
+                         dim3(NTHREADS), 0, s, p);
+    } else {
+      const long t = (m + 63) / 64;
+      hipLaunchKernelGGL((gemm_bf16_nt_kernel<64, 64, true, F16>), dim3((unsigned)(t * (t + 1) / 2)),
+                         dim3(NTHREADS), 0, s, p);
+    }
+    return hipGetLastError();
+  }
+  if (big)
+    hipLaunchKernelGGL((gemm_bf16_nt_kernel<128, 128, false, F16>), dim3((unsigned)tm, (unsigned)tn),
+                       dim3(NTHREADS), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_bf16_nt_kernel<64, 64, false, F16>),
+                       dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(NTHREADS), 0, s, p);
+  return hipGetLastError();
+}
+
 hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, const void *A, long lda,
                         const void *B, long ldb, float beta, const float *C, long ldc, float *D,
-                        long ldd, int tri_out, bool tri_grid, const int *abort_flag) {
+                        long ldd, int tri_out, bool tri_grid, const int *abort_flag, bool f16) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (k % BK != 0 || lda % 8 != 0 || ldb % 8 != 0) return hipErrorInvalidValue;
   Bf16Params p;
@@ -608,28 +691,7 @@ hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, cons
   p.C = C; p.ldc = ldc; p.D = D; p.ldd = ldd;
   p.tri_out = tri_out;
   p.abort_flag = abort_flag;
-  const long tm = (m + 127) / 128, tn = (n + 127) / 128;
-  const long tiles = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
-  const bool big = tiles >= 240;
-  if (tri_grid) {
-    if (m != n) return hipErrorInvalidValue;
-    if (big) {
-      hipLaunchKernelGGL((gemm_bf16_nt_kernel<128, 128, true>), dim3((unsigned)tiles), dim3(NTHREADS),
-                         0, s, p);
-    } else {
-      const long t = (m + 63) / 64;
-      hipLaunchKernelGGL((gemm_bf16_nt_kernel<64, 64, true>), dim3((unsigned)(t * (t + 1) / 2)),
-                         dim3(NTHREADS), 0, s, p);
-    }
-    return hipGetLastError();
-  }
-  if (big)
-    hipLaunchKernelGGL((gemm_bf16_nt_kernel<128, 128, false>), dim3((unsigned)tm, (unsigned)tn),
-                       dim3(NTHREADS), 0, s, p);
-  else
-    hipLaunchKernelGGL((gemm_bf16_nt_kernel<64, 64, false>),
-                       dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(NTHREADS), 0, s, p);
-  return hipGetLastError();
+  return f16 ? launch_nt16<true>(s, p, m, n, tri_grid) : launch_nt16<false>(s, p, m, n, tri_grid);
 }
 
 template <>

@@ -191,11 +191,17 @@ constexpr int kLogdetMaxBlocks = 1024;
 // BF16 products from BF16 operands with K contiguous (gemm_lp.hip):
 // D (FP32) <- alpha A B^T + beta C with A m x k (row r at A + r lda), B n x k;
 // k a multiple of 64, lda and ldb multiples of 8, rows zero-padded to k.
+// f16: the operands are IEEE binary16 instead (FP16 products).
 hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, const void *A, long lda,
                         const void *B, long ldb, float beta, const float *C, long ldc, float *D,
-                        long ldd, int tri_out, bool tri_grid, const int *abort_flag);
+                        long ldd, int tri_out, bool tri_grid, const int *abort_flag, bool f16 = false);
 // T (bf16, row-major, ld ldt) <- the m x kp FP32 panel P, zeros for k >= kp.
 hipError_t panel_to_bf16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt);
+// T (binary16, row-major, ld ldt) <- scale P (scale a power of two); an entry
+// of magnitude 2^16 or more (or not finite) records column col0 + k + 1 in
+// *info when *info is still 0.
+hipError_t panel_to_f16(hipStream_t s, long m, long kp, const float *P, long ldp, void *T, long ldt,
+                        float scale, int *info, long col0);
 // Split copies (rows of 3 ldt bf16 values): TA = [hi | hi | lo], TB = [hi | lo | hi]
 // with x = hi + lo, so that a BF16 product TA TB^T over 3 ldt carries ~16
 // significant bits of the panel.
@@ -219,6 +225,8 @@ hipError_t colmax(hipStream_t s, long m, long n, const double *X, long ldx, doub
 // symmetric matrix from one read of A (64 x 64 tiles; deterministic partial
 // sums in ws of to_float_norm_ws_doubles(n) doubles); *anorm on the device.
 long to_float_norm_ws_doubles(long n);
+// out[0] = max_i |A(i, i)| (device)
+hipError_t diag_absmax(hipStream_t s, long n, const double *A, long lda, double *out);
 hipError_t to_float_norm(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
                          int *overflow, double *ws, double *anorm);
 
@@ -286,10 +294,11 @@ int tuned_batched_per_cu(char prec, bool inverse);
 hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out);
 // Blocked factorisation on the handle's streams; W: diag_max^2 elements.
 // bf16: FP32 data only, trailing updates with BF16 products (split: three
-// BF16 products of a two-term split of every panel value).
+// BF16 products of a two-term split of every panel value; f16_scale > 0:
+// FP16 products of the panel values times f16_scale, a power of two).
 template <typename T>
 hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
-                           void *lp_ws = nullptr, bool split = false);
+                           void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f);
 // Workspace of the bf16 factorisation (two bfloat16 panel copies; six with split).
 size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n, bool split = false);
 

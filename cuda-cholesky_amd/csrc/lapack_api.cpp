@@ -162,7 +162,7 @@ hipError_t timing_end(CULAPACKhandle h, hipStream_t s) {
 // products run over 3 ldt: three BF16 products, ~16 significant bits.
 template <typename T>
 hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false,
-                        void *lp_ws = nullptr, bool split = false) {
+                        void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f) {
   const long NB = potrf_nb(h, prec_of<T>(bf16), n);
   hipStream_t s0 = h->stream, s1 = h->side;
   const bool lower = (uplo == 'L');
@@ -170,6 +170,10 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
   auto at = [&](long r, long c) { return A + c * lda + r; };
   const long ldt = (NB + 63) / 64 * 64;
   split = split && bf16;
+  // FP16 copies of the panels scaled by f16_scale (2^e); the products carry
+  // f16_scale^2, undone exactly by alpha = -f16_scale^-2
+  const bool f16 = bf16 && !split && f16_scale > 0.0f;
+  const float lp_alpha = f16 ? -1.0f / (f16_scale * f16_scale) : -1.0f;
   const long kk = split ? 3 * ldt : ldt;  // K of the bf16 products (and row stride of the copies)
   // per panel buffer: the A-side copy, then (split) the B-side copy
   unsigned short *Pt[2], *Qt[2];
@@ -178,15 +182,16 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     Qt[b] = split ? Pt[b] + n * kk : Pt[b];
   }
   if (bf16 && (!lower || lp_ws == nullptr || !std::is_same<T, float>::value)) return hipErrorInvalidValue;
-  auto convert = [&](hipStream_t s, long m, long kp, const T *P, int b) -> hipError_t {
+  auto convert = [&](hipStream_t s, long m, long kp, const T *P, int b, long col0) -> hipError_t {
     if constexpr (std::is_same<T, float>::value) {
+      if (f16) return hc::panel_to_f16(s, m, kp, P, lda, Pt[b], ldt, f16_scale, h->dinfo, col0);
       return split ? hc::panel_to_bf16_split(s, m, kp, P, lda, Pt[b], Qt[b], ldt)
                    : hc::panel_to_bf16(s, m, kp, P, lda, Pt[b], ldt);
     }
     return hipErrorInvalidValue;
   };
   HC_TRY(factor_panel<T>(h, s0, lower, n, A, lda, W, 0, std::min(NB, n)));
-  if (bf16) HC_TRY(convert(s0, n, std::min(NB, n), A, 0));
+  if (bf16) HC_TRY(convert(s0, n, std::min(NB, n), A, 0, 0));
   for (long j = 0; j < n; j += NB) {
     const int pj = (int)((j / NB) & 1);
     const long jb = std::min(NB, n - j);
@@ -202,8 +207,8 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
       if constexpr (std::is_same<T, float>::value) {
         if (bf16) {
           const unsigned short *Pn = Pt[pj] + (jn - j) * kk, *Qn = Qt[pj] + (jn - j) * kk;
-          HC_TRY(hc::gemm_bf16_nt(s0, n - jn, jbn, kk, -1.0f, Pn, kk, Qn, kk, 1.0f, at(jn, jn), lda,
-                                  at(jn, jn), lda, hc::TRI_LOWER, false, h->dinfo));
+          HC_TRY(hc::gemm_bf16_nt(s0, n - jn, jbn, kk, lp_alpha, Pn, kk, Qn, kk, 1.0f, at(jn, jn), lda,
+                                  at(jn, jn), lda, hc::TRI_LOWER, false, h->dinfo, f16));
           done = true;
         }
       }
@@ -218,7 +223,7 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     HC_TRY(hipEventRecord(h->ev_update, s0));
     HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
     HC_TRY(factor_panel<T>(h, s1, lower, n, A, lda, W, jn, jbn));
-    if (bf16) HC_TRY(convert(s1, n - jn, jbn, at(jn, jn), pj ^ 1));
+    if (bf16) HC_TRY(convert(s1, n - jn, jbn, at(jn, jn), pj ^ 1, jn));
     HC_TRY(hipEventRecord(h->ev_panel, s1));
     const long t0 = jn + jbn, t = n - t0;   // trailing matrix beyond the next panel
     if (t > 0) {
@@ -228,8 +233,8 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
       if constexpr (std::is_same<T, float>::value) {
         if (bf16) {
           const unsigned short *Pn = Pt[pj] + (t0 - j) * kk, *Qn = Qt[pj] + (t0 - j) * kk;
-          HC_TRY(hc::gemm_bf16_nt(s0, t, t, kk, -1.0f, Pn, kk, Qn, kk, 1.0f, at(t0, t0), lda,
-                                  at(t0, t0), lda, hc::TRI_LOWER, true, h->dinfo));
+          HC_TRY(hc::gemm_bf16_nt(s0, t, t, kk, lp_alpha, Pn, kk, Qn, kk, 1.0f, at(t0, t0), lda,
+                                  at(t0, t0), lda, hc::TRI_LOWER, true, h->dinfo, f16));
           done = true;
         }
       }
@@ -530,13 +535,13 @@ hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out) {
 
 template <typename T>
 hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
-                           void *lp_ws, bool split) {
-  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws, split);
+                           void *lp_ws, bool split, float f16_scale) {
+  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws, split, f16_scale);
 }
 template hipError_t potrf_on_device<double>(CULAPACKhandle, char, long, double *, long, double *, bool,
-                                            void *, bool);
+                                            void *, bool, float);
 template hipError_t potrf_on_device<float>(CULAPACKhandle, char, long, float *, long, float *, bool,
-                                           void *, bool);
+                                           void *, bool, float);
 
 size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n, bool split) {
   const long ldt = (potrf_nb(h, 'B', n) + 63) / 64 * 64;

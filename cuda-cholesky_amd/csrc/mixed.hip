@@ -189,7 +189,24 @@ __global__ void __launch_bounds__(256) max_final_kernel(int count, const double 
   if (threadIdx.x == 0) out[0] = r;
 }
 
+// out[0] = max_i |A(i, i)|, one workgroup
+__global__ void __launch_bounds__(256) diag_absmax_kernel(int n, const double *A, long lda, double *out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double a = fabs(A[(long)i * lda + i]);
+    v = (a > v || a != a) ? a : v;
+  }
+  const double r = block_max<256>(v);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
 }  // namespace
+
+hipError_t diag_absmax(hipStream_t s, long n, const double *A, long lda, double *out) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(diag_absmax_kernel, dim3(1), dim3(256), 0, s, (int)n, A, lda, out);
+  return hipGetLastError();
+}
 
 long to_float_norm_ws_doubles(long n) {
   const long nt = (n + NT - 1) / NT;

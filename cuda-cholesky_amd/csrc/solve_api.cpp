@@ -155,7 +155,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   if (lda < (n > 1 ? n : 1)) { *info = -5; return bad_arg(name, 5); }
   if (ldb < (n > 1 ? n : 1)) { *info = -7; return bad_arg(name, 7); }
   if (ldx < (n > 1 ? n : 1)) { *info = -9; return bad_arg(name, 9); }
-  if (factor != 'S' && factor != 'B' && factor != 'X') { *info = -10; return bad_arg(name, 10); }
+  if (factor != 'S' && factor != 'B' && factor != 'X' && factor != 'H') { *info = -10; return bad_arg(name, 10); }
   if (refine != 'I' && refine != 'G') { *info = -11; return bad_arg(name, 11); }
   if (iter == nullptr) { *info = -12; return bad_arg(name, 12); }
   *iter = 0;
@@ -187,6 +187,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   double *dnorm = DEV;                       // ||A||_inf
   double *rn = DEV + 1, *xn = DEV + 1 + K;   // column max-norms
   int *ovf = reinterpret_cast<int *>(DEV + 1 + 2 * K);
+  double *dmax = DEV + 2 + 2 * K;            // max |a_ii|
   const double *Ad = dptr<double>(A);
   const double *Bd = dptr<double>(B);
   double *Xd = dptr<double>(X);
@@ -199,9 +200,19 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
   HC_CHECK(hc::to_float_norm(s, ul, N, Ad, (long)lda, S, N, ovf, NW, dnorm));
   bool converted = true;
-  double anorm = 0.0;
+  double anorm = 0.0, amax_diag = 0.0;
+  if (factor == 'H') HC_CHECK(hc::diag_absmax(s, N, Ad, (long)lda, dmax));
   HC_CHECK(hipMemcpyAsync(&anorm, dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (factor == 'H') HC_CHECK(hipMemcpyAsync(&amax_diag, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
   HC_CHECK(hipStreamSynchronize(s));
+  // FP16 panel copies are scaled by 2^e with every entry of the factor below
+  // 2^15: |l_ij| <= sqrt(a_ii) <= sqrt(max a_ii) for an SPD matrix
+  float f16_scale = 1.0f;
+  if (factor == 'H' && amax_diag > 0.0 && std::isfinite(amax_diag)) {
+    int e2 = 0;
+    std::frexp(std::sqrt(amax_diag), &e2);   // sqrt(max a_ii) < 2^e2
+    f16_scale = std::ldexp(1.0f, std::max(-60, std::min(60, 15 - e2)));
+  }
   const double cte = anorm * std::ldexp(1.0, -53) * std::sqrt((double)N);
 
   auto read_norms = [&](Norms &nm) -> CUresult {
@@ -228,7 +239,8 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   };
 
   // One attempt with a low-precision factor; returns the LAPACK ITER code.
-  auto attempt = [&](bool bf16, bool split, long *code) -> CUresult {
+  auto attempt = [&](char kind, long *code) -> CUresult {
+    const bool bf16 = kind != 'S', split = kind == 'X';
     if (!converted) {
       HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
       HC_CHECK(hc::to_float_lower(s, ul, N, Ad, (long)lda, S, N, ovf));
@@ -238,7 +250,8 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     CU_TRY(read_flag(&flag));
     if (flag) { *code = -2; return CUDA_SUCCESS; }       // narrowing overflowed
     HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), s));
-    HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16, wsb + L.P16, split));
+    HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16, wsb + L.P16, split,
+                                        kind == 'H' ? f16_scale : 0.0f));
     long finfo = 0;
     HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipStreamSynchronize(s));
@@ -372,10 +385,10 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
 
   HC_CHECK(hipMemsetAsync(sync, 0, sizeof(int), s));
   long code = 0;
-  CUresult r = attempt(factor != 'S', factor == 'X', &code);
+  CUresult r = attempt(factor, &code);
   if (r != CUDA_SUCCESS) return r;
-  if (code < 0 && factor != 'S') {  // BF16 products too coarse: FP32 next
-    r = attempt(false, false, &code);
+  if (code < 0 && factor != 'S') {  // 16-bit products too coarse: FP32 next
+    r = attempt('S', &code);
     if (r != CUDA_SUCCESS) return r;
     if (used != nullptr) *used = 'S';
   }

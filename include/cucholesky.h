@@ -28,7 +28,7 @@ CUresult cuLAPACKGetTiming(CULAPACKhandle handle, double * ms, double * flops, l
 /*
  * Outer block size of the blocked factorisations on this handle, per
  * precision: 'D' (cuDpotrf), 'Z' (cuZpotrf), 'S' (cuSpotrf and the FP32
- * factor of cuDsposvx), 'B' (its BF16 and split-BF16 factors).  nb = 0
+ * factor of cuDsposvx), 'B' (its BF16, split-BF16 and FP16 factors).  nb = 0
  * restores the default:
  * $CUCHOLESKY_POTRF_NB, else the table measured by autotune.py
  * (tuning.txt beside the library or $CUCHOLESKY_TUNING), else 384 (real) /
@@ -66,12 +66,16 @@ CUresult cuDsposv(CULAPACKhandle handle, CBlasUplo uplo, size_t n, size_t nrhs, 
  * factor = 'S' (FP32 products, as cuDsposv), 'B' (trailing-update products
  * in bfloat16, FP32 accumulation) or 'X' (three bfloat16 products of a
  * two-term split hi + lo of every panel value, ~16 significant bits, FP32
- * accumulation); refine = 'I' (classical iterative refinement, at most 30
+ * accumulation) or 'H' (products in IEEE binary16, 11 significant bits, FP32
+ * accumulation; the panel copies are scaled by a power of two 2^e with
+ * sqrt(max a_ii) 2^e <= 2^15, undone exactly, and an entry that still does
+ * not fit fails the attempt); refine = 'I' (classical iterative refinement, at most 30
  * steps, as cuDsposv) or 'G' (GMRES-IR: each step solves inv(M) A d =
  * inv(M) r by restarted GMRES(30) in FP64, M the low-precision factor,
- * relative tolerance 1e-10; at most 10 steps).  A 'B' or 'X' attempt that
- * fails is retried with 'S' before the FP64 fall-back; *used (may be NULL)
- * reports the precision of the returned solution: 'B', 'X', 'S' or 'D'.
+ * relative tolerance 1e-10; at most 10 steps).  A 'B', 'X' or 'H' attempt
+ * that fails is retried with 'S' before the FP64 fall-back; *used (may be
+ * NULL) reports the precision of the returned solution: 'B', 'X', 'H', 'S'
+ * or 'D'.
  * ITER counts refinement steps (GMRES cycles for 'G').
  */
 CUresult cuDsposvx(CULAPACKhandle handle, CBlasUplo uplo, size_t n, size_t nrhs, CUdeviceptr A,

@@ -272,6 +272,50 @@ def test_dsposv_split_bf16_large(gpu, lapack):
     assert dsposv_converged(A, X, B)
 
 
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("n,nrhs,alpha", [(300, 3, 1.0), (1200, 16, 1e-2), (2100, 4, 1e-3)])
+def test_dsposv_fp16(gpu, lapack, uplo, n, nrhs, alpha):
+    """FP16 products (11 significant bits) refine cond ~ 4 / alpha = 4e3, where
+    BF16 products (8 bits) do not converge."""
+    A, B, X, it, used, Aafter, Astore = run_sposv(gpu, lapack, uplo, n, nrhs, alpha, "H", seed=5)
+    assert it >= 0 and used == "H", (it, used)
+    assert dsposv_converged(A, X, B)
+    np.testing.assert_array_equal(np.nan_to_num(Aafter, nan=1e300), np.nan_to_num(Astore, nan=1e300))
+
+
+@pytest.mark.parametrize("scale", [1e10, 1e-10])
+def test_dsposv_fp16_scaled_matrix(gpu, lapack, scale):
+    """Factor entries reach sqrt(1e10) ~ 1e5, beyond FP16's 65504, or sit near
+    1e-5, in its subnormal range: the panel copies are scaled by a power of
+    two (undone exactly in the products), so the FP16 factor still refines
+    in a few steps."""
+    n = 1500
+    A = np.asfortranarray(O.spd(21, n, 1.0) * scale)
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", n, 8, 1.0, "H", seed=6, A=A)
+    assert used == "H" and 0 <= it <= 4, (it, used)
+    assert dsposv_converged(A, X, B)
+
+
+def test_dsposv_fp16_large(gpu, lapack):
+    # several panels with look-ahead (n > 2 NB), triangle-grid 128 x 128 FP16 trailing updates
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", 4500, 16, 1e-3, "H", seed=4)
+    assert it >= 0 and used == "H", (it, used)
+    assert dsposv_converged(A, X, B)
+
+
+def test_dsposv_fp16_not_spd_falls_back_to_fp64(gpu, lapack):
+    n = 200
+    A = O.spd(9, n, 1.0)
+    A[150, 150] = -5.0
+    B = np.ones((n, 1), order="F")
+    dA, dB = gpu.to_device(A), gpu.to_device(B)
+    dX = torch.zeros_like(dB)
+    st, it, used, info = gpu.cuDsposvx(lapack, ord("L"), n, 1, gpu.ptr(dA), n, gpu.ptr(dB), n,
+                                       gpu.ptr(dX), n, "H", "I")
+    torch.cuda.synchronize()
+    assert st == 0 and it == -3 and used == "D" and info == 151
+
+
 def test_dsposv_bf16_ill_conditioned_falls_back_to_fp32(gpu, lapack):
     # cond ~ 4e5: beyond what a bf16-product factor refines, within FP32's reach
     A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", 800, 2, 1e-5, "B", seed=2)
