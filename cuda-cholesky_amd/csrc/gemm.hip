@@ -103,6 +103,13 @@ __device__ __forceinline__ int xcd_contiguous(int w, int nwg) {
   return x < rem ? x * (base + 1) + i : rem * (base + 1) + (x - rem) * base + i;
 }
 
+// Position i of the order n - 1, 0, n - 2, 1, ... (high_first) or
+// 0, n - 1, 1, n - 2, ...: a permutation of [0, n).
+__device__ __forceinline__ int serpentine(int i, int n, bool high_first) {
+  const int j = i >> 1;
+  return ((i & 1) == 0) == high_first ? n - 1 - j : j;
+}
+
 template <typename T, int TA, int TB, int MT, int NT, bool TG>
 __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
@@ -162,6 +169,12 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     const int o = logical % per;
     bm = first + o % gs;
     bn = o / gs;
+    // A triangular operand makes a tile's K range grow along its row (op(A))
+    // or column (op(B)) index, and each XCD runs a contiguous band of the
+    // logical order: serpentine row / column blocks (heaviest first) so that
+    // every band holds heavy and light tiles alike.
+    if (p.tri_a != TRI_NONE) bm = serpentine(bm, gx, p.tri_a == TRI_LOWER);
+    if (p.tri_b != TRI_NONE) bn = serpentine(bn, gy, p.tri_b == TRI_UPPER);
   }
   const int m0 = bm * MT, n0 = bn * NT;
 
@@ -244,11 +257,19 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     T ra[EA], rb[EB];
     // Out-of-range elements read a valid address (the operand's origin) and
     // are replaced by zero, so the loads carry no branches.
-    // Interior tiles with whole K steps and no triangular operand take
-    // unpredicated loads; edge tiles check every element.
-    const bool fast = m0 + MT <= p.m && n0 + NT <= p.n && (kend - kbeg) % KT == 0 && p.tri_a == TRI_NONE &&
-                      p.tri_b == TRI_NONE;
+    // Interior tiles with whole K steps take unpredicated loads on every K
+    // step that does not meet a triangular operand's diagonal block: the K
+    // range is already cut to the triangle, so such a step lies strictly
+    // inside it.  Edge tiles and diagonal steps check every element.
+    const bool interior = m0 + MT <= p.m && n0 + NT <= p.n && (kend - kbeg) % KT == 0;
+    const bool tri_ops = p.tri_a != TRI_NONE || p.tri_b != TRI_NONE;
     auto load = [&](int kt_off, const T *qa, const T *qb) {
+      bool fast = interior;
+      if (tri_ops) {
+        const int k0 = kbeg + kt_off;
+        fast = fast && (p.tri_a == TRI_NONE || k0 + KT <= m0 || k0 >= m0 + MT) &&
+               (p.tri_b == TRI_NONE || k0 + KT <= n0 || k0 >= n0 + NT);
+      }
       if (fast) {
 #pragma unroll
         for (int q = 0; q < EA; q++) {
