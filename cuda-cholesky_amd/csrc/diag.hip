@@ -456,11 +456,17 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
   });
 }
 
+// Workgroup b inverts the diagonal block of order min(nb, ntot - b nb) at
+// (b nb, b nb) of A into the same place of B (one workgroup: nb = ntot).
 template <typename T, int NMAX>
 __global__ void __launch_bounds__(DIAG_THREADS)
-diag_trtri_kernel(int lower, int unit, int n, const T *A, long lda, T *B, long ldb, T *W, long ldw,
-                  const int *abort_flag) {
+diag_trtri_kernel(int lower, int unit, int ntot, int nb, const T *A, long lda, T *B, long ldb, T *W,
+                  long ldw, const int *abort_flag) {
   if (abort_flag != nullptr && *abort_flag != 0) return;
+  const long off = (long)blockIdx.x * nb;
+  const int n = (int)min((long)nb, (long)ntot - off);
+  A += off * lda + off;
+  B += off * ldb + off;
   constexpr int RB = Reg<T, NMAX>::RB, CB = Reg<T, NMAX>::CB;
   __shared__ T S[NMAX * (NMAX + 1)];
   __shared__ T vec[2 * NMAX];
@@ -478,13 +484,26 @@ diag_trtri_kernel(int lower, int unit, int n, const T *A, long lda, T *B, long l
   if (W != nullptr) store_dense<T, NMAX, DIAG_THREADS>(S, false, n, unit != 0, W, ldw);
 }
 
+// Workgroup b squares the diagonal block of order min(nb, ntot - b nb) at
+// (b nb, b nb) of A into B + b bstride (ld ldb); one workgroup in place:
+// nb = ntot, B = A, ldb = lda.
 template <typename T, int NMAX>
-__global__ void __launch_bounds__(LAUUM_THREADS) diag_lauum_kernel(int lower, int n, T *A, long lda) {
+__global__ void __launch_bounds__(LAUUM_THREADS)
+diag_lauum_kernel(int lower, int ntot, int nb, const T *A, long lda, T *B, long ldb, long bstride) {
   __shared__ T S[NMAX * (NMAX + 1)];
-  load_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, A, lda);
+  const long off = (long)blockIdx.x * nb;
+  const int n = (int)min((long)nb, (long)ntot - off);
+  load_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, A + off * lda + off, lda);
   __syncthreads();
   lds_lauum<T, NMAX>(S, n);
-  store_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, A, lda, false);
+  store_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, B + blockIdx.x * bstride, ldb, false);
+}
+
+// D <- S on the triangle uplo (diagonal included) of an n x n block
+template <typename T>
+__global__ void tri_copy_kernel(int lower, int n, const T *S, long lds, T *D, long ldd) {
+  const int r = blockIdx.x * 16 + (threadIdx.x & 15), c = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (r < n && c < n && (lower ? r >= c : r <= c)) D[(long)c * ldd + r] = S[(long)c * lds + r];
 }
 
 // Block b inverts the nb x nb diagonal block of op(A) starting at b * nb.
@@ -616,8 +635,20 @@ hipError_t diag_trtri(hipStream_t s, char uplo, char diag, int n, const T *A, lo
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
   hipLaunchKernelGGL((diag_trtri_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
-                     uplo == 'L' ? 1 : 0, diag == 'U' ? 1 : 0, n, A, lda, B, ldb, W, ldw,
+                     uplo == 'L' ? 1 : 0, diag == 'U' ? 1 : 0, n, n, A, lda, B, ldb, W, ldw,
                      abort_flag);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t diag_trtri_blocks(hipStream_t s, char uplo, char diag, long n, int nb, T *A, long lda) {
+  constexpr int NMAX = diag_max<T>();
+  if (n <= 0) return hipSuccess;
+  if (nb <= 0 || nb > NMAX || n > (long)INT32_MAX) return hipErrorInvalidValue;
+  const long blocks = (n + nb - 1) / nb;
+  hipLaunchKernelGGL((diag_trtri_kernel<T, NMAX>), dim3((unsigned)blocks), dim3(DIAG_THREADS), 0, s,
+                     uplo == 'L' ? 1 : 0, diag == 'U' ? 1 : 0, (int)n, nb, A, lda, A, lda,
+                     static_cast<T *>(nullptr), 0L, static_cast<const int *>(nullptr));
   return hipGetLastError();
 }
 
@@ -627,7 +658,26 @@ hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda) {
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
   hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3(1), dim3(LAUUM_THREADS), 0, s,
-                     uplo == 'L' ? 1 : 0, n, A, lda);
+                     uplo == 'L' ? 1 : 0, n, n, static_cast<const T *>(A), lda, A, lda, 0L);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t diag_lauum_blocks(hipStream_t s, char uplo, long n, int nb, const T *A, long lda, T *W) {
+  constexpr int NMAX = diag_max<T>();
+  if (n <= 0) return hipSuccess;
+  if (nb <= 0 || nb > NMAX || n > (long)INT32_MAX) return hipErrorInvalidValue;
+  const long blocks = (n + nb - 1) / nb;
+  hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3((unsigned)blocks), dim3(LAUUM_THREADS), 0, s,
+                     uplo == 'L' ? 1 : 0, (int)n, nb, A, lda, W, (long)nb, (long)nb * nb);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t tri_copy(hipStream_t s, char uplo, int n, const T *S, long lds, T *D, long ldd) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((n + 15) / 16));
+  hipLaunchKernelGGL((tri_copy_kernel<T>), grid, dim3(256), 0, s, uplo == 'L' ? 1 : 0, n, S, lds, D, ldd);
   return hipGetLastError();
 }
 
@@ -669,7 +719,10 @@ hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials
   template hipError_t diag_potrf<T>(hipStream_t, char, int, T *, long, T *, long, int *, long); \
   template hipError_t diag_trtri<T>(hipStream_t, char, char, int, const T *, long, T *, long,   \
                                     T *, long, const int *);                                    \
+  template hipError_t diag_trtri_blocks<T>(hipStream_t, char, char, long, int, T *, long);      \
   template hipError_t diag_lauum<T>(hipStream_t, char, int, T *, long);                         \
+  template hipError_t diag_lauum_blocks<T>(hipStream_t, char, long, int, const T *, long, T *); \
+  template hipError_t tri_copy<T>(hipStream_t, char, int, const T *, long, T *, long);          \
   template hipError_t trsm_diag_inverses<T>(hipStream_t, char, char, char, long, int,           \
                                             const T *, long, T *);                              \
   template hipError_t diag_zero_scan<T>(hipStream_t, char, long, const T *, long, int *);       \

@@ -171,6 +171,17 @@ hipError_t diag_potrf(hipStream_t s, char uplo, int n, T *A, long lda, T *W, lon
 template <typename T>
 hipError_t diag_trtri(hipStream_t s, char uplo, char diag, int n, const T *A, long lda, T *B,
                       long ldb, T *W, long ldw, const int *abort_flag);
+// Every nb x nb diagonal block of the n x n triangle inverted in place, one
+// workgroup each (the last block may be smaller).
+template <typename T>
+hipError_t diag_trtri_blocks(hipStream_t s, char uplo, char diag, long n, int nb, T *A, long lda);
+// U U^H / L^H L of every nb x nb diagonal block of the n x n triangle, block
+// b into W + b nb^2 (ld nb, triangle uplo only), one workgroup each.
+template <typename T>
+hipError_t diag_lauum_blocks(hipStream_t s, char uplo, long n, int nb, const T *A, long lda, T *W);
+// D <- S on the triangle uplo (diagonal included) of an n x n block.
+template <typename T>
+hipError_t tri_copy(hipStream_t s, char uplo, int n, const T *S, long lds, T *D, long ldd);
 // In-place U U^H (upper) or L^H L (lower) of a diagonal block.
 template <typename T>
 hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda);

@@ -256,14 +256,19 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
 // [X11 0; -X22 A21 X11  X22] (lower; upper is the mirror).  The reference's
 // loop (lapack/dtrtri.c:360-458) walks nb-column blocks; halving instead keeps
 // the two TRMMs per level large.  X holds an n2 x n1 intermediate.
+// The leaves (IB x IB diagonal blocks at multiples of IB: every split point
+// is one) only read their own block, which no product of the recursion
+// writes, so they are all inverted first, in one launch.
 template <typename T>
-hipError_t trtri_rec(hipStream_t s, char uplo, char diag, long n, T *A, long lda, T *X) {
+hipError_t trtri_rec(hipStream_t s, char uplo, char diag, long n, T *A, long lda, T *X, bool top = true) {
   constexpr int IB = hc::diag_max<T>();
-  if (n <= IB) return hc::diag_trtri<T>(s, uplo, diag, (int)n, A, lda, A, lda, nullptr, 0, nullptr);
+  if (n <= IB) return top ? hc::diag_trtri<T>(s, uplo, diag, (int)n, A, lda, A, lda, nullptr, 0, nullptr)
+                          : hipSuccess;
+  if (top) HC_TRY(hc::diag_trtri_blocks<T>(s, uplo, diag, n, IB, A, lda));
   const long n1 = ((n / 2 + IB - 1) / IB) * IB, n2 = n - n1;
   T *A11 = A, *A22 = A + n1 * lda + n1;
-  HC_TRY(trtri_rec(s, uplo, diag, n1, A11, lda, X));
-  HC_TRY(trtri_rec(s, uplo, diag, n2, A22, lda, X));
+  HC_TRY(trtri_rec(s, uplo, diag, n1, A11, lda, X, false));
+  HC_TRY(trtri_rec(s, uplo, diag, n2, A22, lda, X, false));
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0);
   if (uplo == 'L') {
     T *A21 = A + n1;
@@ -278,14 +283,18 @@ hipError_t trtri_rec(hipStream_t s, char uplo, char diag, long n, T *A, long lda
 // Recursive triangular square: lower L^H L =
 // [L11^H L11 + L21^H L21, .; L22^H L21, L22^H L22]; upper U U^H mirrors it.
 // The reference's loop is lapack/dlauum.c:188-281.
+// A leaf (IB x IB diagonal block at a multiple of IB) still holds its factor
+// block when the recursion reaches it, but the products above it read that
+// block until then, so the leaves are squared beforehand into `leaves`
+// (block b at leaves + b IB^2, one launch) and only copied into place here.
 template <typename T>
-hipError_t lauum_rec(hipStream_t s, char uplo, long n, T *A, long lda, T *X) {
+hipError_t lauum_rec(hipStream_t s, char uplo, long n, T *A, long lda, T *X, const T *leaves) {
   constexpr int IB = hc::diag_max<T>();
-  if (n <= IB) return hc::diag_lauum<T>(s, uplo, (int)n, A, lda);
+  if (n <= IB) return hc::tri_copy<T>(s, uplo, (int)n, leaves, IB, A, lda);
   const long n1 = ((n / 2 + IB - 1) / IB) * IB, n2 = n - n1;
   T *A11 = A, *A22 = A + n1 * lda + n1;
   const T one = hc::from_real<T>(1.0);
-  HC_TRY(lauum_rec(s, uplo, n1, A11, lda, X));
+  HC_TRY(lauum_rec(s, uplo, n1, A11, lda, X, leaves));
   if (uplo == 'L') {
     T *A21 = A + n1;
     HC_TRY(hc::herk<T>(s, 'L', 'C', n1, n2, 1.0, A21, lda, 1.0, A11, lda));
@@ -297,7 +306,7 @@ hipError_t lauum_rec(hipStream_t s, char uplo, long n, T *A, long lda, T *X) {
     HC_TRY(hc::trmm<T>(s, 'R', 'U', 'C', 'N', n1, n2, one, A22, lda, A12, lda, X, n1));
     HC_TRY(hc::copy2d<T>(s, n1, n2, X, n1, A12, lda));
   }
-  return lauum_rec(s, uplo, n2, A22, lda, X);
+  return lauum_rec(s, uplo, n2, A22, lda, X, leaves + (n1 / IB) * (long)IB * IB);
 }
 
 template <typename T>
@@ -374,8 +383,17 @@ CUresult lauum_api(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n,
   hc::DeviceGuard g(h->device);
   HC_CHECK(g.status());
   void *ws = nullptr;
-  HC_CHECK(workspace(h, rec_ws_bytes<T>((long)n) + 256, &ws));
-  HC_CHECK(lauum_rec<T>(h->stream, (char)uplo, (long)n, dptr<T>(A), (long)lda, static_cast<T *>(ws)));
+  constexpr int IB = hc::diag_max<T>();
+  if ((long)n <= IB) {
+    HC_CHECK(hc::diag_lauum<T>(h->stream, (char)uplo, (int)n, dptr<T>(A), (long)lda));
+  } else {
+    const size_t xb = (rec_ws_bytes<T>((long)n) + 255) / 256 * 256;
+    const size_t lb = (size_t)((n + IB - 1) / IB) * IB * IB * sizeof(T);
+    HC_CHECK(workspace(h, xb + lb, &ws));
+    T *leaves = reinterpret_cast<T *>(static_cast<char *>(ws) + xb);
+    HC_CHECK(hc::diag_lauum_blocks<T>(h->stream, (char)uplo, (long)n, IB, dptr<const T>(A), (long)lda, leaves));
+    HC_CHECK(lauum_rec<T>(h->stream, (char)uplo, (long)n, dptr<T>(A), (long)lda, static_cast<T *>(ws), leaves));
+  }
   HC_CHECK(hipStreamSynchronize(h->stream));
   return CUDA_SUCCESS;
 }
