@@ -381,6 +381,15 @@ def test_dsposv_gmres_bf16(gpu, lapack, uplo):
     assert dsposv_converged(A, X, B)
 
 
+@pytest.mark.parametrize("uplo", "LU")
+def test_dsposv_gmres_fp16(gpu, lapack, uplo):
+    """GMRES-IR with the FP16-product factor at cond ~ 4e4 (beyond what its
+    classical refinement contracts), over several panels with look-ahead."""
+    A, B, X, it, used, _, _ = run_sposv(gpu, lapack, uplo, 2600, 4, 1e-4, "H", seed=8, refine="G")
+    assert it >= 0 and used == "H", (it, used)
+    assert dsposv_converged(A, X, B)
+
+
 def test_dsposv_gmres_fp32_ill_conditioned(gpu, lapack):
     A, B, X, it, used, _, _ = run_sposv(gpu, lapack, "L", 900, 3, 1e-6, "S", seed=5, refine="G")
     assert it >= 0 and used == "S"
