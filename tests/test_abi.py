@@ -184,7 +184,7 @@ def test_solver_argument_indices(cc):
         assert cc.cuDpotrs(FAKE, cc.Lower, 10, 2, 0, 9, 0, 10) == (1, -5)
         assert cc.cuDpotrs(FAKE, cc.Lower, 10, 2, 0, 10, 0, 9) == (1, -7)
         assert cc.cuDsposv(FAKE, cc.Upper, 10, 2, 0, 10, 0, 10, 0, 9) == (1, 0, -9)
-        st, it, used, info = cc.cuDsposvx(FAKE, cc.Lower, 10, 2, 0, 10, 0, 10, 0, 10, "H")
+        st, it, used, info = cc.cuDsposvx(FAKE, cc.Lower, 10, 2, 0, 10, 0, 10, 0, 10, "Q")
         assert (st, info) == (1, -10)
         st, it, used, info = cc.cuDsposvx(FAKE, cc.Lower, 10, 2, 0, 10, 0, 10, 0, 10, "S", "X")
         assert (st, info) == (1, -11)
