@@ -280,6 +280,62 @@ def cpu_baseline(torch, n_cpu):
     }
 
 
+def blas3(torch, pkg, n):
+    """Standalone level-3 BLAS rates (best of 3 timed calls after one
+    warm-up, data in HBM): DGEMM / DSYRK / DTRSM / DTRMM at n x n x n and
+    ZGEMM / ZHERK at n / 2, TFLOP/s by the usual counts (GEMM 2n^3, SYRK /
+    TRSM / TRMM n^3; complex 4x; real flops for complex)."""
+    bh = pkg.BLASHandle()
+    out = {"n": n}
+    g = torch.Generator(device="cuda").manual_seed(9)
+    A = torch.randn(n * n, generator=g, device="cuda", dtype=torch.float64)
+    B = torch.randn(n * n, generator=g, device="cuda", dtype=torch.float64)
+    C = torch.zeros(n * n, device="cuda", dtype=torch.float64)
+    T = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64).tril_() / n
+    T.diagonal().add_(1.0)
+    T = T.T.contiguous().reshape(-1)          # column-major lower triangle, well conditioned
+    P = pkg.ptr
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            assert st == 0, st
+        return min(ts)
+
+    d = best(lambda: pkg.cuDgemm2(bh, pkg.NoTrans, pkg.NoTrans, n, n, n, 1.0, P(A), n, P(B), n, 0.0, P(C), n,
+                                  P(C), n))
+    out["dgemm_tflops"] = round(2 * n ** 3 / d / 1e12, 2)
+    d = best(lambda: pkg.cuDsyrk(bh, pkg.Lower, pkg.NoTrans, n, n, 1.0, P(A), n, 0.0, P(C), n))
+    out["dsyrk_tflops"] = round(n ** 3 / d / 1e12, 2)
+    d = best(lambda: pkg.cuDtrmm2(bh, pkg.Left, pkg.Lower, pkg.NoTrans, pkg.NonUnit, n, n, 1.0, P(T), n, P(B),
+                                  n, P(C), n))
+    out["dtrmm_tflops"] = round(n ** 3 / d / 1e12, 2)
+    Bw = B.clone()
+    d = best(lambda: (Bw.copy_(B), pkg.cuDtrsm(bh, pkg.Left, pkg.Lower, pkg.NoTrans, pkg.NonUnit, n, n, 1.0,
+                                               P(T), n, P(Bw), n))[1])
+    out["dtrsm_tflops_incl_copy"] = round(n ** 3 / d / 1e12, 2)
+    del A, B, C, T, Bw
+    nz = n // 2
+    Z = torch.randn(nz * nz * 2, generator=g, device="cuda", dtype=torch.float64)
+    W = torch.randn(nz * nz * 2, generator=g, device="cuda", dtype=torch.float64)
+    Zc = torch.zeros(nz * nz * 2, device="cuda", dtype=torch.float64)
+    d = best(lambda: pkg.cuZgemm2(bh, pkg.NoTrans, pkg.NoTrans, nz, nz, nz, 1.0, P(Z), nz, P(W), nz, 0.0,
+                                  P(Zc), nz, P(Zc), nz))
+    out["zgemm_tflops_real"] = round(8 * nz ** 3 / d / 1e12, 2)
+    d = best(lambda: pkg.cuZherk(bh, pkg.Lower, pkg.NoTrans, nz, nz, 1.0, P(Z), nz, 0.0, P(Zc), nz))
+    out["zherk_tflops_real"] = round(4 * nz ** 3 / d / 1e12, 2)
+    del Z, W, Zc
+    torch.cuda.empty_cache()
+    bh.close()
+    return out
+
+
 def host_api(torch, pkg, A0, n, reps=2):
     """cuMultiGPUDpotrf on a host copy of A: the PCIe-inclusive rate of the
     host-pointer API (upload, factor, download), after one call that sizes the
@@ -393,6 +449,7 @@ def main():
     ap.add_argument("--no-batched-mixed", action="store_true",
                     help="skip the batched mixed-precision solve + logdet + inverse (512 matrices, "
                          "n log-uniform in [128, 2048])")
+    ap.add_argument("--no-blas", action="store_true", help="skip the standalone level-3 BLAS rates")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-pointer cuMultiGPUDpotrf leg (PCIe-inclusive)")
     ap.add_argument("--counter-calibration", action="store_true",
@@ -486,6 +543,9 @@ def main():
         del Z, Z0, Zc
         log("[bench] complex done")
 
+    if not args.no_extra and not args.no_blas:
+        extra["blas3"] = blas3(torch, pkg, 8192)
+        log(f"[bench] blas3: {extra['blas3']}")
     if not args.no_host:
         extra["host_pointer_dpotrf"] = host_api(torch, pkg, A0, n)
         log(f"[bench] cuMultiGPUDpotrf (host arrays) n={n}: {extra['host_pointer_dpotrf']['ms']} ms")

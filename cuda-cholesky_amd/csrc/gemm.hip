@@ -171,10 +171,18 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     bn = o / gs;
     // A triangular operand makes a tile's K range grow along its row (op(A))
     // or column (op(B)) index, and each XCD runs a contiguous band of the
-    // logical order: serpentine row / column blocks (heaviest first) so that
-    // every band holds heavy and light tiles alike.
-    if (p.tri_a != TRI_NONE) bm = serpentine(bm, gx, p.tri_a == TRI_LOWER);
-    if (p.tri_b != TRI_NONE) bn = serpentine(bn, gy, p.tri_b == TRI_UPPER);
+    // logical order (one group of row blocks with all columns).  op(A):
+    // the row blocks are taken in serpentine order (every band holds heavy
+    // and light rows alike), and inside a band row by row, heaviest first,
+    // so that the longest tiles start first.  op(B): every band holds all
+    // columns already; they are taken heaviest first.
+    if (p.tri_a != TRI_NONE) {
+      const int q = o / gy, ne = (gs + 1) / 2;  // q-th heaviest row of the group
+      const int pos = q < ne ? 2 * q : 2 * (gs - ne - 1 - (q - ne)) + 1;
+      bm = serpentine(first + pos, gx, p.tri_a == TRI_LOWER);
+      bn = o % gy;
+    }
+    if (p.tri_b != TRI_NONE && p.tri_b == TRI_UPPER) bn = gy - 1 - bn;
   }
   const int m0 = bm * MT, n0 = bn * NT;
 
