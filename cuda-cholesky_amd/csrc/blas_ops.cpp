@@ -25,8 +25,10 @@ inline const T *op_at(const T *A, long lda, int op, long r, long c) {
   return op == OP_N ? A + c * lda + r : A + r * lda + c;
 }
 
-// Order of the diagonal blocks the TRSM inverts (fits diag_max for both types).
-constexpr int kTrsmNb = 64;
+// Order of the diagonal blocks the TRSM inverts: the largest block one
+// workgroup inverts in LDS (128 real, 64 complex), so that the leaves are
+// few and the coupling updates have K >= 128.
+template <typename T> constexpr int trsm_nb() { return diag_max<T>(); }
 
 }  // namespace
 
@@ -47,9 +49,9 @@ hipError_t trmm(hipStream_t s, char side, char uplo, char trans, char diag, long
   return gemm<T>(s, OP_N, top, m, n, n, alpha, B, ldb, A, lda, zero, nullptr, 0, X, ldx, o);
 }
 
-long trsm_ws_elems(char side, long m, long n) {
+long trsm_ws_elems(char side, long m, long n) {  // enough for either block order
   const long t = (side == 'L') ? m : n;
-  return ((t + kTrsmNb - 1) / kTrsmNb) * kTrsmNb * kTrsmNb;
+  return ((t + 127) / 128) * 128 * 128;
 }
 
 namespace {
@@ -69,12 +71,13 @@ struct TrsmCtx {
 };
 
 // Solve on the triangle range [off, off + len) of op(A) (off a multiple of
-// kTrsmNb).  Recursive halving keeps the coupling updates as large GEMMs; the
+// trsm_nb<T>()).  Recursive halving keeps the coupling updates as large GEMMs; the
 // leaves multiply by the inverted diagonal block in place (the GEMM tile spans
 // the whole block, so every workgroup reads its operand before writing it).
 template <typename T>
 hipError_t trsm_rec(const TrsmCtx<T> &c, long off, long len, T alpha) {
   const T one = from_real<T>(1.0), mone = from_real<T>(-1.0), zero = from_real<T>(0.0);
+  constexpr int kTrsmNb = trsm_nb<T>();
   if (len <= kTrsmNb) {
     const T *Wb = c.W + (off / kTrsmNb) * kTrsmNb * kTrsmNb;
     GemmOpts o;
@@ -128,7 +131,7 @@ hipError_t trsm(hipStream_t s, char side, char uplo, char trans, char diag, long
                 T alpha, const T *A, long lda, T *B, long ldb, T *ws) {
   if (m <= 0 || n <= 0) return hipSuccess;
   const long t = (side == 'L') ? m : n;
-  HC_TRY(trsm_diag_inverses<T>(s, uplo, trans, diag, t, kTrsmNb, A, lda, ws));
+  HC_TRY(trsm_diag_inverses<T>(s, uplo, trans, diag, t, trsm_nb<T>(), A, lda, ws));
   TrsmCtx<T> c;
   c.s = s;
   c.left = (side == 'L');
