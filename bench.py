@@ -283,8 +283,9 @@ def cpu_baseline(torch, n_cpu):
 def blas3(torch, pkg, n):
     """Standalone level-3 BLAS rates (best of 3 timed calls after one
     warm-up, data in HBM): DGEMM / DSYRK / DTRSM / DTRMM at n x n x n and
-    ZGEMM / ZHERK at n / 2, TFLOP/s by the usual counts (GEMM 2n^3, SYRK /
-    TRSM / TRMM n^3; complex 4x; real flops for complex)."""
+    SGEMM / SSYRK (FP32 matrix cores) at n, ZGEMM / ZHERK at n / 2, TFLOP/s by
+    the usual counts (GEMM 2n^3, SYRK / TRSM / TRMM n^3; complex 4x; real
+    flops for complex)."""
     bh = pkg.BLASHandle()
     out = {"n": n}
     g = torch.Generator(device="cuda").manual_seed(9)
@@ -321,6 +322,15 @@ def blas3(torch, pkg, n):
                                                P(T), n, P(Bw), n))[1])
     out["dtrsm_tflops_incl_copy"] = round(n ** 3 / d / 1e12, 2)
     del A, B, C, T, Bw
+    As = torch.randn(n * n, generator=g, device="cuda", dtype=torch.float32)
+    Bs = torch.randn(n * n, generator=g, device="cuda", dtype=torch.float32)
+    Cs = torch.zeros(n * n, device="cuda", dtype=torch.float32)
+    d = best(lambda: pkg.cuSgemm2(bh, pkg.NoTrans, pkg.NoTrans, n, n, n, 1.0, P(As), n, P(Bs), n, 0.0, P(Cs), n,
+                                  P(Cs), n))
+    out["sgemm_tflops"] = round(2 * n ** 3 / d / 1e12, 2)
+    d = best(lambda: pkg.cuSsyrk(bh, pkg.Lower, pkg.NoTrans, n, n, 1.0, P(As), n, 0.0, P(Cs), n))
+    out["ssyrk_tflops"] = round(n ** 3 / d / 1e12, 2)
+    del As, Bs, Cs
     nz = n // 2
     Z = torch.randn(nz * nz * 2, generator=g, device="cuda", dtype=torch.float64)
     W = torch.randn(nz * nz * 2, generator=g, device="cuda", dtype=torch.float64)
