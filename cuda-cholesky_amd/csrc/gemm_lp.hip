@@ -141,7 +141,25 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_lp_kernel(LpParams p) {
     const int ar = m0 + PA::r0(tid), ak = PA::k0(tid);
     const int br = n0 + PB::r0(tid), bk = PB::k0(tid);
     float ra[EA], rb[EB];
+    // Interior tiles with whole K steps load through per-thread pointers
+    // advanced by one K step at a time: every element offset is a
+    // compile-time multiple of the (uniform) leading dimension, so the loads
+    // carry no per-element address arithmetic and no predication.
+    const bool interior = m0 + MT <= p.m && n0 + NT <= p.n && kend % KT == 0;
+    const float *pa = (TA == OP_N) ? p.A + (long)ak * p.lda + ar : p.A + (long)ar * p.lda + ak;
+    const float *pb = (TB == OP_N) ? p.B + (long)br * p.ldb + bk : p.B + (long)bk * p.ldb + br;
+    const long a_kstride = (TA == OP_N) ? (long)KT * p.lda : (long)KT;
+    const long b_kstride = (TB == OP_N) ? (long)KT : (long)KT * p.ldb;
     auto load = [&](int kb) {
+      if (interior) {
+#pragma unroll
+        for (int q = 0; q < EA; q++)
+          ra[q] = pa[(TA == OP_N) ? (long)PA::dk(q) * p.lda + PA::dr(q) : (long)PA::dr(q) * p.lda + PA::dk(q)];
+#pragma unroll
+        for (int q = 0; q < EB; q++)
+          rb[q] = pb[(TB == OP_N) ? (long)PB::dr(q) * p.ldb + PB::dk(q) : (long)PB::dk(q) * p.ldb + PB::dr(q)];
+        return;
+      }
 #pragma unroll
       for (int q = 0; q < EA; q++) {
         const int gi = ar + PA::dr(q), gk = kb + ak + PA::dk(q);
@@ -180,7 +198,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_lp_kernel(LpParams p) {
     __syncthreads();
     for (int kt = 0; kt < ntiles; kt++) {
       const bool more = kt + 1 < ntiles;
-      if (more) load((kt + 1) * KT);
+      if (more) {
+        pa += a_kstride;
+        pb += b_kstride;
+        load((kt + 1) * KT);
+      }
       const SE *cA = sA[kt & 1], *cB = sB[kt & 1];
       if constexpr (MODE == MODE_F32) {
 #pragma unroll
