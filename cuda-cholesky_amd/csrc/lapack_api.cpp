@@ -73,12 +73,8 @@ template <typename T> char prec_of(bool bf16) {
 template <typename T>
 hipError_t factor_panel(CULAPACKhandle h, hipStream_t s, bool lower, long n, T *A, long lda, T *W,
                         long j, long jb) {
-  // panel slice width: one diagonal block in LDS ($CUCHOLESKY_IB may narrow it)
-  static const int IB = [] {
-    const char *e = std::getenv("CUCHOLESKY_IB");
-    const int v = e ? std::atoi(e) : 0;
-    return (v >= 16 && v <= hc::diag_max<T>()) ? v : hc::diag_max<T>();
-  }();
+  // panel slice width: one diagonal block in LDS
+  constexpr int IB = hc::diag_max<T>();
   const T one = hc::from_real<T>(1.0), mone = hc::from_real<T>(-1.0), zero = hc::from_real<T>(0.0);
   auto at = [&](long r, long c) { return A + c * lda + r; };
   const char uplo = lower ? 'L' : 'U';
