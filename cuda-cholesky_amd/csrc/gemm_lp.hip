@@ -244,7 +244,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_lp_kernel(LpParams p) {
   // ---- epilogue: lane lc holds row m0 + wm0 + 16 i + lc; register r of lane
   // group lg holds column n0 + wn0 + 16 j + 4 lg + r.  With beta != 0, C is
   // read in one batch from clamped (valid) addresses, so that all its loads
-  // are in flight together, then combined ----
+  // are in flight together, then combined.  C enters here and not as the
+  // accumulators' starting value: in a trailing update C and the products
+  // nearly cancel, and FP32 sums that start from C round every partial sum
+  // at C's magnitude (measured: the FP32 factor of a cond ~4e6 matrix then
+  // no longer refines) ----
 #pragma unroll
   for (int i = 0; i < BI; i++)
 #pragma unroll
