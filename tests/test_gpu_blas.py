@@ -205,6 +205,16 @@ def test_dtrmm2(gpu, blas, side, uplo, trans, diag, m, n):
     run_trmm(gpu, blas, side, uplo, trans, diag, m, n, False)
 
 
+@pytest.mark.parametrize("side,uplo,trans", list(itertools.product("LR", "LU", "NT")))
+def test_dtrmm2_many_blocks(gpu, blas, side, uplo, trans):
+    """1100 rows (left) or columns (right) of the triangular operand: several
+    groups of row / column blocks, the last one partial, so that the tile
+    order for triangular operands (serpentine across groups, heaviest first
+    inside one) is exercised, with a diagonal block cut by the edge."""
+    m, n = (1100, 300) if side == "L" else (300, 1100)
+    run_trmm(gpu, blas, side, uplo, trans, "N", m, n, False)
+
+
 @pytest.mark.parametrize("m,n", [(1, 1), (45, 70)])
 @pytest.mark.parametrize("side,uplo,trans,diag", VARIANTS_Z)
 def test_ztrmm2(gpu, blas, side, uplo, trans, diag, m, n):
