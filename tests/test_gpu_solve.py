@@ -49,8 +49,9 @@ def from_dev(t):
 
 # ------------------------------------------------------------------ FP32 BLAS
 
+# (640, 384, 256): whole 128 x 128 tiles and whole K steps, the unpredicated loads
 SGEMM_SHAPES = [(1, 1, 1), (16, 16, 4), (37, 29, 41), (64, 128, 16), (130, 70, 200),
-                (300, 257, 129), (1000, 900, 700)]
+                (300, 257, 129), (640, 384, 256), (1000, 900, 700)]
 
 
 @pytest.mark.parametrize("ta", "NT")
