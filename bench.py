@@ -113,7 +113,10 @@ def roofline_live(records):
     }
 
 
-def mixed_precision(torch, pkg, h, n, nrhs, alphas):
+MIXED_VARIANTS = (("S", "I"), ("B", "I"), ("X", "I"), ("H", "I"), ("B", "G"), ("H", "G"))
+
+
+def mixed_precision(torch, pkg, h, n, nrhs, alphas, variants=MIXED_VARIANTS):
     """SPD solve A X = B (n x n, nrhs right-hand sides): FP64 cuDpotrf +
     cuDpotrs, and the mixed-precision cuDsposvx with FP32, BF16, split-BF16 or
     FP16 trailing-update products (FP64 diagonal blocks, FP64 residuals)
@@ -148,7 +151,7 @@ def mixed_precision(torch, pkg, h, n, nrhs, alphas):
         rec["fp64_ms"] = round(t64 * 1e3, 2)
         rec["fp64_backward_error"] = berr(X)
         del F
-        for factor, refine in (("S", "I"), ("B", "I"), ("X", "I"), ("H", "I"), ("B", "G"), ("H", "G")):
+        for factor, refine in variants:
             X = torch.zeros_like(B)
             for rep in range(2):       # the first call also grows the workspace
                 torch.cuda.synchronize()
@@ -460,6 +463,9 @@ def main():
                     help="skip the batched mixed-precision solve + logdet + inverse (512 matrices, "
                          "n log-uniform in [128, 2048])")
     ap.add_argument("--no-blas", action="store_true", help="skip the standalone level-3 BLAS rates")
+    ap.add_argument("--mixed-alphas", default="1,1e-3,1e-6", help="alphas of the mixed-precision leg")
+    ap.add_argument("--mixed-variants", default=",".join(f + r for f, r in MIXED_VARIANTS),
+                    help="factor+refine pairs of the mixed-precision leg, e.g. HI,SI")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-pointer cuMultiGPUDpotrf leg (PCIe-inclusive)")
     ap.add_argument("--counter-calibration", action="store_true",
@@ -563,8 +569,9 @@ def main():
     if not args.no_mixed:
         del A
         torch.cuda.empty_cache()
-        extra["mixed_precision_solve"] = mixed_precision(torch, pkg, h, args.n_mixed, 16,
-                                                         (1.0, 1e-3, 1e-6))
+        alphas = tuple(float(a) for a in args.mixed_alphas.split(","))
+        variants = tuple((v[0], v[1]) for v in args.mixed_variants.split(","))
+        extra["mixed_precision_solve"] = mixed_precision(torch, pkg, h, args.n_mixed, 16, alphas, variants)
 
     if not args.no_batched:
         extra["batched_potrf_logdet"] = batched(torch, pkg, h, 20000, 8, 256, 5, 2000)
