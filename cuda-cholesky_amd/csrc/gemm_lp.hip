@@ -533,11 +533,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
 #pragma unroll
       for (int j = 0; j < BJ; j++) acc[i][j] *= aabs;
   }
-  if (!seed) {  // D = alpha acc + beta C: scale, then add C loaded in one batch
+  if (!seed) {  // D = alpha acc + beta C: scale (the sign is in the A fragments), then add C
 #pragma unroll
     for (int i = 0; i < BI; i++)
 #pragma unroll
-      for (int j = 0; j < BJ; j++) acc[i][j] *= p.alpha;
+      for (int j = 0; j < BJ; j++) acc[i][j] *= aabs;
     if (use_beta) {
       v4f prod[BI][BJ];
 #pragma unroll
