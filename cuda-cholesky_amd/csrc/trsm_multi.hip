@@ -238,6 +238,14 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
   int known = 0;  // positions below are ready and acquired; -1: waits abandoned
   if constexpr (KS) {
     const int h = wave >> 2, kq = wave & 3;
+    // B_i in the layout of the final reduction (element tid + 512 m), loaded
+    // now so that no global load sits between the last wait and the solve
+    T bk[16 * SB / CH_THREADS];
+#pragma unroll
+    for (int m = 0; m < 16 * SB / CH_THREADS; m++) {
+      const int e = tid + m * CH_THREADS, c = e / SB, rr = e % SB, col = c0 + c;
+      bk[m] = (rr < ni && col < nrhs) ? B[(long)col * ldb + ri + rr] : T(0);
+    }
     Acc16<float> a4[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) a4[t].zero();
@@ -290,12 +298,12 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
       for (int r = 0; r < 4; r++) P[lc * (SB + 1) + 64 * h + 16 * lg + 4 * r + t] = a4[t].v[r];
     __syncthreads();
     // Y = B_i - (sum of the four quarters, in a fixed order)
-    for (int e = tid; e < 16 * SB; e += CH_THREADS) {
-      const int c = e / SB, rr = e % SB, col = c0 + c;
-      const T b = (rr < ni && col < nrhs) ? B[(long)col * ldb + ri + rr] : T(0);
+#pragma unroll
+    for (int m = 0; m < 16 * SB / CH_THREADS; m++) {
+      const int e = tid + m * CH_THREADS, c = e / SB, rr = e % SB;
       const float *Q = reinterpret_cast<const float *>(sm.P) + c * (SB + 1) + rr;
       const float sum = ((Q[0] + Q[16 * (SB + 1)]) + Q[32 * (SB + 1)]) + Q[48 * (SB + 1)];
-      sm.Y[c * (SB + 1) + rr] = b - sum;
+      sm.Y[c * (SB + 1) + rr] = bk[m] - sum;
     }
   } else if (s > 0) {
     Slice<T> m, mn, xs;
