@@ -53,6 +53,23 @@ void release_graph(__cubatchedplan_st *p) {
   p->graph = nullptr;
 }
 
+// Frees everything a (possibly partly built) plan holds; every member is
+// null or valid.
+CUresult plan_release(__cubatchedplan_st *p) {
+  CUresult first = CUDA_SUCCESS;
+  auto keep = [&](hipError_t e) {
+    if (e != hipSuccess && first == CUDA_SUCCESS) first = hc::to_cu(e);
+  };
+  if (p->capture != nullptr) keep(hipStreamSynchronize(p->capture));
+  release_graph(p);
+  if (p->capture != nullptr) keep(hipStreamDestroy(p->capture));
+  if (p->d_n != nullptr) keep(hipFree(p->d_n));
+  if (p->d_off != nullptr) keep(hipFree(p->d_off));
+  if (p->d_order != nullptr) keep(hipFree(p->d_order));
+  std::free(p);
+  return first;
+}
+
 // One graph launch of the batch; the graph is re-captured when the buffers
 // or the operation change.
 CUresult run(const char *name, CUbatchedplan p, CUdeviceptr A, CUdeviceptr logdet, CUdeviceptr info,
@@ -70,10 +87,14 @@ CUresult run(const char *name, CUbatchedplan p, CUdeviceptr A, CUdeviceptr logde
     const hipError_t e = launch(p, p->capture, A, logdet, info, inverse);
     hipGraph_t graph = nullptr;
     const hipError_t e2 = hipStreamEndCapture(p->capture, &graph);
-    HC_CHECK(e);
-    HC_CHECK(e2);
+    hipError_t e3 = (e == hipSuccess) ? e2 : e;
+    if (e3 == hipSuccess) e3 = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {  // no half-built graph is kept
+      if (graph != nullptr) (void)hipGraphDestroy(graph);
+      p->exec = nullptr;
+      HC_CHECK(e3);
+    }
     p->graph = graph;
-    HC_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     p->gA = A;
     p->gLog = logdet;
     p->gInfo = info;
@@ -101,6 +122,8 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
     if (n[i] > (size_t)hc::kBatchedMaxN) return bad_arg(name, 5);
   hc::DeviceGuard g(h->device);
   HC_CHECK(g.status());
+  int cus = 0;
+  HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
   auto *p = static_cast<__cubatchedplan_st *>(std::calloc(1, sizeof(__cubatchedplan_st)));
   if (p == nullptr) return CUDA_ERROR_OUT_OF_MEMORY;
   p->device = h->device;
@@ -121,8 +144,6 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
   // workgroup that waits for a slot would start its share late); fewer per CU
   // when the autotuner measured that faster ($CUCHOLESKY_BATCHED_PER_CU
   // overrides both)
-  int cus = 0;
-  HC_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
   const char *env = std::getenv("CUCHOLESKY_BATCHED_PER_CU");
   const int forced = env != nullptr ? std::atoi(env) : 0;
   auto per_cu = [&](bool inverse) {
@@ -133,15 +154,20 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
   };
   p->grid = std::max(1, std::min<int>((int)count, cus * per_cu(false)));
   p->grid_inverse = std::max(1, std::min<int>((int)count, cus * per_cu(true)));
+  hipError_t e = hipSuccess;
   if (count > 0) {
-    HC_CHECK(hipMalloc(&p->d_n, count * sizeof(int)));
-    HC_CHECK(hipMalloc(&p->d_off, count * sizeof(long)));
-    HC_CHECK(hipMalloc(&p->d_order, count * sizeof(int)));
-    HC_CHECK(hipMemcpy(p->d_n, nv.data(), count * sizeof(int), hipMemcpyHostToDevice));
-    HC_CHECK(hipMemcpy(p->d_off, off.data(), count * sizeof(long), hipMemcpyHostToDevice));
-    HC_CHECK(hipMemcpy(p->d_order, order.data(), count * sizeof(int), hipMemcpyHostToDevice));
+    e = hipMalloc(&p->d_n, count * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_off, count * sizeof(long));
+    if (e == hipSuccess) e = hipMalloc(&p->d_order, count * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(p->d_n, nv.data(), count * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_off, off.data(), count * sizeof(long), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_order, order.data(), count * sizeof(int), hipMemcpyHostToDevice);
   }
-  HC_CHECK(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)plan_release(p);
+    HC_CHECK(e);
+  }
   *plan = p;
   return CUDA_SUCCESS;
 }
@@ -161,13 +187,7 @@ CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan p) {
   hc::DeviceGuard g(p->device);
   HC_CHECK(g.status());
   HC_CHECK(hipDeviceSynchronize());
-  release_graph(p);
-  if (p->capture != nullptr) HC_CHECK(hipStreamDestroy(p->capture));
-  if (p->d_n != nullptr) HC_CHECK(hipFree(p->d_n));
-  if (p->d_off != nullptr) HC_CHECK(hipFree(p->d_off));
-  if (p->d_order != nullptr) HC_CHECK(hipFree(p->d_order));
-  std::free(p);
-  return CUDA_SUCCESS;
+  return plan_release(p);
 }
 
 }  // extern "C"

@@ -19,6 +19,8 @@
 // (see gemm.hip).
 #include "internal.h"
 
+#include <cmath>
+
 namespace hc {
 namespace {
 
@@ -400,13 +402,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   const int lc = lane & 15, lg = lane >> 4;
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
 
-  // With |alpha| a power of two the accumulators start from (beta / |alpha|) C
-  // (loaded at the start, so the C read overlaps the whole K loop), the sign
-  // of alpha goes into the A fragments and the sums are scaled by |alpha| at
-  // the end: both scalings are exact.  Otherwise C is combined in the epilogue.
+  // |alpha| is a power of two (the launcher checks): the accumulators start
+  // from (beta / |alpha|) C (loaded at the start, so the C read overlaps the
+  // whole K loop), the sign of alpha goes into the A fragments and the sums
+  // are scaled by |alpha| at the end; both scalings are exact.
   const float aabs = fabsf(p.alpha);
-  const unsigned abits = __float_as_uint(aabs);
-  const bool seed = (abits & 0x007fffffu) == 0u && abits >= 0x00800000u && abits < 0x7f800000u;
   const bool use_beta = p.beta != 0.0f;
   v4f acc[BI][BJ];
   // C loads from clamped (valid) addresses with the condition outside the
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   };
   // issued after the first operand loads, consumed by the first products
   auto seed_acc = [&]() __attribute__((always_inline)) {
-    if (seed && use_beta) {
+    if (use_beta) {
       const float cs = p.beta / aabs;
       if (cs == 1.0f) load_c(true, 1.0f);
       else load_c(false, cs);
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   };
   const unsigned sgn = p.alpha < 0.0f ? 0x80008000u : 0u;  // bf16 sign bits of a fragment
 
-  const int kend = p.alpha == 0.0f ? 0 : p.k;
+  const int kend = p.k;
   if (kend > 0) {
     // chunk q of a thread: row (tid + q 256) / 8, k offset 8 ((tid + q 256) % 8).
     // Two register sets: the loads of K step kt + 2 are issued while step kt
@@ -527,29 +527,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
     seed_acc();
   }
 
-  if (seed && aabs != 1.0f) {
+  if (aabs != 1.0f) {
 #pragma unroll
     for (int i = 0; i < BI; i++)
 #pragma unroll
       for (int j = 0; j < BJ; j++) acc[i][j] *= aabs;
-  }
-  if (!seed) {  // D = alpha acc + beta C: scale (the sign is in the A fragments), then add C
-#pragma unroll
-    for (int i = 0; i < BI; i++)
-#pragma unroll
-      for (int j = 0; j < BJ; j++) acc[i][j] *= aabs;
-    if (use_beta) {
-      v4f prod[BI][BJ];
-#pragma unroll
-      for (int i = 0; i < BI; i++)
-#pragma unroll
-        for (int j = 0; j < BJ; j++) prod[i][j] = acc[i][j];
-      load_c(false, p.beta);
-#pragma unroll
-      for (int i = 0; i < BI; i++)
-#pragma unroll
-        for (int j = 0; j < BJ; j++) acc[i][j] += prod[i][j];
-    }
   }
 #pragma unroll
   for (int i = 0; i < BI; i++) {
@@ -709,6 +691,11 @@ hipError_t gemm_bf16_nt(hipStream_t s, long m, long n, long k, float alpha, cons
                         long ldd, int tri_out, bool tri_grid, const int *abort_flag, bool f16) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (k % BK != 0 || lda % 8 != 0 || ldb % 8 != 0) return hipErrorInvalidValue;
+  // |alpha| must be a normal power of two (the scaling of the seeded
+  // accumulators is then exact); every caller passes -2^-2e.
+  int e2 = 0;
+  const float mant = std::frexp(std::fabs(alpha), &e2);
+  if (!(mant == 0.5f) || !std::isnormal(alpha)) return hipErrorInvalidValue;
   Bf16Params p;
   p.m = (int)m; p.n = (int)n; p.k = (int)k;
   p.alpha = alpha; p.beta = beta;

@@ -29,6 +29,30 @@ CUresult cuBLASDestroy(CUBLAShandle handle) {
   return CUDA_SUCCESS;
 }
 
+// Releases whatever a (possibly partly built) LAPACK handle holds; every
+// member is null or valid.
+static CUresult lapack_release(__culapackhandle_st *h) {
+  CUresult first = CUDA_SUCCESS;
+  auto keep = [&](hipError_t e) {
+    if (e != hipSuccess && first == CUDA_SUCCESS) first = hc::to_cu(e);
+  };
+  if (h->stream != nullptr) keep(hipStreamSynchronize(h->stream));
+  if (h->side != nullptr) keep(hipStreamSynchronize(h->side));
+  if (h->ws != nullptr) keep(hipFree(h->ws));
+  for (long i = 0; i < 2 * h->tcap; i++) keep(hipEventDestroy(h->tev[i]));
+  std::free(h->tev);
+  std::free(h->tflops);
+  if (h->dinfo != nullptr) keep(hipFree(h->dinfo));
+  if (h->hinfo != nullptr) keep(hipHostFree(h->hinfo));
+  if (h->ev_panel != nullptr) keep(hipEventDestroy(h->ev_panel));
+  if (h->ev_update != nullptr) keep(hipEventDestroy(h->ev_update));
+  if (h->stream != nullptr) keep(hipStreamDestroy(h->stream));
+  if (h->side != nullptr) keep(hipStreamDestroy(h->side));
+  if (h->blas != nullptr) cuBLASDestroy(h->blas);
+  std::free(h);
+  return first;
+}
+
 CUresult cuLAPACKCreate(CULAPACKhandle *handle) {
   if (handle == nullptr) return hc::bad_arg("cuLAPACKCreate", 1);
   auto *h = static_cast<__culapackhandle_st *>(std::calloc(1, sizeof(__culapackhandle_st)));
@@ -40,13 +64,20 @@ CUresult cuLAPACKCreate(CULAPACKhandle *handle) {
   }
   h->device = h->blas->device;
   // Streams are created blocking (ordered after work on the null stream), at
-  // the default priority.
-  HC_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamDefault));
-  HC_CHECK(hipStreamCreateWithFlags(&h->side, hipStreamDefault));
-  HC_CHECK(hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming));
-  HC_CHECK(hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming));
-  HC_CHECK(hipMalloc(&h->dinfo, 64));
-  HC_CHECK(hipHostMalloc(&h->hinfo, 64, hipHostMallocDefault));
+  // the default priority.  On any failure everything created so far is
+  // released before the error is returned.
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->side, hipStreamDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&h->dinfo, 64);
+  if (e == hipSuccess) e = hipHostMalloc(&h->hinfo, 64, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    const CUresult err = hc::to_cu(e);
+    hc::report("cuLAPACKCreate", __func__, __FILE__, __LINE__, err);
+    (void)lapack_release(h);
+    return err;
+  }
   *handle = h;
   return CUDA_SUCCESS;
 }
@@ -54,21 +85,10 @@ CUresult cuLAPACKCreate(CULAPACKhandle *handle) {
 CUresult cuLAPACKDestroy(CULAPACKhandle handle) {
   if (handle == nullptr) return CUDA_SUCCESS;
   hc::DeviceGuard g(handle->device);
-  HC_CHECK(hipStreamSynchronize(handle->stream));
-  HC_CHECK(hipStreamSynchronize(handle->side));
-  if (handle->ws != nullptr) HC_CHECK(hipFree(handle->ws));
-  for (long i = 0; i < 2 * handle->tcap; i++) HC_CHECK(hipEventDestroy(handle->tev[i]));
-  std::free(handle->tev);
-  std::free(handle->tflops);
-  HC_CHECK(hipFree(handle->dinfo));
-  HC_CHECK(hipHostFree(handle->hinfo));
-  HC_CHECK(hipEventDestroy(handle->ev_panel));
-  HC_CHECK(hipEventDestroy(handle->ev_update));
-  HC_CHECK(hipStreamDestroy(handle->stream));
-  HC_CHECK(hipStreamDestroy(handle->side));
-  cuBLASDestroy(handle->blas);
-  std::free(handle);
-  return CUDA_SUCCESS;
+  HC_CHECK(g.status());
+  const CUresult r = lapack_release(handle);
+  if (r != CUDA_SUCCESS) hc::report("cuLAPACKDestroy", __func__, __FILE__, __LINE__, r);
+  return r;
 }
 
 // Timing of the trailing updates inside cuDpotrf / cuZpotrf (an extension

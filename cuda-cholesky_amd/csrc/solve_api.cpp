@@ -397,8 +397,9 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
 
   // Full FP64 factorisation and solve; A is overwritten by its factor.
   if (used != nullptr) *used = 'D';
+  // (the workspace of the attempts, sized at entry, still holds W, T, Y and
+  // the sync words used below)
   double *Aw = dptr<double>(A);
-  HC_CHECK(hc::lapack_workspace(h, L.total, &base));
   HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), s));
   HC_CHECK(hc::potrf_on_device<double>(h, ul, N, Aw, (long)lda,
                                        reinterpret_cast<double *>(wsb + L.W), false));
