@@ -252,6 +252,61 @@ def test_batched_inverse_then_factor_replays(gpu, lapack):
                         assert O.potrf_backward_error(mats[i], M, "L") <= 10 * n * EPS
 
 
+# ---------------------------------------------------------------- persistent loop
+
+@pytest.mark.parametrize("prec", "DS")
+@pytest.mark.parametrize("inverse", [False, True], ids=["potrf", "potri"])
+def test_batched_persistent_loop_reuse(gpu, lapack, monkeypatch, prec, inverse):
+    """One workgroup per CU (CUCHOLESKY_BATCHED_PER_CU=1, read at plan
+    creation) and more than two matrices per workgroup: every workgroup runs
+    the persistent loop several times, reusing its LDS tiles and failure flag
+    after a matrix that is not positive definite.  Factor or inverse, logdet
+    and info of every matrix against the oracle."""
+    monkeypatch.setenv("CUCHOLESKY_BATCHED_PER_CU", "1")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    count = 2 * cus + 97
+    sizes = [int(v) for v in np.random.default_rng(21).integers(0, 257, size=count)]
+    sizes[:6] = [0, 1, 256, 255, 16, 17]
+    buf, offs, mats = make_batch(8, sizes)
+    bad = {}
+    for i in range(10, count, count // 9):   # planted non-SPD matrices, every size class
+        n = sizes[i]
+        if n < 2:
+            continue
+        k = int(np.random.default_rng(i).integers(1, n))
+        A = mats[i]
+        L = np.linalg.cholesky(A)
+        A[k, k] = np.dot(L[k, :k], L[k, :k]) - 1.0
+        buf[offs[i]:offs[i] + n * n] = A.ravel(order="F")
+        bad[i] = k
+    eps = EPS if prec == "D" else EPS32
+    fn = run_inverse if inverse else run
+    out, logdet, info = fn(gpu, lapack, "L", prec, sizes, buf, offs)
+    for i, (o, n) in enumerate(zip(offs, sizes)):
+        if i in bad:
+            assert info[i] == bad[i] + 1 and np.isnan(logdet[i]), (i, n, info[i])
+            continue
+        assert info[i] == 0, (i, n, info[i])
+        if n == 0:
+            assert logdet[i] == 0.0
+            continue
+        A = mats[i] if prec == "D" else mats[i].astype(np.float32).astype(np.float64)
+        M = out[o:o + n * n].reshape((n, n), order="F")
+        F, rinfo = O.dpotrf("L", np.asfortranarray(A.copy()))
+        assert rinfo == 0
+        want = 2.0 * math.fsum(np.log(np.diag(F)))
+        assert abs(logdet[i] - want) <= 10 * n * eps * max(1.0, abs(want)), (i, n)
+        if inverse:
+            assert O.inverse_residual(A, M, "L") <= 10 * n * eps, (i, n)
+            ref, _ = O.dpotri("L", F.copy(order="F"))
+            Rf = O.herm_from(ref, "L")
+            assert np.linalg.norm(O.herm_from(M, "L") - Rf) <= 10 * n * eps * np.linalg.cond(A) * np.linalg.norm(Rf)
+        else:
+            assert O.potrf_backward_error(A, M, "L") <= 10 * n * eps, (i, n)
+            d = np.linalg.norm(np.tril(M) - np.tril(F)) / np.linalg.norm(np.tril(F))
+            assert d <= 100 * n * eps, (i, n, d)
+
+
 # ---------------------------------------------------------------- mixed-precision batched solve
 
 def test_batched_mixed_solve_inverse_logdet(gpu, lapack):
@@ -313,3 +368,111 @@ def test_batched_mixed_solve_not_spd(gpu, lapack):
     assert st == 0
     assert info[1] == k + 1 and np.isnan(ld[1])
     assert info[0] == 0 and info[2] == 0 and it[0] >= 0 and it[2] >= 0
+
+
+# ---------------------------------------------------------------- task-graph plan (cuDsposvxBatchedPlan*)
+
+def solve_plan_case(gpu, lapack, uplo, sizes, alphas, nrhs, seed, bad=None, reps=1):
+    """Run one plan `reps` times (graph replays, the inputs restored between
+    runs) and check every matrix against the oracle on its own input."""
+    count = len(sizes)
+    offA = [int(v) for v in np.cumsum([0] + [n * n + 5 for n in sizes[:-1]])]   # gaps between matrices
+    offB = [int(v) for v in np.cumsum([0] + [n * nrhs + 3 for n in sizes[:-1]])]
+    mats = [O.spd(700 + seed * 31 + i, n, a) if n > 0 else np.zeros((0, 0))
+            for i, (n, a) in enumerate(zip(sizes, alphas))]
+    for i, k in (bad or {}).items():
+        L = np.linalg.cholesky(mats[i])
+        mats[i][k, k] = np.dot(L[k, :k], L[k, :k]) - 1.0
+    lenA = offA[-1] + sizes[-1] ** 2 + 5
+    lenB = max(offB[-1] + sizes[-1] * nrhs + 3, 1)
+    bufA = np.full(lenA, np.nan)
+    for i, n in enumerate(sizes):
+        M = mats[i].copy()
+        mask = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+        M[mask] = np.nan                                  # the unreferenced triangle is never read
+        bufA[offA[i]:offA[i] + n * n] = M.ravel(order="F")
+    rng = np.random.default_rng(seed)
+    bufB = rng.standard_normal(lenB)
+    dA = torch.from_numpy(bufA).cuda()
+    dB = torch.from_numpy(bufB).cuda()
+    dX = torch.full_like(dB, -3.0)
+    dI = torch.full_like(dA, -5.0)
+    it = torch.full((count,), -99, dtype=torch.int32, device="cuda")
+    ld = torch.zeros(count, dtype=torch.float64, device="cuda")
+    info = torch.full((count,), -99, dtype=torch.int32, device="cuda")
+    with gpu.BatchedSolvePlan(lapack, ord(uplo), sizes, offA, nrhs, offB) as plan:
+        assert plan.status == 0
+        outs = []
+        for rep in range(reps):
+            dX.fill_(-3.0)
+            dI.fill_(-5.0)
+            torch.cuda.synchronize()
+            assert plan.run(gpu.ptr(dA), gpu.ptr(dB), gpu.ptr(dX), gpu.ptr(dI), gpu.ptr(it), gpu.ptr(ld),
+                            gpu.ptr(info)) == 0
+            torch.cuda.synchronize()
+            outs.append((dX.cpu().numpy().copy(), dI.cpu().numpy().copy(), it.cpu().numpy().copy(),
+                         ld.cpu().numpy().copy(), info.cpu().numpy().copy()))
+    np.testing.assert_array_equal(np.isnan(dA.cpu().numpy()), np.isnan(bufA))   # A is never written
+    for rep in range(1, reps):                            # replays are bit-identical
+        for a, b in zip(outs[0], outs[rep]):
+            np.testing.assert_array_equal(a, b)
+    X, Inv, its, lds, infos = outs[0]
+    for i, n in enumerate(sizes):
+        if bad and i in bad:
+            assert infos[i] == bad[i] + 1 and np.isnan(lds[i]), (i, infos[i])
+            continue
+        assert infos[i] == 0 and its[i] >= 0, (i, n, infos[i], its[i])
+        if n == 0:
+            assert lds[i] == 0.0
+            continue
+        A = mats[i]
+        F, rinfo = O.dpotrf("L", np.asfortranarray(A.copy()))
+        assert rinfo == 0
+        want = 2.0 * math.fsum(np.log(np.diag(F)))
+        assert abs(lds[i] - want) <= 10 * n * EPS * max(1.0, abs(want)), (i, n, lds[i], want)
+        Xi = np.asarray(Inv[offA[i]:offA[i] + n * n]).reshape((n, n), order="F")
+        assert O.inverse_residual(A, Xi, uplo) <= 10 * n * EPS, (i, n)
+        ref, _ = O.dpotri("L", F.copy(order="F"))
+        Rf = O.herm_from(ref, "L")
+        assert np.linalg.norm(O.herm_from(Xi, uplo) - Rf) <= 10 * n * EPS * np.linalg.cond(A) * np.linalg.norm(Rf)
+        # the other triangle of the inverse output is left alone
+        other = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+        assert np.all(Xi[other] == -5.0), i
+        if nrhs:
+            Bi = bufB[offB[i]:offB[i] + n * nrhs].reshape((n, nrhs), order="F")
+            Xs = X[offB[i]:offB[i] + n * nrhs].reshape((n, nrhs), order="F")
+            R = Bi - A @ Xs
+            anorm = np.abs(A).sum(axis=1).max()
+            # LAPACK DSPOSV's stopping test, column by column
+            assert np.all(np.abs(R).max(axis=0) <= np.abs(Xs).max(axis=0) * anorm * EPS * np.sqrt(n) * 1.01), (i, n)
+    return outs[0]
+
+
+@pytest.mark.parametrize("uplo", "LU")
+def test_solve_plan_uneven_sizes(gpu, lapack, uplo):
+    sizes = [1, 63, 64, 65, 130, 0, 257, 700, 2, 128]
+    alphas = [1.0, 1e-2, 1e-4, 1.0, 1e-3, 1.0, 1e-1, 1e-4, 1.0, 1e-2]
+    solve_plan_case(gpu, lapack, uplo, sizes, alphas, nrhs=4, seed=1)
+
+
+@pytest.mark.parametrize("nrhs", [0, 1, 7])
+def test_solve_plan_rhs_counts(gpu, lapack, nrhs):
+    """nrhs = 0 (inverse + logdet only), 1, and 7 (two groups of right-hand sides)."""
+    sizes = [100, 33, 300, 64]
+    solve_plan_case(gpu, lapack, "L", sizes, [1e-3, 1.0, 1e-2, 1e-4], nrhs=nrhs, seed=2 + nrhs)
+
+
+def test_solve_plan_replay_and_failures(gpu, lapack):
+    """Graph replays give identical results; a matrix that is not positive
+    definite reports its first failing column (tiles 0 and 2) and NaN logdet,
+    its neighbours are solved."""
+    sizes = [200, 150, 90, 400, 70]
+    bad = {1: 5, 3: 140}
+    solve_plan_case(gpu, lapack, "L", sizes, [1.0, 1.0, 1e-2, 1.0, 1e-3], nrhs=4, seed=5, bad=bad, reps=3)
+
+
+def test_solve_plan_many_small(gpu, lapack):
+    """More matrices than persistent workgroups: each workgroup runs many tasks."""
+    sizes = [int(v) for v in np.random.default_rng(11).integers(1, 200, size=700)]
+    alphas = list(np.exp(np.random.default_rng(12).uniform(np.log(1e-4), 0.0, size=700)))
+    solve_plan_case(gpu, lapack, "L", sizes, alphas, nrhs=2, seed=13)
