@@ -138,6 +138,11 @@ SIGNATURES = {
     "cuDsposvxBatched": [_h, _enum, _sz, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                          _ptr, _sz, ctypes.POINTER(ctypes.c_size_t), _ptr, _ptr, _ptr, _longp,
                          ctypes.POINTER(ctypes.c_double), _longp],
+    "cuDsposvxBatchedPlanCreate": [_h, _enum, _sz, ctypes.POINTER(ctypes.c_size_t),
+                                   ctypes.POINTER(ctypes.c_size_t), _sz, ctypes.POINTER(ctypes.c_size_t),
+                                   ctypes.POINTER(_h)],
+    "cuDsposvxBatchedRun": [_h, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _stream],
+    "cuDsposvxBatchedPlanDestroy": [_h],
     "cuPotrfBatchedPlanCreate": [_h, _enum, ctypes.c_char, _sz, ctypes.POINTER(ctypes.c_size_t),
                                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_h)],
     "cuPotrfBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
@@ -206,6 +211,27 @@ SIGNATURES = {
     "cuMultiGPUZpotri": [_h, _enum, _sz, _hp, _sz, _longp],
 }
 
+# host-memory routines of include/lapack.h and include/blas.h (host arrays;
+# void, except the log-determinants): name -> (restype, argument types)
+_f, _d = ctypes.c_float, ctypes.c_double
+_fp = ctypes.c_void_p        # host array
+_HOST_LAPACK = {p + r: [_enum, _sz, _fp, _sz, _longp] for p in "sdcz" for r in ("potrf", "lauum", "potri")}
+_HOST_LAPACK.update({p + "trtri": [_enum, _enum, _sz, _fp, _sz, _longp] for p in "sdcz"})
+_HOST_LAPACK.update({p + "trtri2": [_enum, _enum, _sz, _fp, _sz, _fp, _sz, _longp] for p in "sdcz"})
+_scal = {"s": _f, "d": _d, "c": _C, "z": _Z}
+_real = {"s": _f, "d": _d, "c": _f, "z": _d}
+HOST_SIGNATURES = {name: (None, args) for name, args in _HOST_LAPACK.items()}
+HOST_SIGNATURES.update({p + "logdet": (_real[p], [_fp, _sz, _sz]) for p in "sdcz"})
+for p in "sdcz":
+    herk = ("syrk" if p in "sd" else "herk")
+    HOST_SIGNATURES[p + herk] = (None, [_enum, _enum, _sz, _sz, _real[p], _fp, _sz, _real[p], _fp, _sz])
+    HOST_SIGNATURES[p + "gemm"] = (None, [_enum, _enum, _sz, _sz, _sz, _scal[p], _fp, _sz, _fp, _sz, _scal[p],
+                                          _fp, _sz])
+    for r in ("trmm", "trsm"):
+        HOST_SIGNATURES[p + r] = (None, [_enum, _enum, _enum, _enum, _sz, _sz, _scal[p], _fp, _sz, _fp, _sz])
+    HOST_SIGNATURES[p + "trmm2"] = (None, [_enum, _enum, _enum, _enum, _sz, _sz, _scal[p], _fp, _sz, _fp, _sz,
+                                           _fp, _sz])
+
 XERBLA_T = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_long)
 ERRHANDLER_T = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
@@ -226,6 +252,10 @@ def lib():
             fn.argtypes = args
             fn.restype = _res
         L.cuGetErrorString.restype = ctypes.c_char_p
+        for name, (res, args) in HOST_SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
         _lib = L
     return _lib
 
@@ -364,6 +394,35 @@ def cuDsposvxBatched(handle, uplo, n, offA, A, nrhs, offB, B, X, Ainv=0):
     st = lib().cuDsposvxBatched(_hv(handle), uplo, count, arr(n), arr(offA), A, nrhs, arr(offB), B, X,
                                 Ainv, it, ld, info)
     return st, list(it)[:count], list(ld)[:count], list(info)[:count]
+
+
+class BatchedSolvePlan:
+    """cuDsposvxBatchedPlanCreate / Run / Destroy: the task-graph solve +
+    logdet + inverse of packed matrices of orders n (host sequences n, offA,
+    offB); run() takes device addresses and int / double / int device arrays
+    for iter, logdet and info."""
+
+    def __init__(self, handle, uplo, n, offA, nrhs, offB):
+        count = len(n)
+        arr = lambda v: (ctypes.c_size_t * max(count, 1))(*[int(x) for x in v])
+        self.h = _h()
+        self.status = lib().cuDsposvxBatchedPlanCreate(_hv(handle), uplo, count, arr(n), arr(offA), nrhs,
+                                                       arr(offB if offB is not None else [0] * count),
+                                                       ctypes.byref(self.h))
+
+    def run(self, A, B, X, Ainv, iter_, logdet, info, stream=None):
+        return lib().cuDsposvxBatchedRun(self.h, A, B, X, Ainv, iter_, logdet, info, stream)
+
+    def close(self):
+        if self.h:
+            lib().cuDsposvxBatchedPlanDestroy(self.h)
+            self.h = _h()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class BatchedPlan:

@@ -114,19 +114,46 @@ CUresult cuPotriBatchedRun(CUbatchedplan plan, CUdeviceptr A, CUdeviceptr logdet
 CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan plan);
 
 /*
- * Batched mixed-precision solve + log-determinant + inverse of `count` SPD
- * matrices of any sizes: matrix i is n[i] x n[i] (column-major, ld n[i]) at
+ * Batched SPD solve + log-determinant + inverse of `count` matrices of any
+ * sizes, as one persistent kernel over 64 x 64 tiles of the whole batch.
+ * Matrix i is n[i] x n[i] (column-major, ld n[i]; triangle uplo is read) at
  * element offset offA[i] of A; its right-hand sides and solution are
  * n[i] x nrhs (ld n[i]) at element offset offB[i] of B and X.  Per matrix:
- *   X_i = A_i^-1 B_i by an FP32 factor with FP64 iterative refinement
- *   (cuDsposvx, factor 'S', refine 'I'; iter[i] its ITER code; A_i unchanged
- *   when iter[i] >= 0, its FP64 factor otherwise);
- *   logdet[i] = log det A_i from an FP64 factorisation of a copy of A_i;
- *   when Ainv != 0, triangle uplo of A_i^-1 at element offset offA[i] of Ainv;
- *   info[i] = 0 or the order of the leading minor that is not positive
- *   definite (logdet[i] is then NaN).
- * iter, logdet and info are host arrays of `count` elements.  Synchronous;
- * the matrices are shared out to worker threads with their own streams.
+ *   - the Cholesky factor and its inverse in FP64 (matrix cores), in plan
+ *     workspaces; A is never written;
+ *   - when Ainv != 0, triangle uplo of A_i^-1 at element offset offA[i] of
+ *     Ainv (LAPACK xPOTRI semantics);
+ *   - X_i = A_i^-1 B_i refined in FP64 with residuals of the original A_i
+ *     (r = b - A x, x += A^-1 r) until every column meets LAPACK DSPOSV's
+ *     test ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n), at most 30 steps;
+ *     iter[i] = steps taken (-31: no convergence);
+ *   - logdet[i] = log det A_i = sum_j log L_jj^2 (NaN on failure);
+ *   - info[i] = 0, or the order of the leading minor that is not positive
+ *     definite (then X_i and the inverse are not written), or -1 when the
+ *     kernel abandoned a dependency wait (a fault; nothing is valid).
+ *
+ * cuDsposvxBatchedPlanCreate takes the sizes and offsets on the host, builds
+ * and orders the task graph and allocates the workspaces (about three padded
+ * copies of the batch: 3 sum (64 ceil(n_i / 64))^2 doubles, plus n_i nrhs
+ * doubles per matrix).  cuDsposvxBatchedRun is asynchronous on `stream`:
+ * one HIP graph launch (captured on the first run with given buffers,
+ * replayed afterwards); iter and info are device int arrays, logdet a device
+ * double array, of `count` elements.  A plan runs one batch at a time: runs
+ * that overlap race.
+ */
+typedef struct __cubatchedsolveplan_st * CUbatchedsolveplan;
+CUresult cuDsposvxBatchedPlanCreate(CULAPACKhandle handle, CBlasUplo uplo, size_t count, const size_t * n,
+                                    const size_t * offA, size_t nrhs, const size_t * offB,
+                                    CUbatchedsolveplan * plan);
+CUresult cuDsposvxBatchedRun(CUbatchedsolveplan plan, CUdeviceptr A, CUdeviceptr B, CUdeviceptr X,
+                             CUdeviceptr Ainv, CUdeviceptr iter, CUdeviceptr logdet, CUdeviceptr info,
+                             CUstream stream);
+CUresult cuDsposvxBatchedPlanDestroy(CUbatchedsolveplan plan);
+
+/*
+ * Synchronous form: a plan for the batch, one run on the handle's stream,
+ * iter / logdet / info copied to the host arrays, the plan destroyed.
+ * Returns CUDA_ERROR_LAUNCH_FAILED if the kernel abandoned a dependency wait.
  */
 CUresult cuDsposvxBatched(CULAPACKhandle handle, CBlasUplo uplo, size_t count, const size_t * n,
                           const size_t * offA, CUdeviceptr A, size_t nrhs, const size_t * offB,
