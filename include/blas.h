@@ -8,9 +8,10 @@
  * CUdeviceptr is a device byte address.  All calls are asynchronous on the
  * given stream.
  *
- * The host-memory CPU routines of the reference header (dgemm, dsyrk, ...)
- * are not part of this library: the GPU path no longer factors diagonal
- * blocks on the host, so it has no CPU BLAS inside.
+ * The host-memory routines of the reference header (dgemm, dsyrk, ...,
+ * which the reference runs on the CPU) are exported too and run on the GPU:
+ * operands are staged through pinned host buffers, the device routine runs,
+ * the result comes back (synchronous).
  */
 #ifndef BLAS_H
 #define BLAS_H
@@ -43,6 +44,59 @@ typedef enum { CBlasNonUnit = 'N', CBlasUnit = 'U' } CBlasDiag;
  */
 typedef void (*xerbla_t)(const char *, long);
 extern xerbla_t xerbla;
+
+/*
+ * Host-memory BLAS-3 (replace the reference's CPU implementations,
+ * include/blas.h:45-150): column-major host arrays, synchronous, run on the
+ * current GPU.  Argument errors go to xerbla with the reference's indices
+ * (GEMM lda / ldb / ldc: 8 / 10 / 13; SYRK / HERK lda / ldc: 7 / 10, HERK
+ * trans = T: 2; TRMM / TRSM lda / ldb: 9 / 11, TRMM2 ldx: 13); enum values
+ * are checked first (index = position).  A device failure is reported
+ * through errorHandler.
+ */
+/* C <- alpha op(A) op(A)^T + beta C (replaces blas/ssyrk.c, dsyrk.c:34-161). */
+void ssyrk(CBlasUplo, CBlasTranspose, size_t, size_t, float, const float *, size_t, float, float *, size_t);
+void dsyrk(CBlasUplo, CBlasTranspose, size_t, size_t, double, const double *, size_t, double, double *, size_t);
+/* C <- alpha op(A) op(A)^H + beta C, real alpha and beta (replaces blas/cherk.c, zherk.c:35). */
+void cherk(CBlasUplo, CBlasTranspose, size_t, size_t, float, const cuComplex_t *, size_t, float, cuComplex_t *, size_t);
+void zherk(CBlasUplo, CBlasTranspose, size_t, size_t, double, const cuDoubleComplex_t *, size_t, double,
+           cuDoubleComplex_t *, size_t);
+/* C <- alpha op(A) op(B) + beta C (replaces blas/sgemm.c, dgemm.c:34-146, cgemm.c, zgemm.c:34). */
+void sgemm(CBlasTranspose, CBlasTranspose, size_t, size_t, size_t, float, const float *, size_t, const float *,
+           size_t, float, float *, size_t);
+void dgemm(CBlasTranspose, CBlasTranspose, size_t, size_t, size_t, double, const double *, size_t,
+           const double *, size_t, double, double *, size_t);
+void cgemm(CBlasTranspose, CBlasTranspose, size_t, size_t, size_t, cuComplex_t, const cuComplex_t *, size_t,
+           const cuComplex_t *, size_t, cuComplex_t, cuComplex_t *, size_t);
+void zgemm(CBlasTranspose, CBlasTranspose, size_t, size_t, size_t, cuDoubleComplex_t, const cuDoubleComplex_t *,
+           size_t, const cuDoubleComplex_t *, size_t, cuDoubleComplex_t, cuDoubleComplex_t *, size_t);
+/* B <- alpha op(A) B or alpha B op(A), in place (replaces blas/strmm.c, dtrmm.c:44-204, ctrmm.c, ztrmm.c:44). */
+void strmm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, float, const float *, size_t, float *,
+           size_t);
+void dtrmm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, double, const double *, size_t,
+           double *, size_t);
+void ctrmm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, cuComplex_t, const cuComplex_t *,
+           size_t, cuComplex_t *, size_t);
+void ztrmm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, cuDoubleComplex_t,
+           const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *, size_t);
+/* X <- alpha op(A) B or alpha B op(A), B unchanged (replaces blas/dtrmm.c:206-372 and twins). */
+void strmm2(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, float, const float *, size_t,
+            const float *, size_t, float *, size_t);
+void dtrmm2(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, double, const double *, size_t,
+            const double *, size_t, double *, size_t);
+void ctrmm2(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, cuComplex_t, const cuComplex_t *,
+            size_t, const cuComplex_t *, size_t, cuComplex_t *, size_t);
+void ztrmm2(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, cuDoubleComplex_t,
+            const cuDoubleComplex_t *, size_t, const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *, size_t);
+/* B <- alpha op(A)^-1 B or alpha B op(A)^-1 (replaces blas/strsm.c, dtrsm.c:34-216, ctrsm.c, ztrsm.c:34). */
+void strsm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, float, const float *, size_t, float *,
+           size_t);
+void dtrsm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, double, const double *, size_t,
+           double *, size_t);
+void ctrsm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, cuComplex_t, const cuComplex_t *,
+           size_t, cuComplex_t *, size_t);
+void ztrsm(CBlasSide, CBlasUplo, CBlasTranspose, CBlasDiag, size_t, size_t, cuDoubleComplex_t,
+           const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *, size_t);
 
 /** Handle: binds a device and its kernels (reference include/blas.h:153-155). */
 typedef struct __cublashandle_st * CUBLAShandle;

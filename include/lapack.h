@@ -11,8 +11,10 @@
  * CUDA_ERROR_INVALID_VALUE.
  *
  * Unlike the reference, no diagonal block travels to the host: every step runs
- * on the GPU, so the host CPU routines of the reference header are not part of
- * this library.
+ * on the GPU.  The host-memory routines of the reference header (dpotrf,
+ * dtrtri, ... without a prefix, which the reference runs on the CPU) are
+ * exported too and run on the GPU: operands are staged through pinned host
+ * buffers to the device, the device routine runs, the result comes back.
  */
 #ifndef LAPACK_H
 #define LAPACK_H
@@ -22,6 +24,46 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/*
+ * Host-memory routines (replace the reference's CPU implementations,
+ * include/lapack.h:17-60): column-major host arrays, synchronous, run on the
+ * current GPU.  Argument errors: *info = -(position) and xerbla, as the
+ * reference (lda: -4; dtrtri lda: -5; dtrtri2 lda / ldb: -5 / -7).  A
+ * numerical failure is info > 0.  A device failure is reported through
+ * errorHandler and sets *info = -1000 - CUresult.
+ */
+/* Cholesky factorisation in place (replaces lapack/spotrf.c, dpotrf.c:34-146, cpotrf.c, zpotrf.c:36-148). */
+void spotrf(CBlasUplo, size_t, float *, size_t, long *);
+void dpotrf(CBlasUplo, size_t, double *, size_t, long *);
+void cpotrf(CBlasUplo, size_t, cuComplex_t *, size_t, long *);
+void zpotrf(CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+/* Triangular inverse in place (replaces lapack/dtrtri.c:98-164 and twins). */
+void strtri(CBlasUplo, CBlasDiag, size_t, float *, size_t, long *);
+void dtrtri(CBlasUplo, CBlasDiag, size_t, double *, size_t, long *);
+void ctrtri(CBlasUplo, CBlasDiag, size_t, cuComplex_t *, size_t, long *);
+void ztrtri(CBlasUplo, CBlasDiag, size_t, cuDoubleComplex_t *, size_t, long *);
+/* Triangular inverse out of place, B <- inv(A) on triangle uplo (replaces lapack/dtrtri.c:226-299 and twins). */
+void strtri2(CBlasUplo, CBlasDiag, size_t, const float *, size_t, float *, size_t, long *);
+void dtrtri2(CBlasUplo, CBlasDiag, size_t, const double *, size_t, double *, size_t, long *);
+void ctrtri2(CBlasUplo, CBlasDiag, size_t, const cuComplex_t *, size_t, cuComplex_t *, size_t, long *);
+void ztrtri2(CBlasUplo, CBlasDiag, size_t, const cuDoubleComplex_t *, size_t, cuDoubleComplex_t *, size_t, long *);
+/* U U^H / L^H L in place (replaces lapack/dlauum.c:71-129 and twins). */
+void slauum(CBlasUplo, size_t, float *, size_t, long *);
+void dlauum(CBlasUplo, size_t, double *, size_t, long *);
+void clauum(CBlasUplo, size_t, cuComplex_t *, size_t, long *);
+void zlauum(CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+/* Inverse from the Cholesky factor (replaces lapack/dpotri.c:4-23 and twins). */
+void spotri(CBlasUplo, size_t, float *, size_t, long *);
+void dpotri(CBlasUplo, size_t, double *, size_t, long *);
+void cpotri(CBlasUplo, size_t, cuComplex_t *, size_t, long *);
+void zpotri(CBlasUplo, size_t, cuDoubleComplex_t *, size_t, long *);
+/* 2 sum log(Re x[i incx]), logs in FP64 (replaces lapack/dlogdet.c:10-25,
+ * slogdet.c, clogdet.c, zlogdet.c:11-26, include/lapack.h:161-164). */
+float slogdet(const float *, size_t, size_t);
+double dlogdet(const double *, size_t, size_t);
+float clogdet(const cuComplex_t *, size_t, size_t);
+double zlogdet(const cuDoubleComplex_t *, size_t, size_t);
 
 /** Handle (reference include/lapack.h:63-65). */
 typedef struct __culapackhandle_st * CULAPACKhandle;

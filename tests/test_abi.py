@@ -26,7 +26,8 @@ def declared_functions():
         text = open(os.path.join(INCLUDE, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
-        for m in re.finditer(r"^\s*(?:CUresult|const char \*)\s*(\w+)\s*\(", text, flags=re.M):
+        for m in re.finditer(r"^\s*(?:CUresult|const char \*|void|float|double)\s+(\w+)\s*\(", text,
+                             flags=re.M):
             names.add(m.group(1))
     return names
 
@@ -37,7 +38,7 @@ def test_library_exports_every_declared_symbol(cc):
     assert len(names) >= 25
     for name in sorted(names):
         assert hasattr(L, name), f"{name} declared in include/ but not exported"
-        assert name in cc.SIGNATURES, f"{name} not bound by the Python mirror"
+        assert name in cc.SIGNATURES or name in cc.HOST_SIGNATURES, f"{name} not bound by the Python mirror"
     for var in ("xerbla", "errorHandler"):
         assert ctypes.c_void_p.in_dll(L, var).value is not None
 
@@ -304,3 +305,89 @@ def test_single_and_single_complex_argument_indices(cc):
     assert cc.cuCpotrf(FAKE, L, 0, 0, 1) == (0, 0)
     assert cc.cuSlogdet(FAKE, 0, 1, 0) == (0, 0.0)
     assert cc.cuClogdet(FAKE, 0, 1, 0) == (0, 0.0)
+
+
+# ---------------------------------------------------------------- host-memory routines
+
+def _host(cc, name):
+    return getattr(cc.lib(), name)
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_host_lapack_argument_errors(cc, p):
+    """The host-memory LAPACK routines report lda < n as the reference's CPU
+    routines do (lapack/dpotrf.c:99-107: info = -4; dtrtri.c:101-104: -5;
+    dtrtri2 lda / ldb: -5 / -7, the ldb test last) and an unknown uplo as
+    argument 1, before any device work."""
+    import ctypes as C
+    info = C.c_long(7)
+    for r in ("potrf", "lauum", "potri"):
+        with cc.Xerbla() as x:
+            _host(cc, p + r)(cc.Lower, 10, None, 9, C.byref(info))
+        assert info.value == -4 and x.calls == [(p + r, 4)]
+        with cc.Xerbla() as x:
+            _host(cc, p + r)(ord("X"), 10, None, 10, C.byref(info))
+        assert info.value == -1 and x.calls == [(p + r, 1)]
+        info.value = 7
+        _host(cc, p + r)(cc.Lower, 0, None, 1, C.byref(info))   # n = 0: quick return
+        assert info.value == 0
+    with cc.Xerbla() as x:
+        _host(cc, p + "trtri")(cc.Upper, cc.NonUnit, 10, None, 3, C.byref(info))
+    assert info.value == -5 and x.calls == [(p + "trtri", 5)]
+    with cc.Xerbla() as x:
+        _host(cc, p + "trtri")(cc.Upper, ord("Q"), 10, None, 10, C.byref(info))
+    assert info.value == -2 and x.calls == [(p + "trtri", 2)]
+    for lda, ldb, want in ((9, 10, -5), (10, 9, -7), (9, 9, -7)):
+        with cc.Xerbla() as x:
+            _host(cc, p + "trtri2")(cc.Lower, cc.NonUnit, 10, None, lda, None, ldb, C.byref(info))
+        assert info.value == want and x.calls == [(p + "trtri2", -want)]
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_host_blas_argument_errors(cc, p):
+    """Host BLAS-3: the reference CPU indices (blas/dgemm.c:41-51 lda / ldb /
+    ldc -> 8 / 10 / 13; blas/dsyrk.c:40-45 lda / ldc -> 7 / 10, zherk.c:42
+    trans = T -> 2; blas/dtrmm.c:50-55, dtrsm.c:40-45 lda / ldb -> 9 / 11,
+    dtrmm.c:213-219 ldx -> 13) and no device work on quick returns."""
+    one = {"s": 1.0, "d": 1.0, "c": cc.cscalar(1.0), "z": cc.zscalar(1.0)}[p]
+    zero = {"s": 0.0, "d": 0.0, "c": cc.cscalar(0.0), "z": cc.zscalar(0.0)}[p]
+    gemm = _host(cc, p + "gemm")
+    for (m, n, k, lda, ldb, ldc), idx in (((5, 4, 3, 4, 3, 5), 8), ((5, 4, 3, 5, 2, 5), 10),
+                                          ((5, 4, 3, 5, 3, 4), 13)):
+        with cc.Xerbla() as x:
+            gemm(cc.NoTrans, cc.NoTrans, m, n, k, one, None, lda, None, ldb, one, None, ldc)
+        assert x.calls == [(p + "gemm", idx)]
+    with cc.Xerbla() as x:
+        gemm(ord("X"), cc.NoTrans, 5, 4, 3, one, None, 5, None, 3, one, None, 5)
+    assert x.calls == [(p + "gemm", 1)]
+    with cc.Xerbla() as x:  # quick returns read nothing (null arrays)
+        gemm(cc.NoTrans, cc.NoTrans, 0, 4, 3, one, None, 1, None, 3, one, None, 1)
+        gemm(cc.NoTrans, cc.NoTrans, 4, 4, 3, zero, None, 4, None, 3, one, None, 4)
+    assert x.calls == []
+    herk = _host(cc, p + ("syrk" if p in "sd" else "herk"))
+    name = p + ("syrk" if p in "sd" else "herk")
+    for (n, k, lda, ldc), idx in (((5, 3, 4, 5), 7), ((5, 3, 5, 4), 10)):
+        with cc.Xerbla() as x:
+            herk(cc.Lower, cc.NoTrans, n, k, 1.0, None, lda, 1.0, None, ldc)
+        assert x.calls == [(name, idx)]
+    if p in "cz":
+        with cc.Xerbla() as x:
+            herk(cc.Lower, cc.Trans, 5, 3, 1.0, None, 5, 1.0, None, 5)
+        assert x.calls == [(name, 2)]
+    for r in ("trmm", "trsm"):
+        f = _host(cc, p + r)
+        for (m, n, lda, ldb), idx in (((5, 4, 4, 5), 9), ((5, 4, 5, 4), 11)):
+            with cc.Xerbla() as x:
+                f(cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, m, n, one, None, lda, None, ldb)
+            assert x.calls == [(p + r, idx)]
+        with cc.Xerbla() as x:
+            f(ord("Z"), cc.Lower, cc.NoTrans, cc.NonUnit, 5, 4, one, None, 5, None, 5)
+        assert x.calls == [(p + r, 1)]
+    with cc.Xerbla() as x:
+        _host(cc, p + "trmm2")(cc.Left, cc.Lower, cc.NoTrans, cc.NonUnit, 5, 4, one, None, 5, None, 5, None, 4)
+    assert x.calls == [(p + "trmm2", 13)]
+
+
+def test_host_logdet_empty(cc):
+    for p in "sdcz":
+        assert _host(cc, p + "logdet")(None, 1, 0) == 0.0
