@@ -78,6 +78,16 @@ __device__ void wait_geq(const int *c, int target, int *err) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// *p read once for the whole workgroup (thread 0, broadcast through LDS):
+// another workgroup may change it meanwhile, and threads that disagreed
+// would take different sides of a branch that holds barriers.
+__device__ int read_uniform(const int *p, int *s_word) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_word = ld_acq(p);
+  __syncthreads();
+  return *s_word;
+}
+
 // Publish everything this workgroup wrote, then add v to each counter.
 __device__ void signal(int *c0, int *c1) {
   __syncthreads();
@@ -435,11 +445,12 @@ __device__ void block_max5(double (&v)[5], double *red) {
   for (int g = 0; g < 5; g++) v[g] = red[g * NTHR];
 }
 
-__device__ __attribute__((noinline)) void solve_task(const DagParams &p, const DagMat &mt, int m, double *smem) {
+__device__ __attribute__((noinline)) void solve_task(const DagParams &p, const DagMat &mt, int m, double *smem,
+                                                      int *s_word) {
   const int tid = threadIdx.x, n = mt.n, T = mt.T;
   const long P = (long)TB * T;
   int *cnt = p.cnt + mt.cbase;
-  const int failed = ld_acq(cnt + 3 * T + 2);
+  const int failed = read_uniform(cnt + 3 * T + 2, s_word);
   int iter = 0;
   double logdet = 0.0;
   if (failed == 0 && n > 0) {
@@ -548,7 +559,7 @@ __device__ __attribute__((noinline)) void task_diag(const DagParams &p, const Da
   const int tid = threadIdx.x, j = c.j, n = c.n, T = c.T;
   const long P = c.P;
   if (j > 0) wait_geq(c.col + j - 1, T - j + 1, p.err);
-  if (ld_acq(c.failed) == 0) {
+  if (read_uniform(c.failed, s_flag) == 0) {
     load_a_tile<UPPER>(Pt, c.A, n, j, j, true);
     __syncthreads();
     store_tile(c.Af + (long)TB * j * P + TB * j, P, Pt, false, false);
@@ -576,12 +587,13 @@ __device__ __attribute__((noinline)) void task_diag(const DagParams &p, const Da
 }
 
 template <bool UPPER>
-__device__ __attribute__((noinline)) void task_panel(const DagParams &p, const Ctx &c, double *smem) {
+__device__ __attribute__((noinline)) void task_panel(const DagParams &p, const Ctx &c, double *smem,
+                                                      int *s_word) {
   double *Pt = smem, *Qt = smem + TILE_D;
   const int i = c.i, j = c.j, T = c.T;
   const long P = c.P;
   if (j > 0) wait_geq(c.col + j - 1, T - j + 1, p.err);
-  const bool skip0 = ld_acq(c.failed) != 0;
+  const bool skip0 = read_uniform(c.failed, s_word) != 0;
   Acc a;
   if (!skip0) {
     load_a_tile<UPPER>(Pt, c.A, c.n, i, j, false);
@@ -593,7 +605,7 @@ __device__ __attribute__((noinline)) void task_panel(const DagParams &p, const C
     tile_gemm<0, 0>(a, c.F + TB * i, P, c.F + TB * j, P, TB * j, -1.0, Qt);
   }
   wait_geq(c.diag + j, 1, p.err);
-  if (!skip0 && ld_acq(c.failed) == 0) {
+  if (!skip0 && read_uniform(c.failed, s_word) == 0) {
     acc_to_lds(a, Pt);                                       // C(r, k) at [k LD + r]
     load_tile(Qt, c.Xw + (long)TB * j * P + TB * j, P);      // X_jj(c, k) at [k LD + c]
     __syncthreads();
@@ -607,13 +619,14 @@ __device__ __attribute__((noinline)) void task_panel(const DagParams &p, const C
   signal(c.col + j, nullptr);
 }
 
-__device__ __attribute__((noinline)) void task_trtri(const DagParams &p, const Ctx &c, double *smem) {
+__device__ __attribute__((noinline)) void task_trtri(const DagParams &p, const Ctx &c, double *smem,
+                                                      int *s_word) {
   double *Pt = smem, *Qt = smem + TILE_D;
   const int i = c.i, j = c.j;
   const long P = c.P;
   wait_geq(c.diag + i, 1, p.err);
   if (i - j - 1 > 0) wait_geq(c.trow + j, i - j - 1, p.err);
-  if (ld_acq(c.failed) == 0) {
+  if (read_uniform(c.failed, s_word) == 0) {
     Acc a;
     acc_zero(a);
     tile_gemm<0, 1>(a, c.F + (long)TB * j * P + TB * i, P, c.Xw + (long)TB * j * P + TB * j, P, TB * (i - j),
@@ -633,13 +646,13 @@ __device__ __attribute__((noinline)) void task_trtri(const DagParams &p, const C
 
 template <bool UPPER>
 __device__ __attribute__((noinline)) void task_lauum(const DagParams &p, const DagMat &mt, const Ctx &c,
-                                                      double *smem) {
+                                                      double *smem, int *s_word) {
   double *Pt = smem, *Qt = smem + TILE_D;
   const int i = c.i, j = c.j, T = c.T;
   const long P = c.P;
   wait_geq(c.ttot, T * (T - 1) / 2, p.err);
   wait_geq(c.diag + T - 1, 1, p.err);
-  if (ld_acq(c.failed) == 0) {
+  if (read_uniform(c.failed, s_word) == 0) {
     Acc a;
     acc_zero(a);
     tile_gemm<1, 1>(a, c.Xw + (long)TB * i * P + TB * i, P, c.Xw + (long)TB * j * P + TB * i, P,
@@ -685,12 +698,12 @@ __global__ void __launch_bounds__(NTHR, 2) dag_kernel(DagParams p) {
     c.A = p.A + mt.offA;
     switch (tk.type) {
       case kTaskDiag: task_diag<UPPER>(p, mt, c, smem, &s_flag); break;
-      case kTaskPanel: task_panel<UPPER>(p, c, smem); break;
-      case kTaskTrtri: task_trtri(p, c, smem); break;
-      case kTaskLauum: task_lauum<UPPER>(p, mt, c, smem); break;
+      case kTaskPanel: task_panel<UPPER>(p, c, smem, &s_flag); break;
+      case kTaskTrtri: task_trtri(p, c, smem, &s_flag); break;
+      case kTaskLauum: task_lauum<UPPER>(p, mt, c, smem, &s_flag); break;
       default:  // kTaskSolve
         wait_geq(c.ltot, mt.T * (mt.T + 1) / 2, p.err);
-        solve_task(p, mt, tk.m, smem);
+        solve_task(p, mt, tk.m, smem, &s_flag);
         __syncthreads();
         break;
     }
