@@ -58,12 +58,49 @@ struct View {
   }
 };
 
+// Wave 0: factor the leaf at g0 whose rows are in a[] (lane r + 16 q holds
+// row r), write L_pp to the matrix, accumulate log L_rr^2 of the valid rows
+// into logpart and put inv(L_pp) into Li; returns the failing pivot's column
+// (g0 + k) or -1.
+template <typename ST, bool LOWER>
+__device__ __forceinline__ int leaf_step(double (&a)[LEAF], const View<ST, LOWER> &v, int g0, double *Li,
+                                         double &logpart) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  double unused = 0.0;
+  const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
+  if (b >= 0) return g0 + b;
+  // log of this lane's own pivot L_rr^2, all rows at once (off the pivot
+  // chain); summed over the 16 rows once per matrix
+  double dg = a[0];
+  t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(r == decltype(C)::value, a[decltype(C)::value], dg); });
+  if (q == 0 && g0 + r < v.n) logpart += 2.0 * log(dg);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {  // lane group q writes columns q + 4 j of its row
+    // a value select (a conditional over the array elements themselves
+    // would select their addresses and keep the array in memory)
+    const double f = t16::sel(q == 0, a[4 * j], t16::sel(q == 1, a[4 * j + 1], t16::sel(q == 2, a[4 * j + 2], a[4 * j + 3])));
+    v.put(g0 + r, g0 + q + 4 * j, f);
+  }
+  double x[4];
+  t16::leaf_inverse_cols<double>(a, x);
+#pragma unroll
+  for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
+  return -1;
+}
+
+// The factor-only kernel is held to three waves per SIMD (168 registers):
+// more matrices in flight per CU hide the per-matrix pivot chain.
+template <bool INVERSE>
+constexpr int batched_waves_per_eu() { return INVERSE ? 2 : 3; }
+
 template <typename ST, bool LOWER, bool INVERSE>
 __global__ void __launch_bounds__(BT_THREADS)
+__attribute__((amdgpu_waves_per_eu(batched_waves_per_eu<INVERSE>(), 8)))
 potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *order, ST *A,
                      double *logdet, int *info) {
   __shared__ double Li[LEAF * LL];                 // inv(L_pp)
   __shared__ double P[BT_NPMAX * LEAF * LL];       // panel tiles L_Ip
+  __shared__ double Dg[LEAF * LL];                 // wave 0: the next diagonal tile, layout change
   __shared__ int flag;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
@@ -74,39 +111,17 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
     const int np = (n + LEAF - 1) / LEAF;
     double logpart = 0.0;  // wave 0, lanes 0..15: sum of log(L_rr^2) of row r of every leaf
     int bad = -1;
-    for (int p = 0; p < np; p++) {
+    // Leaf p + 1 is factored by wave 0 during step p's trailing update
+    // (look-ahead), from the tile (p + 1, p + 1) it updates itself; step
+    // p = -1 only factors leaf 0 (one call site keeps the leaf inlined once).
+    for (int p = -1; p < np; p++) {
       const int g0 = p * LEAF;
-      if (wave == 0) {
-        const int r = li, q = lk;
-        double a[LEAF];  // row r of the leaf (row layout, t16::leaf_factor_rows)
-        t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = v.get(g0 + r, g0 + decltype(C)::value); });
-        double unused = 0.0;
-        const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
-        if (b < 0) {
-          // log of this lane's own pivot L_rr^2, all rows at once (off the
-          // pivot chain); summed over the 16 rows once per matrix
-          double dg = a[0];
-          t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(r == decltype(C)::value, a[decltype(C)::value], dg); });
-          if (q == 0 && g0 + r < n) logpart += 2.0 * log(dg);
-#pragma unroll
-          for (int j = 0; j < 4; j++) {  // lane group q writes columns q + 4 j of its row
-            // a value select (a conditional over the array elements themselves
-            // would select their addresses and keep the array in memory)
-            const double f = t16::sel(q == 0, a[4 * j], t16::sel(q == 1, a[4 * j + 1], t16::sel(q == 2, a[4 * j + 2], a[4 * j + 3])));
-            v.put(g0 + r, g0 + q + 4 * j, f);
-          }
-          double x[4];
-          t16::leaf_inverse_cols<double>(a, x);
-#pragma unroll
-          for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
-        }
-        if (lane == 0) flag = b < 0 ? -1 : g0 + b;
+      if (p >= 0) {
+        bad = flag;
+        if (bad >= 0) break;
       }
-      __syncthreads();
-      bad = flag;
-      if (bad >= 0) break;
       // panel: L_Ip = A_Ip inv(L_pp)^T, to the matrix and to LDS
-      for (int I = p + 1 + wave; I < np; I += BT_NW) {
+      for (int I = p + 1 + wave; p >= 0 && I < np; I += BT_NW) {
         TileAcc<double> acc;
         t16::acc_zero(acc);
 #pragma unroll
@@ -124,25 +139,47 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
         }
       }
       __syncthreads();
-      // trailing update of the lower tiles, operands from LDS.  Tiles are
-      // dealt round-robin to the waves; each wave loads TG of its tiles
-      // before computing any, so their global-memory latencies overlap.
-      {
-        const int m = np - 1 - p;               // trailing tile rows / columns
-        const int ntile = m * (m + 1) / 2;
-        for (int t0 = wave * TG; t0 < ntile; t0 += BT_NW * TG) {
+      const int m = np - 1 - p;  // trailing tile rows / columns
+      if (m > 0 && wave == 0) {
+        // look-ahead: tile (p + 1, p + 1) updated in registers, then leaf p + 1
+        const int D = p + 1;
+        TileAcc<double> acc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc.re[r] = v.get(D * LEAF + lk + 4 * r, D * LEAF + li);
+        if (p >= 0) {
+          const double *PD = P + D * LEAF * LL;
+#pragma unroll
+          for (int k0 = 0; k0 < LEAF; k0 += 4) {
+            const double x = -PD[(k0 + lk) * LL + li];
+            const double y = PD[(k0 + lk) * LL + li];
+            acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc.re, 0, 0, 0);
+          }
+        }
+        t16::acc_store(acc, Dg, LL);  // element (row, col) at col LL + row
+        t16::wave_lds_fence();
+        double a[LEAF];
+        t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
+        const int b = leaf_step<ST, LOWER>(a, v, D * LEAF, Li, logpart);
+        if (lane == 0) flag = b;
+      } else if (m > 0 && p >= 0) {
+        // the other trailing tiles (I, J), p < J <= I, without (p+1, p+1),
+        // dealt round-robin to waves 1..3; each wave loads TG of its tiles
+        // before computing any, so their global-memory latencies overlap
+        const int ntile = m * (m + 1) / 2 - 1;
+        constexpr int NWT = BT_NW - 1;
+        for (int t0 = (wave - 1) * TG; t0 < ntile; t0 += NWT * TG) {
           int ti[TG], tj[TG];
           TileAcc<double> acc[TG];
 #pragma unroll
           for (int g = 0; g < TG; g++) {
-            // tile t0 + g of the column-major lower triangle of the m x m tile block
+            // tile t0 + g + 1 of the column-major lower triangle of the m x m tile block
             const bool live = t0 + g < ntile;
-            int t = live ? t0 + g : 0, J = 0;  // t < m (m >= 1 here): the walk ends
-            while (t >= m - J) {
-              t -= m - J;
+            int tt = live ? t0 + g + 1 : 1, J = 0;  // tt < m (m + 1) / 2: the walk ends
+            while (tt >= m - J) {
+              tt -= m - J;
               J++;
             }
-            ti[g] = p + 1 + J + t;
+            ti[g] = p + 1 + J + tt;
             tj[g] = p + 1 + J;
 #pragma unroll
             for (int r = 0; r < 4; r++)
