@@ -275,6 +275,7 @@ def cpu_baseline(torch, n_cpu):
         "value": round(tflops, 6),
         "unit": "TFLOP/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": (f"oracle dpotrf (the reference CPU algorithm restated: nb=32 blocked, OpenMP "
                    f"column loops, gcc -O2) lower n={n_cpu} FP64, A = G G^T/n + 0.1 I, one run "
@@ -371,16 +372,18 @@ def host_api(torch, pkg, A0, n, reps=2):
     return out
 
 
-def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs):
-    """Batched mixed-precision solve + logdet + inverse (cuDsposvxBatched) of
-    `count` FP64 SPD matrices A_i = G_i G_i^T / n_i + alpha_i I with n_i
-    log-uniform in [nmin, nmax] and alpha_i log-uniform in [1e-4, 1], `nrhs`
-    N(0,1) right-hand sides each.  Wall time of the (synchronous) call; per
-    matrix: the normwise backward error of X (max over right-hand sides,
+def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs, reps=3):
+    """Batched solve + logdet + inverse of `count` FP64 SPD matrices
+    A_i = G_i G_i^T / n_i + alpha_i I with n_i log-uniform in [nmin, nmax] and
+    alpha_i log-uniform in [1e-4, 1], `nrhs` N(0,1) right-hand sides each:
+    one plan (cuDsposvxBatchedPlanCreate), one graph launch per batch
+    (cuDsposvxBatchedRun), timed with HIP events on the launch stream (median
+    of `reps` runs after the capturing run).  Per matrix: the normwise
+    backward error of X (max over right-hand sides,
     ||b - A x||_inf / (||A||_inf ||x||_inf)), the inverse residual
     ||I - A A^-1||_F / (||A||_F ||A^-1||_F) and the logdet against the FP64
-    Cholesky factor of the same matrix (torch.linalg.cholesky, FP64).  The
-    same pipeline with plain FP64 per matrix (cuDpotrf + cuDpotrs + cuDlogdet
+    Cholesky factor of the same matrix (torch.linalg.cholesky).  The same
+    work with plain FP64 routines per matrix (cuDpotrf + cuDpotrs + cuDlogdet
     + cuDpotri on a copy) is timed beside it."""
     import numpy as np
     rng = np.random.default_rng(9)
@@ -398,19 +401,34 @@ def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs):
         M.diagonal().add_(float(alphas[i]))
         A[offA[i]:offA[i] + n * n] = M.reshape(-1)
     B = torch.randn(int((sizes * nrhs).sum()), generator=g, device="cuda", dtype=torch.float64)
-    A0 = A.clone()
     X = torch.zeros_like(B)
     Ainv = torch.empty_like(A)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st, it, ld, info = pkg.cuDsposvxBatched(h, pkg.Lower, sizes.tolist(), offA.tolist(), pkg.ptr(A), nrhs,
-                                            offB.tolist(), pkg.ptr(B), pkg.ptr(X), pkg.ptr(Ainv))
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    assert st == 0 and all(v == 0 for v in info), (st, info[:8])
+    it = torch.zeros(count, dtype=torch.int32, device="cuda")
+    ld = torch.zeros(count, dtype=torch.float64, device="cuda")
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    t_plan = time.perf_counter()
+    plan = pkg.BatchedSolvePlan(h, pkg.Lower, sizes.tolist(), offA.tolist(), nrhs, offB.tolist())
+    t_plan = time.perf_counter() - t_plan
+    assert plan.status == 0, plan.status
+    times = []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        assert plan.run(pkg.ptr(A), pkg.ptr(B), pkg.ptr(X), pkg.ptr(Ainv), pkg.ptr(it), pkg.ptr(ld),
+                        pkg.ptr(info), s.cuda_stream) == 0
+        e1.record(s)
+        s.synchronize()
+        if r > 0:                      # the first run captures the graph
+            times.append(e0.elapsed_time(e1) * 1e-3)
+    plan.close()
+    t = sorted(times)[len(times) // 2]
+    info_h, it_h, ld_h = info.cpu().numpy(), it.cpu().numpy(), ld.cpu().numpy()
+    assert np.all(info_h == 0), info_h[:8]
     berr, ires, lderr = [], [], []
     for i, n in enumerate(sizes.tolist()):
-        M = A0[offA[i]:offA[i] + n * n].view(n, n)          # symmetric
+        M = A[offA[i]:offA[i] + n * n].view(n, n)           # symmetric; A is never written
         Xi = X[offB[i]:offB[i] + n * nrhs].view(nrhs, n).T
         Bi = B[offB[i]:offB[i] + n * nrhs].view(nrhs, n).T
         R = Bi - M @ Xi
@@ -420,9 +438,9 @@ def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs):
         ires.append((torch.linalg.matrix_norm(torch.eye(n, device="cuda", dtype=torch.float64) - M @ S)
                      / (torch.linalg.matrix_norm(M) * torch.linalg.matrix_norm(S))).item())
         ref = 2.0 * torch.log(torch.diagonal(torch.linalg.cholesky(M))).sum().item()
-        lderr.append(abs(ld[i] - ref) / max(1.0, abs(ref)))
+        lderr.append(abs(float(ld_h[i]) - ref) / max(1.0, abs(ref)))
     # plain FP64 per matrix: factor, solve, logdet, inverse
-    A64 = A0.clone()
+    A64 = A.clone()
     X64 = B.clone()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -435,14 +453,65 @@ def batched_mixed(torch, pkg, h, count, nmin, nmax, nrhs):
         assert st1 == st2 == st3 == st4 == 0 and inf1 == inf2 == inf4 == 0
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t1
-    flops = float((sizes.astype(np.float64) ** 3).sum())
+    n3 = float((sizes.astype(np.float64) ** 3).sum())
     q = lambda v: {"max": float(np.max(v)), "median": float(np.median(v))}
+    # algorithmic work of the task graph: factor n^3/3, L^-1 n^3/3, L^-T L^-1 n^3/3 (FP64 matrix cores)
     return {"count": count, "n_range": [nmin, nmax], "alpha_range": [1e-4, 1.0], "nrhs": nrhs,
-            "ms": round(t * 1e3, 2), "matrices_per_s": round(count / t, 1),
+            "ms": round(t * 1e3, 3), "matrices_per_s": round(count / t, 1),
+            "plan_create_ms": round(t_plan * 1e3, 1),
             "fp64_per_matrix_ms": round(t64 * 1e3, 2), "speedup_vs_fp64_per_matrix": round(t64 / t, 2),
-            "flops_potrf_n3_3_plus_potri_n3": 4 * flops / 3,
-            "iter_histogram": {str(k): int(v) for k, v in zip(*np.unique(it, return_counts=True))},
+            "flops_potrf_trtri_lauum": n3, "tflops": round(n3 / t / 1e12, 3),
+            "mfma_frac": round(n3 / t / 1e12 / FP64_PEAK, 4),
+            "iter_histogram": {str(k): int(v) for k, v in zip(*np.unique(it_h, return_counts=True))},
             "backward_error": q(berr), "inverse_residual": q(ires), "logdet_rel_error": q(lderr)}
+
+
+def pipeline_1024(torch, pkg, h, reps=5):
+    """The n = 1024 workload: dpotrf (lower) -> dlogdet -> dpotri of
+    A = G G^T / n + I (the oracle's generator, seed 1), on the GPU (best of
+    `reps`, wall time per pipeline) and with the oracle (the reference's CPU
+    algorithm) on this host; both results checked."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n = 1024
+    A = O.spd(1, n, 1.0)
+    dA0 = pkg.to_device(A, "cuda")
+    best = None
+    for r in range(reps + 1):
+        dA = dA0.clone()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st1, i1 = pkg.cuDpotrf(h, pkg.Lower, n, pkg.ptr(dA), n)
+        st2, lg = pkg.cuDlogdet(h, pkg.ptr(dA), n + 1, n)
+        st3, i3 = pkg.cuDpotri(h, pkg.Lower, n, pkg.ptr(dA), n)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        assert st1 == st2 == st3 == 0 and i1 == i3 == 0
+        if r > 0:
+            best = dt if best is None else min(best, dt)
+    X = pkg.from_device(dA)
+    t = time.perf_counter()
+    F, info = O.dpotrf("L", A.copy(order="F"))
+    lref = O.dlogdet(np.ascontiguousarray(F.reshape(-1, order="F")), n + 1, n)
+    Xref, info2 = O.dpotri("L", F.copy(order="F"))
+    tc = time.perf_counter() - t
+    assert info == info2 == 0
+    return {"n": n, "gpu_ms": round(best * 1e3, 3), "cpu_ms": round(tc * 1e3, 2), "cpu_threads": O.threads(),
+            "speedup": round(tc / best, 1), "gpu_tflops_potrf_plus_potri_n3": round((n ** 3 / 3 + n ** 3) / best / 1e12, 3),
+            "logdet": lg, "logdet_rel_diff_vs_cpu": abs(lg - lref) / abs(lref),
+            "inverse_residual": O.inverse_residual(A, X, "L")}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -578,6 +647,11 @@ def main():
 
     if not args.no_batched_mixed:
         extra["batched_mixed_solve_inverse"] = batched_mixed(torch, pkg, h, 512, 128, 2048, 4)
+        log(f"[bench] batched solve+inverse: {extra['batched_mixed_solve_inverse']}")
+
+    if not args.no_extra:
+        extra["pipeline_n1024"] = pipeline_1024(torch, pkg, h)
+        log(f"[bench] n=1024 pipeline: {extra['pipeline_n1024']}")
 
     cpu = None
     if not args.no_cpu:
