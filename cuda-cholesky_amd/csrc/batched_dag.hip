@@ -88,11 +88,15 @@ __device__ int read_uniform(const int *p, int *s_word) {
   return *s_word;
 }
 
-// Publish everything this workgroup wrote, then add v to each counter.
+// Publish everything this workgroup wrote, then add 1 to each counter.  Every
+// wave releases its own stores at agent scope (waits for them and writes the
+// XCD's L2 back) before the barrier: the barrier itself only waits for LDS
+// traffic, so a release by thread 0 alone would not cover the other waves'
+// global stores still in flight.
 __device__ void signal(int *c0, int *c1) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(c0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (c1 != nullptr) __hip_atomic_fetch_add(c1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

@@ -369,13 +369,12 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
       if (k < ni && col < nrhs) X[(long)col * ldx + ri + k] = acc.v[r];
     }
   }
-  // publish: the barrier orders every wave's stores before wave 0's
-  // agent-scope release, then the flag
+  // publish: every wave releases its own stores at agent scope (the barrier
+  // alone waits only for LDS traffic, not for other waves' global stores),
+  // then the barrier, then the flag
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
-  if (wave == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(ready + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (wave == 0 && lane == 0) __hip_atomic_store(ready + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Clears the tickets and ready flags (sync[1 ..]); the error word is left.
