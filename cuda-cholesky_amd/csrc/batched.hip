@@ -23,6 +23,8 @@
 // Storage is FP64 or FP32 (ST); arithmetic is FP64 throughout.  Only the
 // referenced triangle (uplo) is read or written; tiles reaching past n read
 // the identity and are not written back.
+#include <algorithm>
+
 #include "internal.h"
 #include "tile16.h"
 
@@ -49,8 +51,10 @@ struct View {
   // The load is unconditional, from a clamped (always valid) address, and the
   // value is selected afterwards: no branch per element, so loads stay in
   // flight together and register arrays filled from them stay in registers.
-  __device__ double get(int r, int c) const {
-    const double x = (double)M[off(min(r, n - 1), min(c, n - 1))];
+  __device__ double get(int r, int c) const { return fix(raw(r, c), r, c); }
+  // the two halves of get: the load from the clamped address, and the select
+  __device__ double raw(int r, int c) const { return (double)M[off(min(r, n - 1), min(c, n - 1))]; }
+  __device__ double fix(double x, int r, int c) const {
     return (r < n && c < n) ? (r >= c ? x : 0.0) : (r == c ? 1.0 : 0.0);
   }
   __device__ void put(int r, int c, double v) const {
@@ -289,7 +293,273 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident factorisation (factor + log-determinant, no inverse).
+//
+// One workgroup factors one matrix (n <= 16 NPMAX) at a time and holds the
+// matrix's whole lower tile triangle in registers, so that its HBM traffic is
+// one read of the triangle and one write of L.  The workgroup's waves have
+// two roles, each a separate code path with its own registers (the kernel's
+// register budget is the larger path, not their sum):
+//   * the leaf wave (the last) holds the diagonal tiles: at step p it updates
+//     tile (p + 1, p + 1), factors and inverts leaf p + 1 (look-ahead: the
+//     panel of step p + 1 then starts at once), and updates the diagonal
+//     tiles below;
+//   * the NW - 1 tile waves hold the off-diagonal tiles (I, J), I > J,
+//     numbered column by column (u = J (np - 1) - J (J - 1) / 2 + I - J - 1,
+//     np = ceil(n / 16)): wave u % (NW - 1), slot u / (NW - 1).  The
+//     trailing tiles of any step and the panel tiles of any column are
+//     consecutive numbers, dealt evenly over the tile waves.
+// A tile is held as T = A_IJ^T in the MFMA result layout (lane l, register
+// r: T(l / 16 + 4 r, l % 16) = A_IJ(l % 16, l / 16 + 4 r)), which is also the
+// operand fragment of A_IJ for k-step r: tiles enter the matrix cores with no
+// layout change.
+//   step p, panel:    T_Ip <- inv(L_pp) T_Ip = L_Ip^T (I > p), to LDS
+//                     (element (row, k) at P_I[k LL + row]); then copied to
+//                     the matrix by the tile waves;
+//   step p, trailing: T_IJ -= L_Jp L_Ip^T (p < J <= I).
+// Both roles meet at the same two barriers per step.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// The slot code (I | J << 8, or -1) of slot s, read from lane s of `codes`
+// into a scalar register.  The empty asm makes it opaque, so that nothing
+// derived from it is hoisted out of the step loop and kept live per slot.
+__device__ __forceinline__ int slot_code(int codes, int s) {
+  int c = __builtin_amdgcn_readlane(codes, s);
+  asm volatile("" : "+s"(c));
+  return c;
+}
+
+// T -= L_J L_I^T with the panel tiles L_I, L_J in LDS (P_I, P_J); lo is the
+// lane's fragment offset at k-step 0.
+__device__ __forceinline__ void tile_update(v4d &T, const double *PI, const double *PJ, int lo) {
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    T = __builtin_amdgcn_mfma_f64_16x16x4f64(-PJ[r * 4 * LL + lo], PI[r * 4 * LL + lo], T, 0, 0, 0);
+}
+
+template <typename ST, bool LOWER, int NPMAX, int NW, int OCC>
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int *order, ST *A,
+                         double *logdet, int *info) {
+  constexpr int NWT = NW - 1;                                            // tile waves
+  constexpr int NO = NPMAX * (NPMAX - 1) / 2, S = (NO + NWT - 1) / NWT;  // off-diagonal tiles, slots
+  static_assert(S <= 64, "slot codes live in one lane each");
+  __shared__ double P[NPMAX * LEAF * LL];  // panel tiles L_Ip
+  __shared__ double Li[LEAF * LL];         // inv(L_pp)
+  __shared__ double Dg[LEAF * LL];         // the diagonal tile, layout change
+  __shared__ int flag;
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int lo = lk * LL + li;  // this lane's operand fragment in an LDS tile (k-step 0)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int idx = order[t];
+    const int n = nvec[idx];
+    const View<ST, LOWER> v{A + offs[idx], n};
+    if (n == 0) {
+      // nothing to load: the clamped addresses below assume n >= 1
+      if (tid == 0) {
+        info[idx] = 0;
+        if (logdet != nullptr) logdet[idx] = 0.0;
+      }
+      continue;
+    }
+    const int np = (n + LEAF - 1) / LEAF;
+    int bad = -1;
+    if (wave == NW - 1) {
+      // ---- leaf wave: the diagonal tiles ----
+      v4d Td[NPMAX];
+#pragma unroll
+      for (int q = 0; q < NPMAX; q++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) Td[q][r] = v.raw(q * LEAF + li, q * LEAF + lk + 4 * r);
+#pragma unroll
+      for (int q = 0; q < NPMAX; q++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) Td[q][r] = v.fix(Td[q][r], q * LEAF + li, q * LEAF + lk + 4 * r);
+      double logpart = 0.0;  // lanes 0..15: log L_rr^2 of row r of every leaf
+      for (int p = -1; p < np; p++) {
+        if (p >= 0) {
+          bad = flag;
+          if (bad >= 0) break;
+          __syncthreads();  // the panel of column p is in LDS
+        }
+        const int D = p + 1;
+        if (D < np) {
+#pragma unroll
+          for (int q = 0; q < NPMAX; q++)
+            if (q == D) {
+              if (p >= 0) tile_update(Td[q], P + q * LEAF * LL, P + q * LEAF * LL, lo);
+#pragma unroll
+              for (int r = 0; r < 4; r++) Dg[r * 4 * LL + lo] = Td[q][r];  // A_DD(li, lk + 4 r)
+            }
+          t16::wave_lds_fence();
+          double a[LEAF];
+          t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
+          const int b = leaf_step<ST, LOWER>(a, v, D * LEAF, Li, logpart);
+          if (lane == 0) flag = b;
+          if (p >= 0) {
+#pragma unroll
+            for (int q = 0; q < NPMAX; q++)
+              if (q > D && q < np) tile_update(Td[q], P + q * LEAF * LL, P + q * LEAF * LL, lo);
+          }
+        }
+        __syncthreads();
+      }
+      double lsum = logpart;  // lanes 16..63 hold 0
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
+      if (lane == 0) {
+        info[idx] = bad >= 0 ? bad + 1 : 0;
+        if (logdet != nullptr) logdet[idx] = bad >= 0 ? __builtin_nan("") : lsum;
+      }
+    } else {
+      // ---- tile waves: the off-diagonal tiles ----
+      const int no = np * (np - 1) / 2;
+      int codes = -1;  // lane s < S: the code of this wave's slot s
+      {
+        const int u = lane * NWT + wave;
+        if (lane < S && u < no) {
+          int J = 0, start = 0;  // start: number of tile (J + 1, J)
+          while (u >= start + np - 1 - J) {
+            start += np - 1 - J;
+            J++;
+          }
+          codes = (u - start + J + 1) | (J << 8);
+        }
+      }
+      v4d T[S];
+      // every load is in flight before the first select, and each select
+      // overwrites its own load (no second register set); a slot without a
+      // tile loads tile (0, 0) and never uses it (no branch)
+      t16::static_for<0, S>([&](auto Sc) {
+        constexpr int s = decltype(Sc)::value;
+        const int c = max(slot_code(codes, s), 0);
+        const int I = c & 255, J = c >> 8;
+#pragma unroll
+        for (int r = 0; r < 4; r++) T[s][r] = v.raw(I * LEAF + li, J * LEAF + lk + 4 * r);
+      });
+      t16::static_for<0, S>([&](auto Sc) {
+        constexpr int s = decltype(Sc)::value;
+        const int c = max(slot_code(codes, s), 0);
+        const int I = c & 255, J = c >> 8;
+#pragma unroll
+        for (int r = 0; r < 4; r++) T[s][r] = v.fix(T[s][r], I * LEAF + li, J * LEAF + lk + 4 * r);
+      });
+      for (int p = -1; p < np; p++) {
+        if (p >= 0) {
+          bad = flag;
+          if (bad >= 0) break;
+          // panel of column p: T_Ip <- inv(L_pp) T_Ip, to LDS
+          t16::static_for<0, S>([&](auto Sc) {
+            constexpr int s = decltype(Sc)::value;
+            const int c = slot_code(codes, s);
+            if (c >= 0 && (c >> 8) == p) {
+              v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+              for (int r = 0; r < 4; r++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[r * 4 * LL + lo], T[s][r], acc, 0, 0, 0);
+              T[s] = acc;
+              double *Pt = P + (c & 255) * LEAF * LL;
+#pragma unroll
+              for (int r = 0; r < 4; r++) Pt[r * 4 * LL + lo] = acc[r];
+            }
+          });
+          __syncthreads();
+          // the panel to the matrix
+          const int rows = (np - 1 - p) * LEAF;
+          for (int e = tid; e < rows * LEAF; e += NWT * 64) {
+            // lower storage: consecutive threads down a column; upper: along a row
+            const int row = LOWER ? e % rows : e / LEAF, k = LOWER ? e / rows : e % LEAF;
+            v.put((p + 1) * LEAF + row, p * LEAF + k, P[(p + 1 + row / LEAF) * LEAF * LL + k * LL + row % LEAF]);
+          }
+          // the trailing tiles
+          t16::static_for<0, S>([&](auto Sc) {
+            constexpr int s = decltype(Sc)::value;
+            const int c = slot_code(codes, s);
+            if (c >= 0 && (c >> 8) > p) tile_update(T[s], P + (c & 255) * LEAF * LL, P + (c >> 8) * LEAF * LL, lo);
+          });
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();  // flag and LDS are reused by the next matrix
+  }
+}
+
+// Size classes of the register-resident kernel: orders up to
+// kBatchedRegSmallN = 128 (8 x 8 tiles, 4 waves, 3 workgroups per CU) and up
+// to 256 (16 x 16 tiles, 8 waves, one workgroup per CU).
+
+template <typename ST, bool LOWER, bool BIG>
+struct RegKernel;
+template <typename ST, bool LOWER>
+struct RegKernel<ST, LOWER, false> {
+  static constexpr int NPMAX = 8, NW = 4, OCC = 2;
+};
+template <typename ST, bool LOWER>
+struct RegKernel<ST, LOWER, true> {
+  static constexpr int NPMAX = 16, NW = 8, OCC = 2;
+};
+
+template <typename ST, bool LOWER, bool BIG>
+constexpr auto reg_kernel_ptr() {
+  using K = RegKernel<ST, LOWER, BIG>;
+  return potrf_batched_reg_kernel<ST, LOWER, K::NPMAX, K::NW, K::OCC>;
+}
+
 }  // namespace
+
+template <typename ST>
+hipError_t potrf_batched_reg(hipStream_t s, char uplo, int count, int nbig, const int *nvec, const long *offs,
+                             const int *order, ST *A, double *logdet, int *info, int grid_big, int grid_small) {
+  if (count <= 0) return hipSuccess;
+  if (nbig > 0) {
+    const dim3 g((unsigned)std::max(1, std::min(grid_big, nbig))), b(RegKernel<ST, true, true>::NW * 64);
+    if (uplo == 'L')
+      hipLaunchKernelGGL((reg_kernel_ptr<ST, true, true>()), g, b, 0, s, nbig, nvec, offs, order, A, logdet, info);
+    else
+      hipLaunchKernelGGL((reg_kernel_ptr<ST, false, true>()), g, b, 0, s, nbig, nvec, offs, order, A, logdet, info);
+    HC_TRY(hipGetLastError());
+  }
+  const int nsmall = count - nbig;
+  if (nsmall > 0) {
+    const dim3 g((unsigned)std::max(1, std::min(grid_small, nsmall))), b(RegKernel<ST, true, false>::NW * 64);
+    if (uplo == 'L')
+      hipLaunchKernelGGL((reg_kernel_ptr<ST, true, false>()), g, b, 0, s, nsmall, nvec, offs, order + nbig, A,
+                         logdet, info);
+    else
+      hipLaunchKernelGGL((reg_kernel_ptr<ST, false, false>()), g, b, 0, s, nsmall, nvec, offs, order + nbig, A,
+                         logdet, info);
+    HC_TRY(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+template <typename ST>
+int potrf_batched_reg_blocks_per_cu(char uplo, bool big) {
+  int nb = 0;
+  hipError_t e;
+  if (big)
+    e = uplo == 'L' ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, true, true>(),
+                                                                   RegKernel<ST, true, true>::NW * 64, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, false, true>(),
+                                                                   RegKernel<ST, false, true>::NW * 64, 0);
+  else
+    e = uplo == 'L' ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, true, false>(),
+                                                                   RegKernel<ST, true, false>::NW * 64, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, false, false>(),
+                                                                   RegKernel<ST, false, false>::NW * 64, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+template int potrf_batched_reg_blocks_per_cu<double>(char, bool);
+template int potrf_batched_reg_blocks_per_cu<float>(char, bool);
+template hipError_t potrf_batched_reg<double>(hipStream_t, char, int, int, const int *, const long *,
+                                              const int *, double *, double *, int *, int, int);
+template hipError_t potrf_batched_reg<float>(hipStream_t, char, int, int, const int *, const long *,
+                                             const int *, float *, double *, int *, int, int);
 
 template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,

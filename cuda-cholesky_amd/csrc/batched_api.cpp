@@ -19,7 +19,9 @@ struct __cubatchedplan_st {
   char uplo;
   char precision;  // 'D' (FP64 storage) or 'S' (FP32 storage)
   int count;
-  int grid, grid_inverse;  // persistent grid of the factor / factor + inverse kernel
+  int grid_inverse;        // persistent grid of the factor + inverse kernel
+  int nbig;                // matrices of order above kBatchedRegSmallN (first in d_order)
+  int grid_big, grid_small;  // persistent grids of the register-resident factor kernels
   int *d_n;
   long *d_off;
   int *d_order;
@@ -37,13 +39,18 @@ hipError_t launch(const __cubatchedplan_st *p, hipStream_t s, CUdeviceptr A, CUd
                   CUdeviceptr info, bool inverse) {
   double *ld = reinterpret_cast<double *>(logdet);
   int *inf = reinterpret_cast<int *>(info);
+  if (!inverse) {
+    if (p->precision == 'D')
+      return hc::potrf_batched_reg<double>(s, p->uplo, p->count, p->nbig, p->d_n, p->d_off, p->d_order,
+                                           reinterpret_cast<double *>(A), ld, inf, p->grid_big, p->grid_small);
+    return hc::potrf_batched_reg<float>(s, p->uplo, p->count, p->nbig, p->d_n, p->d_off, p->d_order,
+                                        reinterpret_cast<float *>(A), ld, inf, p->grid_big, p->grid_small);
+  }
   if (p->precision == 'D')
     return hc::potrf_batched<double>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
-                                     reinterpret_cast<double *>(A), ld, inf,
-                                     inverse ? p->grid_inverse : p->grid, inverse);
+                                     reinterpret_cast<double *>(A), ld, inf, p->grid_inverse, true);
   return hc::potrf_batched<float>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
-                                  reinterpret_cast<float *>(A), ld, inf,
-                                  inverse ? p->grid_inverse : p->grid, inverse);
+                                  reinterpret_cast<float *>(A), ld, inf, p->grid_inverse, true);
 }
 
 void release_graph(__cubatchedplan_st *p) {
@@ -152,8 +159,16 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
     const int want = forced > 0 ? forced : hc::tuned_batched_per_cu(precision, inverse);
     return want > 0 ? std::min(want, occ) : occ;
   };
-  p->grid = std::max(1, std::min<int>((int)count, cus * per_cu(false)));
   p->grid_inverse = std::max(1, std::min<int>((int)count, cus * per_cu(true)));
+  // register-resident factor kernels: every CU filled to their occupancy
+  for (size_t i = 0; i < count; i++)
+    if (nv[i] > hc::kBatchedRegSmallN) p->nbig++;
+  auto reg_occ = [&](bool big) {
+    return precision == 'D' ? hc::potrf_batched_reg_blocks_per_cu<double>((char)uplo, big)
+                            : hc::potrf_batched_reg_blocks_per_cu<float>((char)uplo, big);
+  };
+  p->grid_big = cus * (forced > 0 ? std::min(forced, reg_occ(true)) : reg_occ(true));
+  p->grid_small = cus * (forced > 0 ? std::min(forced, reg_occ(false)) : reg_occ(false));
   hipError_t e = hipSuccess;
   if (count > 0) {
     e = hipMalloc(&p->d_n, count * sizeof(int));

@@ -271,8 +271,10 @@ def test_workload4_batched_20000(gpu, lapack, prec, inverse):
     A0, offs = packed_batch(sizes, 0.5, seed=6)
     dtype = torch.float64 if prec == "D" else torch.float32
     eps = EPS if prec == "D" else EPS32
-    work = A0.to(dtype)
-    Ain = work.double()                     # the input as stored (FP32 rounded for 'S')
+    work = A0.to(dtype, copy=True)
+    # the input as stored (FP32 rounded for 'S'); a copy, since for 'D' the
+    # conversion alone would return the tensor that is factored in place
+    Ain = work.to(torch.float64, copy=True)
     logd = torch.zeros(count, dtype=torch.float64, device="cuda")
     info = torch.full((count,), -7, dtype=torch.int32, device="cuda")
     with gpu.BatchedPlan(lapack, gpu.Lower, prec, sizes.tolist(), offs.tolist()) as plan:
