@@ -298,27 +298,29 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
 //
 // One workgroup factors one matrix (n <= 16 NPMAX) at a time and holds the
 // matrix's whole lower tile triangle in registers, so that its HBM traffic is
-// one read of the triangle and one write of L.  The workgroup's waves have
-// two roles, each a separate code path with its own registers (the kernel's
-// register budget is the larger path, not their sum):
-//   * the leaf wave (the last) holds the diagonal tiles: at step p it updates
-//     tile (p + 1, p + 1), factors and inverts leaf p + 1 (look-ahead: the
-//     panel of step p + 1 then starts at once), and updates the diagonal
-//     tiles below;
-//   * the NW - 1 tile waves hold the off-diagonal tiles (I, J), I > J,
-//     numbered column by column (u = J (np - 1) - J (J - 1) / 2 + I - J - 1,
-//     np = ceil(n / 16)): wave u % (NW - 1), slot u / (NW - 1).  The
-//     trailing tiles of any step and the panel tiles of any column are
-//     consecutive numbers, dealt evenly over the tile waves.
+// one read of the triangle and one write of L.  The critical path of a
+// matrix is its chain of 16-column leaves (each one's factor and inverse
+// needs the previous one's panel), so the roles are cut around it:
+//   * the leaf wave (the last) holds no tiles: at step p it waits (LDS flag)
+//     for tile (p + 1, p + 1), factors and inverts leaf p + 1 and records its
+//     pivots (their logs are taken once per matrix, off the chain);
+//   * the NW - 1 tile waves hold every tile.  Off-diagonal tile (I, J),
+//     I > J, numbered column by column (u = J (np - 1) - J (J - 1) / 2 +
+//     I - J - 1, np = ceil(n / 16)), is in wave u % (NW - 1), slot
+//     u / (NW - 1): the trailing tiles of any step and the panel tiles of any
+//     column are consecutive numbers, dealt evenly.  Diagonal tile (J, J) is
+//     in wave J % (NW - 1), diagonal slot J / (NW - 1).
 // A tile is held as T = A_IJ^T in the MFMA result layout (lane l, register
 // r: T(l / 16 + 4 r, l % 16) = A_IJ(l % 16, l / 16 + 4 r)), which is also the
 // operand fragment of A_IJ for k-step r: tiles enter the matrix cores with no
-// layout change.
-//   step p, panel:    T_Ip <- inv(L_pp) T_Ip = L_Ip^T (I > p), to LDS
-//                     (element (row, k) at P_I[k LL + row]); then copied to
-//                     the matrix by the tile waves;
-//   step p, trailing: T_IJ -= L_Jp L_Ip^T (p < J <= I).
-// Both roles meet at the same two barriers per step.
+// layout change.  Per step p, with D = p + 1:
+//   panel:    T_Ip <- inv(L_pp) T_Ip = L_Ip^T (I > p), to LDS (element
+//             (row, k) at P_I[k LL + row]) and to the matrix;
+//   barrier;
+//   the owner of (D, D) updates it first, hands it to the leaf wave through
+//   LDS, then every tile wave updates its trailing tiles T_IJ -= L_Jp L_Ip^T
+//   (p < J <= I) while the leaf is factored;
+//   barrier.
 // ---------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -339,20 +341,50 @@ __device__ __forceinline__ void tile_update(v4d &T, const double *PI, const doub
     T = __builtin_amdgcn_mfma_f64_16x16x4f64(-PJ[r * 4 * LL + lo], PI[r * 4 * LL + lo], T, 0, 0, 0);
 }
 
+// The leaf wave's step: factor the leaf at g0 (rows in a[], lane r + 16 q
+// holds row r), write L_pp to the matrix, L_rr (rows below n) to piv[g0 + r]
+// and inv(L_pp) to Li; returns the failing pivot's column (g0 + k) or -1.
+template <typename ST, bool LOWER>
+__device__ __forceinline__ int leaf_step_piv(double (&a)[LEAF], const View<ST, LOWER> &v, int g0, double *Li,
+                                             double *piv) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  double unused = 0.0;
+  const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
+  if (b >= 0) return g0 + b;
+  double dg = a[0];
+  t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(r == decltype(C)::value, a[decltype(C)::value], dg); });
+  if (q == 0 && g0 + r < v.n) piv[g0 + r] = dg;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {  // lane group q writes columns q + 4 j of its row
+    const double f = t16::sel(q == 0, a[4 * j], t16::sel(q == 1, a[4 * j + 1], t16::sel(q == 2, a[4 * j + 2], a[4 * j + 3])));
+    v.put(g0 + r, g0 + q + 4 * j, f);
+  }
+  double x[4];
+  t16::leaf_inverse_cols<double>(a, x);
+#pragma unroll
+  for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
+  return -1;
+}
+
 template <typename ST, bool LOWER, int NPMAX, int NW, int OCC>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int *order, ST *A,
                          double *logdet, int *info) {
   constexpr int NWT = NW - 1;                                            // tile waves
   constexpr int NO = NPMAX * (NPMAX - 1) / 2, S = (NO + NWT - 1) / NWT;  // off-diagonal tiles, slots
+  constexpr int SD = (NPMAX + NWT - 1) / NWT;                           // diagonal slots
   static_assert(S <= 64, "slot codes live in one lane each");
   __shared__ double P[NPMAX * LEAF * LL];  // panel tiles L_Ip
   __shared__ double Li[LEAF * LL];         // inv(L_pp)
-  __shared__ double Dg[LEAF * LL];         // the diagonal tile, layout change
-  __shared__ int flag;
+  __shared__ double Dg[LEAF * LL];         // tile (D, D) on its way to the leaf wave
+  __shared__ double piv[NPMAX * LEAF];     // L_rr
+  __shared__ double red[NW];
+  __shared__ int flag, dready;             // failing column or -1; D once tile (D, D) is in Dg
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int lo = lk * LL + li;  // this lane's operand fragment in an LDS tile (k-step 0)
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (tid == 0) dready = -1;
+  __syncthreads();
   for (int t = blockIdx.x; t < count; t += gridDim.x) {
     const int idx = order[t];
     const int n = nvec[idx];
@@ -367,57 +399,41 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
     }
     const int np = (n + LEAF - 1) / LEAF;
     int bad = -1;
-    if (wave == NW - 1) {
-      // ---- leaf wave: the diagonal tiles ----
-      v4d Td[NPMAX];
-#pragma unroll
-      for (int q = 0; q < NPMAX; q++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) Td[q][r] = v.raw(q * LEAF + li, q * LEAF + lk + 4 * r);
-#pragma unroll
-      for (int q = 0; q < NPMAX; q++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) Td[q][r] = v.fix(Td[q][r], q * LEAF + li, q * LEAF + lk + 4 * r);
-      double logpart = 0.0;  // lanes 0..15: log L_rr^2 of row r of every leaf
+    if (wave == NWT) {
+      // ---- leaf wave ----
       for (int p = -1; p < np; p++) {
         if (p >= 0) {
           bad = flag;
-          if (bad >= 0) break;
+          if (bad != -1) break;
           __syncthreads();  // the panel of column p is in LDS
         }
         const int D = p + 1;
         if (D < np) {
-#pragma unroll
-          for (int q = 0; q < NPMAX; q++)
-            if (q == D) {
-              if (p >= 0) tile_update(Td[q], P + q * LEAF * LL, P + q * LEAF * LL, lo);
-#pragma unroll
-              for (int r = 0; r < 4; r++) Dg[r * 4 * LL + lo] = Td[q][r];  // A_DD(li, lk + 4 r)
+          // bounded wait (4 s of the 100 MHz real-time counter); abandoned,
+          // the matrix reports info = -1
+          const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+          bool ready = true;
+          while (__hip_atomic_load(&dready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != D) {
+            __builtin_amdgcn_s_sleep(1);
+            if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {
+              ready = false;
+              break;
             }
-          t16::wave_lds_fence();
-          double a[LEAF];
-          t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
-          const int b = leaf_step<ST, LOWER>(a, v, D * LEAF, Li, logpart);
-          if (lane == 0) flag = b;
-          if (p >= 0) {
-#pragma unroll
-            for (int q = 0; q < NPMAX; q++)
-              if (q > D && q < np) tile_update(Td[q], P + q * LEAF * LL, P + q * LEAF * LL, lo);
           }
+          int b = -2;
+          if (ready) {
+            double a[LEAF];
+            t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
+            b = leaf_step_piv<ST, LOWER>(a, v, D * LEAF, Li, piv);
+          }
+          if (lane == 0) flag = b;
         }
         __syncthreads();
       }
-      double lsum = logpart;  // lanes 16..63 hold 0
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
-      if (lane == 0) {
-        info[idx] = bad >= 0 ? bad + 1 : 0;
-        if (logdet != nullptr) logdet[idx] = bad >= 0 ? __builtin_nan("") : lsum;
-      }
     } else {
-      // ---- tile waves: the off-diagonal tiles ----
+      // ---- tile waves ----
       const int no = np * (np - 1) / 2;
-      int codes = -1;  // lane s < S: the code of this wave's slot s
+      int codes = -1;  // lane s < S: the code of this wave's off-diagonal slot s
       {
         const int u = lane * NWT + wave;
         if (lane < S && u < no) {
@@ -429,7 +445,7 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
           codes = (u - start + J + 1) | (J << 8);
         }
       }
-      v4d T[S];
+      v4d T[S], Td[SD];
       // every load is in flight before the first select, and each select
       // overwrites its own load (no second register set); a slot without a
       // tile loads tile (0, 0) and never uses it (no branch)
@@ -440,6 +456,12 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
 #pragma unroll
         for (int r = 0; r < 4; r++) T[s][r] = v.raw(I * LEAF + li, J * LEAF + lk + 4 * r);
       });
+#pragma unroll
+      for (int q = 0; q < SD; q++) {
+        const int J = min(q * NWT + wave, np - 1);
+#pragma unroll
+        for (int r = 0; r < 4; r++) Td[q][r] = v.raw(J * LEAF + li, J * LEAF + lk + 4 * r);
+      }
       t16::static_for<0, S>([&](auto Sc) {
         constexpr int s = decltype(Sc)::value;
         const int c = max(slot_code(codes, s), 0);
@@ -447,11 +469,17 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
 #pragma unroll
         for (int r = 0; r < 4; r++) T[s][r] = v.fix(T[s][r], I * LEAF + li, J * LEAF + lk + 4 * r);
       });
+#pragma unroll
+      for (int q = 0; q < SD; q++) {
+        const int J = min(q * NWT + wave, np - 1);
+#pragma unroll
+        for (int r = 0; r < 4; r++) Td[q][r] = v.fix(Td[q][r], J * LEAF + li, J * LEAF + lk + 4 * r);
+      }
       for (int p = -1; p < np; p++) {
         if (p >= 0) {
           bad = flag;
-          if (bad >= 0) break;
-          // panel of column p: T_Ip <- inv(L_pp) T_Ip, to LDS
+          if (bad != -1) break;
+          // panel of column p: T_Ip <- inv(L_pp) T_Ip, to LDS and to the matrix
           t16::static_for<0, S>([&](auto Sc) {
             constexpr int s = decltype(Sc)::value;
             const int c = slot_code(codes, s);
@@ -461,20 +489,36 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
               for (int r = 0; r < 4; r++)
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[r * 4 * LL + lo], T[s][r], acc, 0, 0, 0);
               T[s] = acc;
-              double *Pt = P + (c & 255) * LEAF * LL;
+              const int I = c & 255;
+              double *Pt = P + I * LEAF * LL;
 #pragma unroll
-              for (int r = 0; r < 4; r++) Pt[r * 4 * LL + lo] = acc[r];
+              for (int r = 0; r < 4; r++) {
+                Pt[r * 4 * LL + lo] = acc[r];
+                v.put(I * LEAF + li, p * LEAF + lk + 4 * r, acc[r]);  // L_Ip(li, lk + 4 r)
+              }
             }
           });
           __syncthreads();
-          // the panel to the matrix
-          const int rows = (np - 1 - p) * LEAF;
-          for (int e = tid; e < rows * LEAF; e += NWT * 64) {
-            // lower storage: consecutive threads down a column; upper: along a row
-            const int row = LOWER ? e % rows : e / LEAF, k = LOWER ? e / rows : e % LEAF;
-            v.put((p + 1) * LEAF + row, p * LEAF + k, P[(p + 1 + row / LEAF) * LEAF * LL + k * LL + row % LEAF]);
+        }
+        const int D = p + 1;
+        if (D < np && wave == D % NWT) {
+          // tile (D, D): its last update, then to the leaf wave
+#pragma unroll
+          for (int q = 0; q < SD; q++)
+            if (q == D / NWT) {
+              if (p >= 0) tile_update(Td[q], P + D * LEAF * LL, P + D * LEAF * LL, lo);
+#pragma unroll
+              for (int r = 0; r < 4; r++) Dg[r * 4 * LL + lo] = Td[q][r];  // A_DD(li, lk + 4 r)
+            }
+          if (lane == 0) __hip_atomic_store(&dready, D, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (p >= 0) {
+          // the other trailing tiles
+#pragma unroll
+          for (int q = 0; q < SD; q++) {
+            const int J = q * NWT + wave;
+            if (J > D && J < np) tile_update(Td[q], P + J * LEAF * LL, P + J * LEAF * LL, lo);
           }
-          // the trailing tiles
           t16::static_for<0, S>([&](auto Sc) {
             constexpr int s = decltype(Sc)::value;
             const int c = slot_code(codes, s);
@@ -484,82 +528,106 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
         __syncthreads();
       }
     }
-    __syncthreads();  // flag and LDS are reused by the next matrix
+    // log-determinant: 2 sum log L_rr over the pivots, one per thread
+    double lg = 0.0;
+    if (bad == -1)
+      for (int r = tid; r < n; r += NW * 64) lg += log(piv[r]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
+    if (lane == 0) red[wave] = lg;
+    if (tid == 0) dready = -1;  // every wave is past its last use of the flag
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < NW; w++) sum += red[w];
+      // bad: failing column, -1 (none) or -2 (a wait was abandoned)
+      info[idx] = bad >= 0 ? bad + 1 : (bad == -2 ? -1 : 0);
+      if (logdet != nullptr) logdet[idx] = bad != -1 ? __builtin_nan("") : 2.0 * sum;
+    }
+    __syncthreads();  // flag, piv and LDS tiles are reused by the next matrix
   }
 }
 
-// Size classes of the register-resident kernel: orders up to
-// kBatchedRegSmallN = 128 (8 x 8 tiles, 4 waves, 3 workgroups per CU) and up
-// to 256 (16 x 16 tiles, 8 waves, one workgroup per CU).
-
-template <typename ST, bool LOWER, bool BIG>
-struct RegKernel;
-template <typename ST, bool LOWER>
-struct RegKernel<ST, LOWER, false> {
-  static constexpr int NPMAX = 8, NW = 4, OCC = 2;
+// Size classes of the register-resident kernel, by tile rows np (order n):
+//   0: np <= 4  (n <= 64):  2 waves, 3 waves per SIMD (6 workgroups per CU);
+//   1: np <= 8  (n <= 128): 4 waves, 3 waves per SIMD (3 workgroups per CU);
+//   2: np <= 16 (n <= 256): 8 waves, 2 waves per SIMD (1 workgroup per CU).
+// More workgroups per CU for the small classes overlap more leaf chains.
+template <int CLS>
+struct RegClass;
+template <>
+struct RegClass<0> {
+  static constexpr int NPMAX = 4, NW = 2, OCC = 3;
 };
-template <typename ST, bool LOWER>
-struct RegKernel<ST, LOWER, true> {
+template <>
+struct RegClass<1> {
+  static constexpr int NPMAX = 8, NW = 4, OCC = 3;
+};
+template <>
+struct RegClass<2> {
   static constexpr int NPMAX = 16, NW = 8, OCC = 2;
 };
 
-template <typename ST, bool LOWER, bool BIG>
+template <typename ST, bool LOWER, int CLS>
 constexpr auto reg_kernel_ptr() {
-  using K = RegKernel<ST, LOWER, BIG>;
+  using K = RegClass<CLS>;
   return potrf_batched_reg_kernel<ST, LOWER, K::NPMAX, K::NW, K::OCC>;
+}
+
+template <typename ST, bool LOWER, int CLS>
+void reg_launch(hipStream_t s, int grid, int count, const int *nvec, const long *offs, const int *order, ST *A,
+                double *logdet, int *info) {
+  hipLaunchKernelGGL((reg_kernel_ptr<ST, LOWER, CLS>()), dim3((unsigned)std::max(1, std::min(grid, count))),
+                     dim3(RegClass<CLS>::NW * 64), 0, s, count, nvec, offs, order, A, logdet, info);
+}
+
+template <typename ST, bool LOWER, int CLS>
+int reg_occupancy() {
+  int nb = 0;
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, LOWER, CLS>(), RegClass<CLS>::NW * 64, 0);
+  return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
 }  // namespace
 
+int batched_reg_class(int n) { return n <= 64 ? 0 : (n <= 128 ? 1 : 2); }
+
 template <typename ST>
-hipError_t potrf_batched_reg(hipStream_t s, char uplo, int count, int nbig, const int *nvec, const long *offs,
-                             const int *order, ST *A, double *logdet, int *info, int grid_big, int grid_small) {
-  if (count <= 0) return hipSuccess;
-  if (nbig > 0) {
-    const dim3 g((unsigned)std::max(1, std::min(grid_big, nbig))), b(RegKernel<ST, true, true>::NW * 64);
-    if (uplo == 'L')
-      hipLaunchKernelGGL((reg_kernel_ptr<ST, true, true>()), g, b, 0, s, nbig, nvec, offs, order, A, logdet, info);
-    else
-      hipLaunchKernelGGL((reg_kernel_ptr<ST, false, true>()), g, b, 0, s, nbig, nvec, offs, order, A, logdet, info);
-    HC_TRY(hipGetLastError());
-  }
-  const int nsmall = count - nbig;
-  if (nsmall > 0) {
-    const dim3 g((unsigned)std::max(1, std::min(grid_small, nsmall))), b(RegKernel<ST, true, false>::NW * 64);
-    if (uplo == 'L')
-      hipLaunchKernelGGL((reg_kernel_ptr<ST, true, false>()), g, b, 0, s, nsmall, nvec, offs, order + nbig, A,
-                         logdet, info);
-    else
-      hipLaunchKernelGGL((reg_kernel_ptr<ST, false, false>()), g, b, 0, s, nsmall, nvec, offs, order + nbig, A,
-                         logdet, info);
-    HC_TRY(hipGetLastError());
+hipError_t potrf_batched_reg(hipStream_t s, char uplo, const int cnt[3], const int grid[3], const int *nvec,
+                             const long *offs, const int *order, ST *A, double *logdet, int *info) {
+  // order lists the matrices by decreasing order: class 2, then 1, then 0
+  const int *o = order;
+  for (int c = 2; c >= 0; c--) {
+    if (cnt[c] > 0) {
+      if (uplo == 'L') {
+        if (c == 2) reg_launch<ST, true, 2>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
+        else if (c == 1) reg_launch<ST, true, 1>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
+        else reg_launch<ST, true, 0>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
+      } else {
+        if (c == 2) reg_launch<ST, false, 2>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
+        else if (c == 1) reg_launch<ST, false, 1>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
+        else reg_launch<ST, false, 0>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
+      }
+      HC_TRY(hipGetLastError());
+    }
+    o += cnt[c];
   }
   return hipSuccess;
 }
 
 template <typename ST>
-int potrf_batched_reg_blocks_per_cu(char uplo, bool big) {
-  int nb = 0;
-  hipError_t e;
-  if (big)
-    e = uplo == 'L' ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, true, true>(),
-                                                                   RegKernel<ST, true, true>::NW * 64, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, false, true>(),
-                                                                   RegKernel<ST, false, true>::NW * 64, 0);
-  else
-    e = uplo == 'L' ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, true, false>(),
-                                                                   RegKernel<ST, true, false>::NW * 64, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reg_kernel_ptr<ST, false, false>(),
-                                                                   RegKernel<ST, false, false>::NW * 64, 0);
-  return (e == hipSuccess && nb > 0) ? nb : 1;
+int potrf_batched_reg_blocks_per_cu(char uplo, int cls) {
+  if (uplo == 'L') return cls == 2 ? reg_occupancy<ST, true, 2>() : (cls == 1 ? reg_occupancy<ST, true, 1>() : reg_occupancy<ST, true, 0>());
+  return cls == 2 ? reg_occupancy<ST, false, 2>() : (cls == 1 ? reg_occupancy<ST, false, 1>() : reg_occupancy<ST, false, 0>());
 }
 
-template int potrf_batched_reg_blocks_per_cu<double>(char, bool);
-template int potrf_batched_reg_blocks_per_cu<float>(char, bool);
-template hipError_t potrf_batched_reg<double>(hipStream_t, char, int, int, const int *, const long *,
-                                              const int *, double *, double *, int *, int, int);
-template hipError_t potrf_batched_reg<float>(hipStream_t, char, int, int, const int *, const long *,
-                                             const int *, float *, double *, int *, int, int);
+template int potrf_batched_reg_blocks_per_cu<double>(char, int);
+template int potrf_batched_reg_blocks_per_cu<float>(char, int);
+template hipError_t potrf_batched_reg<double>(hipStream_t, char, const int *, const int *, const int *,
+                                              const long *, const int *, double *, double *, int *);
+template hipError_t potrf_batched_reg<float>(hipStream_t, char, const int *, const int *, const int *,
+                                             const long *, const int *, float *, double *, int *);
 
 template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,

@@ -20,8 +20,7 @@ struct __cubatchedplan_st {
   char precision;  // 'D' (FP64 storage) or 'S' (FP32 storage)
   int count;
   int grid_inverse;        // persistent grid of the factor + inverse kernel
-  int nbig;                // matrices of order above kBatchedRegSmallN (first in d_order)
-  int grid_big, grid_small;  // persistent grids of the register-resident factor kernels
+  int reg_count[3], reg_grid[3];  // register-resident factor kernels: matrices and grid per size class
   int *d_n;
   long *d_off;
   int *d_order;
@@ -41,10 +40,10 @@ hipError_t launch(const __cubatchedplan_st *p, hipStream_t s, CUdeviceptr A, CUd
   int *inf = reinterpret_cast<int *>(info);
   if (!inverse) {
     if (p->precision == 'D')
-      return hc::potrf_batched_reg<double>(s, p->uplo, p->count, p->nbig, p->d_n, p->d_off, p->d_order,
-                                           reinterpret_cast<double *>(A), ld, inf, p->grid_big, p->grid_small);
-    return hc::potrf_batched_reg<float>(s, p->uplo, p->count, p->nbig, p->d_n, p->d_off, p->d_order,
-                                        reinterpret_cast<float *>(A), ld, inf, p->grid_big, p->grid_small);
+      return hc::potrf_batched_reg<double>(s, p->uplo, p->reg_count, p->reg_grid, p->d_n, p->d_off, p->d_order,
+                                           reinterpret_cast<double *>(A), ld, inf);
+    return hc::potrf_batched_reg<float>(s, p->uplo, p->reg_count, p->reg_grid, p->d_n, p->d_off, p->d_order,
+                                        reinterpret_cast<float *>(A), ld, inf);
   }
   if (p->precision == 'D')
     return hc::potrf_batched<double>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
@@ -161,14 +160,12 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
   };
   p->grid_inverse = std::max(1, std::min<int>((int)count, cus * per_cu(true)));
   // register-resident factor kernels: every CU filled to their occupancy
-  for (size_t i = 0; i < count; i++)
-    if (nv[i] > hc::kBatchedRegSmallN) p->nbig++;
-  auto reg_occ = [&](bool big) {
-    return precision == 'D' ? hc::potrf_batched_reg_blocks_per_cu<double>((char)uplo, big)
-                            : hc::potrf_batched_reg_blocks_per_cu<float>((char)uplo, big);
-  };
-  p->grid_big = cus * (forced > 0 ? std::min(forced, reg_occ(true)) : reg_occ(true));
-  p->grid_small = cus * (forced > 0 ? std::min(forced, reg_occ(false)) : reg_occ(false));
+  for (size_t i = 0; i < count; i++) p->reg_count[hc::batched_reg_class(nv[i])]++;
+  for (int c = 0; c < 3; c++) {
+    const int occ = precision == 'D' ? hc::potrf_batched_reg_blocks_per_cu<double>((char)uplo, c)
+                                     : hc::potrf_batched_reg_blocks_per_cu<float>((char)uplo, c);
+    p->reg_grid[c] = cus * (forced > 0 ? std::min(forced, occ) : occ);
+  }
   hipError_t e = hipSuccess;
   if (count > 0) {
     e = hipMalloc(&p->d_n, count * sizeof(int));

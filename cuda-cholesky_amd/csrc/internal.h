@@ -284,15 +284,15 @@ template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,
                          const int *order, ST *A, double *logdet, int *info, int grid, bool inverse);
 
-// Register-resident factorisation + logdet (no inverse): the first nbig
-// entries of `order` (orders above 128) run on the 16 x 16-tile kernel with
-// grid_big workgroups, the rest (orders up to 128) on the 8 x 8-tile kernel
-// with grid_small.
+// Register-resident factorisation + logdet (no inverse), in three size
+// classes (batched_reg_class: orders up to 64, up to 128, up to 256).
+// `order` lists class 2, then 1, then 0 (decreasing order); cnt[c] matrices of
+// class c run on grid[c] persistent workgroups.
+int batched_reg_class(int n);
 template <typename ST>
-hipError_t potrf_batched_reg(hipStream_t s, char uplo, int count, int nbig, const int *nvec, const long *offs,
-                             const int *order, ST *A, double *logdet, int *info, int grid_big, int grid_small);
-constexpr int kBatchedRegSmallN = 128;
-template <typename ST> int potrf_batched_reg_blocks_per_cu(char uplo, bool big);
+hipError_t potrf_batched_reg(hipStream_t s, char uplo, const int cnt[3], const int grid[3], const int *nvec,
+                             const long *offs, const int *order, ST *A, double *logdet, int *info);
+template <typename ST> int potrf_batched_reg_blocks_per_cu(char uplo, int cls);
 
 // GMRES basis helpers: right-hand side k owns V + (k nv + i) n, i < nv.
 hipError_t basis_dots(hipStream_t s, long n, long nrhs, int nv, int first, int nvec, const double *V,

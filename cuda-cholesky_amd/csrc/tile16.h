@@ -115,12 +115,11 @@ __device__ __forceinline__ int row_bcast_b32(int v) {
   // every lane reads a valid source lane, so no "old" value is needed
   return __builtin_amdgcn_mov_dpp(v, 0x150 + M, 0xf, 0xf, true);
 }
+// A 64-bit value moves in one v_mov_b64_dpp (gfx950 has 64-bit DPP moves
+// with row_newbcast), not as two 32-bit halves.
 template <int M>
 __device__ __forceinline__ double row_bcast(double v) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)row_bcast_b32<M>((int)(unsigned)b);
-  const unsigned hi = (unsigned)row_bcast_b32<M>((int)(unsigned)(b >> 32));
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + M, 0xf, 0xf, true);
 }
 template <int M>
 __device__ __forceinline__ zd row_bcast(zd v) {
