@@ -148,6 +148,7 @@ SIGNATURES = {
     "cuPotrfBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
     "cuPotriBatchedRun": [_h, _ptr, _ptr, _ptr, _stream],
     "cuPotrfBatchedPlanDestroy": [_h],
+    "cuPotrfBatchedNoAlloc": [_h, _enum, ctypes.c_char, _sz, _ptr, _ptr, _ptr, _ptr, _ptr, _stream],
     "cuLAPACKSetTiming": [_h, ctypes.c_int],
     "cuLAPACKSetBlockSize": [_h, ctypes.c_char, ctypes.c_long],
     "cuLAPACKGetTiming": [_h, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),

@@ -366,10 +366,34 @@ __device__ __forceinline__ int leaf_step_piv(double (&a)[LEAF], const View<ST, L
   return -1;
 }
 
-template <typename ST, bool LOWER, int NPMAX, int NW, int OCC>
+// Size classes of the register-resident kernel, by tile rows np (order n):
+//   0: np <= 4  (n <= 64):  2 waves, 3 waves per SIMD (6 workgroups per CU);
+//   1: np <= 8  (n <= 128): 4 waves, 3 waves per SIMD (3 workgroups per CU);
+//   2: np <= 16 (n <= 256): 8 waves, 2 waves per SIMD (1 workgroup per CU).
+// More workgroups per CU for the small classes overlap more leaf chains.
+template <int CLS>
+struct RegClass;
+template <>
+struct RegClass<0> {
+  static constexpr int NPMAX = 4, NW = 2, OCC = 3;
+};
+template <>
+struct RegClass<1> {
+  static constexpr int NPMAX = 8, NW = 4, OCC = 3;
+};
+template <>
+struct RegClass<2> {
+  static constexpr int NPMAX = 16, NW = 8, OCC = 2;
+};
+
+// cnt[c]: matrices of class c; `order` lists class 2, then 1, then 0.
+template <typename ST, bool LOWER, int CLS, int NPMAX = RegClass<CLS>::NPMAX, int NW = RegClass<CLS>::NW,
+          int OCC = RegClass<CLS>::OCC>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
-potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int *order, ST *A,
+potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, const int *order, ST *A,
                          double *logdet, int *info) {
+  const int count = cnt[CLS];
+  order += CLS == 2 ? 0 : (CLS == 1 ? cnt[2] : cnt[2] + cnt[1]);
   constexpr int NWT = NW - 1;                                            // tile waves
   constexpr int NO = NPMAX * (NPMAX - 1) / 2, S = (NO + NWT - 1) / NWT;  // off-diagonal tiles, slots
   constexpr int SD = (NPMAX + NWT - 1) / NWT;                           // diagonal slots
@@ -548,37 +572,16 @@ potrf_batched_reg_kernel(int count, const int *nvec, const long *offs, const int
   }
 }
 
-// Size classes of the register-resident kernel, by tile rows np (order n):
-//   0: np <= 4  (n <= 64):  2 waves, 3 waves per SIMD (6 workgroups per CU);
-//   1: np <= 8  (n <= 128): 4 waves, 3 waves per SIMD (3 workgroups per CU);
-//   2: np <= 16 (n <= 256): 8 waves, 2 waves per SIMD (1 workgroup per CU).
-// More workgroups per CU for the small classes overlap more leaf chains.
-template <int CLS>
-struct RegClass;
-template <>
-struct RegClass<0> {
-  static constexpr int NPMAX = 4, NW = 2, OCC = 3;
-};
-template <>
-struct RegClass<1> {
-  static constexpr int NPMAX = 8, NW = 4, OCC = 3;
-};
-template <>
-struct RegClass<2> {
-  static constexpr int NPMAX = 16, NW = 8, OCC = 2;
-};
-
 template <typename ST, bool LOWER, int CLS>
 constexpr auto reg_kernel_ptr() {
-  using K = RegClass<CLS>;
-  return potrf_batched_reg_kernel<ST, LOWER, K::NPMAX, K::NW, K::OCC>;
+  return potrf_batched_reg_kernel<ST, LOWER, CLS>;
 }
 
 template <typename ST, bool LOWER, int CLS>
-void reg_launch(hipStream_t s, int grid, int count, const int *nvec, const long *offs, const int *order, ST *A,
+void reg_launch(hipStream_t s, int grid, const int *cnt, const int *nvec, const long *offs, const int *order, ST *A,
                 double *logdet, int *info) {
-  hipLaunchKernelGGL((reg_kernel_ptr<ST, LOWER, CLS>()), dim3((unsigned)std::max(1, std::min(grid, count))),
-                     dim3(RegClass<CLS>::NW * 64), 0, s, count, nvec, offs, order, A, logdet, info);
+  hipLaunchKernelGGL((reg_kernel_ptr<ST, LOWER, CLS>()), dim3((unsigned)std::max(1, grid)),
+                     dim3(RegClass<CLS>::NW * 64), 0, s, cnt, nvec, offs, order, A, logdet, info);
 }
 
 template <typename ST, bool LOWER, int CLS>
@@ -591,27 +594,22 @@ int reg_occupancy() {
 
 }  // namespace
 
-int batched_reg_class(int n) { return n <= 64 ? 0 : (n <= 128 ? 1 : 2); }
-
 template <typename ST>
-hipError_t potrf_batched_reg(hipStream_t s, char uplo, const int cnt[3], const int grid[3], const int *nvec,
-                             const long *offs, const int *order, ST *A, double *logdet, int *info) {
-  // order lists the matrices by decreasing order: class 2, then 1, then 0
-  const int *o = order;
+hipError_t potrf_batched_reg(hipStream_t s, char uplo, const int *d_cnt, const int *h_cnt, const int grid[3],
+                             const int *nvec, const long *offs, const int *order, ST *A, double *logdet, int *info) {
   for (int c = 2; c >= 0; c--) {
-    if (cnt[c] > 0) {
-      if (uplo == 'L') {
-        if (c == 2) reg_launch<ST, true, 2>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
-        else if (c == 1) reg_launch<ST, true, 1>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
-        else reg_launch<ST, true, 0>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
-      } else {
-        if (c == 2) reg_launch<ST, false, 2>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
-        else if (c == 1) reg_launch<ST, false, 1>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
-        else reg_launch<ST, false, 0>(s, grid[c], cnt[c], nvec, offs, o, A, logdet, info);
-      }
-      HC_TRY(hipGetLastError());
+    if (h_cnt != nullptr && h_cnt[c] == 0) continue;
+    const int g = h_cnt != nullptr ? std::min(grid[c], h_cnt[c]) : grid[c];
+    if (uplo == 'L') {
+      if (c == 2) reg_launch<ST, true, 2>(s, g, d_cnt, nvec, offs, order, A, logdet, info);
+      else if (c == 1) reg_launch<ST, true, 1>(s, g, d_cnt, nvec, offs, order, A, logdet, info);
+      else reg_launch<ST, true, 0>(s, g, d_cnt, nvec, offs, order, A, logdet, info);
+    } else {
+      if (c == 2) reg_launch<ST, false, 2>(s, g, d_cnt, nvec, offs, order, A, logdet, info);
+      else if (c == 1) reg_launch<ST, false, 1>(s, g, d_cnt, nvec, offs, order, A, logdet, info);
+      else reg_launch<ST, false, 0>(s, g, d_cnt, nvec, offs, order, A, logdet, info);
     }
-    o += cnt[c];
+    HC_TRY(hipGetLastError());
   }
   return hipSuccess;
 }
@@ -625,9 +623,9 @@ int potrf_batched_reg_blocks_per_cu(char uplo, int cls) {
 template int potrf_batched_reg_blocks_per_cu<double>(char, int);
 template int potrf_batched_reg_blocks_per_cu<float>(char, int);
 template hipError_t potrf_batched_reg<double>(hipStream_t, char, const int *, const int *, const int *,
-                                              const long *, const int *, double *, double *, int *);
+                                              const int *, const long *, const int *, double *, double *, int *);
 template hipError_t potrf_batched_reg<float>(hipStream_t, char, const int *, const int *, const int *,
-                                             const long *, const int *, float *, double *, int *);
+                                             const int *, const long *, const int *, float *, double *, int *);
 
 template <typename ST>
 hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, const long *offs,

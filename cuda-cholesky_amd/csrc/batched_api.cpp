@@ -24,6 +24,7 @@ struct __cubatchedplan_st {
   int *d_n;
   long *d_off;
   int *d_order;
+  int *d_cnt;  // reg_count on the device
   // graph for the last (A, logdet, info, inverse) run
   hipStream_t capture;
   hipGraph_t graph;
@@ -40,10 +41,10 @@ hipError_t launch(const __cubatchedplan_st *p, hipStream_t s, CUdeviceptr A, CUd
   int *inf = reinterpret_cast<int *>(info);
   if (!inverse) {
     if (p->precision == 'D')
-      return hc::potrf_batched_reg<double>(s, p->uplo, p->reg_count, p->reg_grid, p->d_n, p->d_off, p->d_order,
-                                           reinterpret_cast<double *>(A), ld, inf);
-    return hc::potrf_batched_reg<float>(s, p->uplo, p->reg_count, p->reg_grid, p->d_n, p->d_off, p->d_order,
-                                        reinterpret_cast<float *>(A), ld, inf);
+      return hc::potrf_batched_reg<double>(s, p->uplo, p->d_cnt, p->reg_count, p->reg_grid, p->d_n, p->d_off,
+                                           p->d_order, reinterpret_cast<double *>(A), ld, inf);
+    return hc::potrf_batched_reg<float>(s, p->uplo, p->d_cnt, p->reg_count, p->reg_grid, p->d_n, p->d_off,
+                                        p->d_order, reinterpret_cast<float *>(A), ld, inf);
   }
   if (p->precision == 'D')
     return hc::potrf_batched<double>(s, p->uplo, p->count, p->d_n, p->d_off, p->d_order,
@@ -72,6 +73,7 @@ CUresult plan_release(__cubatchedplan_st *p) {
   if (p->d_n != nullptr) keep(hipFree(p->d_n));
   if (p->d_off != nullptr) keep(hipFree(p->d_off));
   if (p->d_order != nullptr) keep(hipFree(p->d_order));
+  if (p->d_cnt != nullptr) keep(hipFree(p->d_cnt));
   std::free(p);
   return first;
 }
@@ -166,8 +168,9 @@ CUresult cuPotrfBatchedPlanCreate(CULAPACKhandle h, CBlasUplo uplo, char precisi
                                      : hc::potrf_batched_reg_blocks_per_cu<float>((char)uplo, c);
     p->reg_grid[c] = cus * (forced > 0 ? std::min(forced, occ) : occ);
   }
-  hipError_t e = hipSuccess;
-  if (count > 0) {
+  hipError_t e = hipMalloc(&p->d_cnt, 3 * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(p->d_cnt, p->reg_count, 3 * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && count > 0) {
     e = hipMalloc(&p->d_n, count * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&p->d_off, count * sizeof(long));
     if (e == hipSuccess) e = hipMalloc(&p->d_order, count * sizeof(int));

@@ -72,6 +72,8 @@ CUresult cuLAPACKCreate(CULAPACKhandle *handle) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&h->dinfo, 64);
   if (e == hipSuccess) e = hipHostMalloc(&h->hinfo, 64, hipHostMallocDefault);
+  // the no-allocation batched entry point's per-device scratch address
+  if (e == hipSuccess) e = hc::noalloc_setup(h->device);
   if (e != hipSuccess) {
     const CUresult err = hc::to_cu(e);
     hc::report("cuLAPACKCreate", __func__, __FILE__, __LINE__, err);

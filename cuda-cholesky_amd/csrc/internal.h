@@ -286,13 +286,18 @@ hipError_t potrf_batched(hipStream_t s, char uplo, int count, const int *nvec, c
 
 // Register-resident factorisation + logdet (no inverse), in three size
 // classes (batched_reg_class: orders up to 64, up to 128, up to 256).
-// `order` lists class 2, then 1, then 0 (decreasing order); cnt[c] matrices of
-// class c run on grid[c] persistent workgroups.
-int batched_reg_class(int n);
+// `order` lists class 2, then 1, then 0 (decreasing order); d_cnt[c] (device
+// memory) matrices of class c run on grid[c] persistent workgroups.  With the
+// counts also on the host (h_cnt), empty classes are not launched and grids
+// are cut to the class size; with h_cnt null every class is launched.
+__host__ __device__ constexpr int batched_reg_class(int n) { return n <= 64 ? 0 : (n <= 128 ? 1 : 2); }
 template <typename ST>
-hipError_t potrf_batched_reg(hipStream_t s, char uplo, const int cnt[3], const int grid[3], const int *nvec,
-                             const long *offs, const int *order, ST *A, double *logdet, int *info);
+hipError_t potrf_batched_reg(hipStream_t s, char uplo, const int *d_cnt, const int *h_cnt, const int grid[3],
+                             const int *nvec, const long *offs, const int *order, ST *A, double *logdet, int *info);
 template <typename ST> int potrf_batched_reg_blocks_per_cu(char uplo, int cls);
+// Resolves a device's copy of the no-allocation entry point's scratch array
+// (batched_noalloc.hip), once per device, with that device current.
+hipError_t noalloc_setup(int device);
 
 // GMRES basis helpers: right-hand side k owns V + (k nv + i) n, i < nv.
 hipError_t basis_dots(hipStream_t s, long n, long nrhs, int nv, int first, int nvec, const double *V,

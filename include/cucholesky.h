@@ -114,6 +114,26 @@ CUresult cuPotriBatchedRun(CUbatchedplan plan, CUdeviceptr A, CUdeviceptr logdet
 CUresult cuPotrfBatchedPlanDestroy(CUbatchedplan plan);
 
 /*
+ * cuPotrfBatchedNoAlloc: the batched factorisation + logdet of
+ * cuPotrfBatchedRun in one call that allocates nothing.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The sizes and offsets are
+ * DEVICE arrays: n (int32[count], each n_i <= 256; a matrix with n_i out of
+ * range gets info = -1 and logdet = NaN and is not touched) and offsets
+ * (int64[count], element offsets into A); logdet (double[count]) and info
+ * (int32[count]) are required.  count <= 65536 (a larger count is rejected
+ * with CUDA_ERROR_INVALID_VALUE and XERBLA parameter 4).  The call launches
+ * its kernels on `stream` and returns; its only working memory is a fixed
+ * array in the library's device image (the size-sorted order), shared by
+ * every call on the device and left holding the last call's contents: calls
+ * that overlap on one device (unordered streams) race.  The handle must have
+ * been created on the device that runs the call.
+ */
+CUresult cuPotrfBatchedNoAlloc(CULAPACKhandle handle, CBlasUplo uplo, char precision, size_t count,
+                               CUdeviceptr n, CUdeviceptr offsets, CUdeviceptr A, CUdeviceptr logdet,
+                               CUdeviceptr info, CUstream stream);
+
+/*
  * Batched SPD solve + log-determinant + inverse of `count` matrices of any
  * sizes, as one persistent kernel over 64 x 64 tiles of the whole batch.
  * Matrix i is n[i] x n[i] (column-major, ld n[i]; triangle uplo is read) at
