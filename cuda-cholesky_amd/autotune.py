@@ -6,19 +6,30 @@
   every order in the sweep, each candidate block size is set with
   cuLAPACKSetBlockSize and the factorisation timed (best of a few runs, fresh
   input each time).  The winner for each order covers all n up to it.
-* Persistent schedule of the batched factorisation (cuPotrfBatchedRun /
-  cuPotriBatchedRun): workgroups per CU, from 1 up to the kernel's occupancy,
-  for FP64 / FP32 storage with and without the inverse, timed on a batch of
-  uneven sizes (median of a few graph launches, HIP events on the launch
-  stream).
+* Persistent schedule of the batched factorisation + inverse
+  (cuPotriBatchedRun): workgroups per CU, from 1 up to the kernel's
+  occupancy, for FP64 / FP32 storage, timed on a batch of uneven sizes
+  (median of a few graph launches, HIP events on the launch stream).  The
+  factor-only runs (cuPotrfBatchedRun) take the register-resident kernels,
+  whose grid is their occupancy.
+* Tile choice of each GEMM kernel family: a launch takes the family's larger
+  tile when it has at least `min_tiles` of them.  Each candidate threshold is
+  set through $CUCHOLESKY_TILE_MIN_<family> (read by the library on every
+  launch) and the routines that use the family are timed at several orders;
+  the threshold with the least total time wins.  Families: D (double tile
+  kernel, 128 vs 64: cuDpotrf, cuDpotri), Z (complex, 64 vs 32: cuZpotrf),
+  S (FP32 kernel, 128 vs 64: cuSpotrf), H and W (16-bit kernel, 128 vs 64 and
+  256 vs 128: the FP16-product factor of cuDsposvx).  A threshold above any
+  launch's tile count means "always the smaller tile".
 
 The table goes to tuning.txt beside the library (or --out), where the library
 reads it at first use:
 
     potrf_nb <precision> <n_max> <nb>
     batched_per_cu <precision> <inverse 0|1> <workgroups per CU>
+    gemm_tile <family> <min_tiles>
 
-    python cuda-cholesky_amd/autotune.py [--quick] [--only nb|batched] [--out PATH]
+    python cuda-cholesky_amd/autotune.py [--quick] [--only nb|batched|tiles] [--out PATH]
 """
 import argparse
 import importlib.util
@@ -115,7 +126,7 @@ def tune_batched(torch, pkg, h, count, reps, per_cu_max=6):
     for prec, dtype in (("D", torch.float64), ("S", torch.float32)):
         src = A0.to(dtype)
         work = torch.empty_like(src)
-        for inverse in (False, True):
+        for inverse in (True,):
             times = {}
             for per_cu in range(1, per_cu_max + 1):
                 os.environ["CUCHOLESKY_BATCHED_PER_CU"] = str(per_cu)
@@ -144,12 +155,79 @@ def tune_batched(torch, pkg, h, count, reps, per_cu_max=6):
     return lines
 
 
+# family: (routine, orders, candidate thresholds); 1 << 30: never the larger tile
+TILE_SWEEP = {
+    "D": ("D", [4096, 8192, 16384], [60, 120, 240, 480, 1 << 30]),
+    "Z": ("Z", [2048, 4096, 8192], [60, 120, 240, 480, 1 << 30]),
+    "S": ("S", [4096, 8192, 16384], [60, 120, 240, 480, 1 << 30]),
+    "H": ("H", [8192, 16384], [120, 240, 480, 1 << 30]),
+    "W": ("H", [16384, 32768], [128, 256, 512, 1024, 1 << 30]),
+}
+TILE_QUICK = {
+    "D": ("D", [8192], [120, 240, 1 << 30]),
+    "W": ("H", [16384], [256, 512, 1 << 30]),
+}
+
+
+def time_routine(torch, pkg, h, routine, n, A0, B, reps):
+    """Best wall time of one factorisation (D, Z, S) or of the FP16-product
+    solve (H) of order n on a fresh copy of A0."""
+    best = float("inf")
+    for _ in range(reps):
+        A = A0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if routine == "H":
+            X = torch.zeros_like(B)
+            st, _, used, info = pkg.cuDsposvx(h, pkg.Lower, n, B.shape[0], pkg.ptr(A), n, pkg.ptr(B), n,
+                                              pkg.ptr(X), n, "H", "I")
+        elif routine == "D":
+            st, info = pkg.cuDpotrf(h, pkg.Lower, n, pkg.ptr(A), n)
+        elif routine == "Z":
+            st, info = pkg.cuZpotrf(h, pkg.Upper, n, pkg.ptr(A), n)
+        else:
+            st, info = pkg.cuSpotrf(h, pkg.Lower, n, pkg.ptr(A), n)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+        if st != 0 or info != 0:
+            raise RuntimeError(f"{routine} n={n}: status {st} info {info}")
+    return best
+
+
+def tune_tiles(torch, pkg, h, sweep, reps):
+    """gemm_tile lines: per family, the threshold with the least total time
+    over the family's orders."""
+    lines = []
+    for fam, (routine, orders, cands) in sweep.items():
+        dtype = {"D": torch.float64, "Z": torch.complex128, "S": torch.float32, "H": torch.float64}[routine]
+        total = {c: 0.0 for c in cands}
+        detail = []
+        for n in orders:
+            A0 = spd(torch, n, dtype, seed=n)
+            B = torch.randn(4, n, device="cuda", dtype=torch.float64) if routine == "H" else None
+            row = []
+            for c in cands:
+                os.environ[f"CUCHOLESKY_TILE_MIN_{fam}"] = str(c)
+                time_routine(torch, pkg, h, routine, n, A0, B, 1)   # workspace, warm-up
+                t = time_routine(torch, pkg, h, routine, n, A0, B, reps)
+                total[c] += t
+                row.append(f"{'never' if c >= 1 << 30 else c}:{t * 1e3:.2f}ms")
+            os.environ.pop(f"CUCHOLESKY_TILE_MIN_{fam}", None)
+            detail.append(f"n={n} " + " ".join(row))
+            del A0, B
+            torch.cuda.empty_cache()
+        best = min(total, key=total.get)
+        lines.append(f"gemm_tile {fam} {best}   # {'; '.join(detail)}")
+        print(lines[-1], flush=True)
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--quick", action="store_true", help="a reduced sweep (a minute or two)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.txt"))
-    ap.add_argument("--only", choices=["nb", "batched"], help="run one of the two sweeps")
+    ap.add_argument("--only", choices=["nb", "batched", "tiles"], help="run one of the sweeps")
     args = ap.parse_args()
     import torch
     pkg = package()
@@ -158,8 +236,9 @@ def main():
     sweep = QUICK if args.quick else SWEEP
     lines = ["# schedule parameters measured by autotune.py on " + torch.cuda.get_device_name(0),
              "# potrf_nb <precision> <n_max> <nb>   (seconds per candidate in the trailing comment)",
-             "# batched_per_cu <precision> <inverse> <workgroups per CU>"]
-    if args.only == "batched":
+             "# batched_per_cu <precision> <inverse> <workgroups per CU>",
+             "# gemm_tile <family> <min_tiles>   (milliseconds per order and candidate in the trailing comment)"]
+    if args.only in ("batched", "tiles"):
         sweep = {}
     for prec, (orders, cands) in sweep.items():
         dtype = {"D": torch.float64, "Z": torch.complex128, "S": torch.float32, "B": torch.float64}[prec]
@@ -179,14 +258,17 @@ def main():
             print(lines[-1], flush=True)
             del A0, B
             torch.cuda.empty_cache()
-    if args.only != "nb":
+    if args.only in (None, "batched"):
         lines += tune_batched(torch, pkg, h, 4000 if args.quick else 20000, args.reps + 2)
+    if args.only in (None, "tiles"):
+        lines += tune_tiles(torch, pkg, h, TILE_QUICK if args.quick else TILE_SWEEP, args.reps)
     h.close()
     if args.only is not None and os.path.exists(args.out):
-        # keep the entries of the sweep that did not run
-        keep = "batched_per_cu" if args.only == "nb" else "potrf_nb"
+        # keep the entries of the sweeps that did not run
+        ran = {"nb": "potrf_nb", "batched": "batched_per_cu", "tiles": "gemm_tile"}[args.only]
         with open(args.out) as f:
-            lines += [ln.rstrip("\n") for ln in f if ln.startswith(keep)]
+            lines += [ln.rstrip("\n") for ln in f
+                      if not ln.startswith("#") and not ln.startswith(ran) and ln.strip()]
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {args.out}", flush=True)

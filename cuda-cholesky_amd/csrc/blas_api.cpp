@@ -136,127 +136,7 @@ CUresult trsm_api(const char *name, CUBLAShandle handle, CBlasSide side, CBlasUp
   return CUDA_SUCCESS;
 }
 
-// ---- single and single complex on widened copies ----
-//
-// The operands are widened on the caller's stream into stream-ordered double /
-// double complex copies (hipMallocAsync, freed on the same stream once the
-// call's work is queued), the double / double-complex building block runs on
-// them, and the result is narrowed into the caller's output.  An operand the
-// routine does not read (A and B when alpha = 0, C when beta = 0) is not
-// widened.
-
-template <typename Hi>
-struct Wide {
-  hipStream_t s;
-  Hi *p = nullptr;
-  hipError_t alloc(size_t elems) {
-    return hipMallocAsync(reinterpret_cast<void **>(&p), (elems > 0 ? elems : 1) * sizeof(Hi), s);
-  }
-  ~Wide() {
-    if (p != nullptr) (void)hipFreeAsync(p, s);
-  }
-};
-
-template <typename T> int op_of(CBlasTranspose t) {
-  return hc::is_complex<T>::value ? hc::op_code((char)t) : (t == CBlasNoTrans ? hc::OP_N : hc::OP_T);
-}
-
-template <typename Lo, typename Hi>
-CUresult gemm_wide(const char *name, CUBLAShandle handle, CBlasTranspose ta, CBlasTranspose tb,
-                   size_t m, size_t n, size_t k, Hi alpha, CUdeviceptr A, size_t lda, CUdeviceptr B,
-                   size_t ldb, Hi beta, CUdeviceptr C, size_t ldc, CUdeviceptr D, size_t ldd,
-                   CUstream stream) {
-  if (CUresult r = gemm_check(name, handle, ta, tb, m, n, k, lda, ldb, ldc, ldd); r != CUDA_SUCCESS)
-    return r;
-  if (m == 0 || n == 0 || (C == D && (hc::is_zero(alpha) || k == 0) && hc::is_one(beta)))
-    return CUDA_SUCCESS;
-  hc::DeviceGuard g(handle->device);
-  HC_CHECK(g.status());
-  const long kk = hc::is_zero(alpha) ? 0 : (long)k;
-  const long rA = ta == CBlasNoTrans ? (long)m : kk, cA = ta == CBlasNoTrans ? kk : (long)m;
-  const long rB = tb == CBlasNoTrans ? kk : (long)n, cB = tb == CBlasNoTrans ? (long)n : kk;
-  const long lA = rA > 0 ? rA : 1, lB = rB > 0 ? rB : 1;
-  Wide<Hi> wa{stream}, wb{stream}, wc{stream};
-  HC_CHECK(wa.alloc((size_t)(lA * cA)));
-  HC_CHECK(wb.alloc((size_t)(lB * cB)));
-  HC_CHECK(wc.alloc(m * n));
-  HC_CHECK((hc::convert2d<Lo, Hi>(stream, rA, cA, dptr<const Lo>(A), (long)lda, wa.p, lA)));
-  HC_CHECK((hc::convert2d<Lo, Hi>(stream, rB, cB, dptr<const Lo>(B), (long)ldb, wb.p, lB)));
-  if (!hc::is_zero(beta))
-    HC_CHECK((hc::convert2d<Lo, Hi>(stream, (long)m, (long)n, dptr<const Lo>(C), (long)ldc, wc.p, (long)m)));
-  HC_CHECK((hc::gemm<Hi>(stream, op_of<Hi>(ta), op_of<Hi>(tb), (long)m, (long)n, kk, alpha, wa.p, lA, wb.p,
-                         lB, beta, wc.p, (long)m, wc.p, (long)m)));
-  HC_CHECK((hc::convert2d<Hi, Lo>(stream, (long)m, (long)n, wc.p, (long)m, dptr<Lo>(D), (long)ldd)));
-  return CUDA_SUCCESS;
-}
-
-template <typename Lo, typename Hi>
-CUresult herk_wide(const char *name, CUBLAShandle handle, CBlasUplo uplo, CBlasTranspose trans, size_t n,
-                   size_t k, double alpha, CUdeviceptr A, size_t lda, double beta, CUdeviceptr C,
-                   size_t ldc, CUstream stream) {
-  if (CUresult r = herk_check(name, hc::is_complex<Hi>::value, handle, uplo, trans, n, k, lda, ldc);
-      r != CUDA_SUCCESS)
-    return r;
-  if (n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return CUDA_SUCCESS;
-  hc::DeviceGuard g(handle->device);
-  HC_CHECK(g.status());
-  const long kk = alpha == 0.0 ? 0 : (long)k;
-  const long rA = trans == CBlasNoTrans ? (long)n : kk, cA = trans == CBlasNoTrans ? kk : (long)n;
-  const long lA = rA > 0 ? rA : 1;
-  Wide<Hi> wa{stream}, wc{stream};
-  HC_CHECK(wa.alloc((size_t)(lA * cA)));
-  HC_CHECK(wc.alloc(n * n));
-  HC_CHECK((hc::convert2d<Lo, Hi>(stream, rA, cA, dptr<const Lo>(A), (long)lda, wa.p, lA)));
-  // C whole: the triangle not referenced round-trips exactly
-  HC_CHECK((hc::convert2d<Lo, Hi>(stream, (long)n, (long)n, dptr<const Lo>(C), (long)ldc, wc.p, (long)n)));
-  HC_CHECK((hc::herk<Hi>(stream, (char)uplo, (char)trans, (long)n, kk, alpha, wa.p, lA, beta, wc.p, (long)n)));
-  HC_CHECK((hc::convert2d<Hi, Lo>(stream, (long)n, (long)n, wc.p, (long)n, dptr<Lo>(C), (long)ldc)));
-  return CUDA_SUCCESS;
-}
-
-// Triangular multiply (X given: X <- alpha op(A) B / alpha B op(A)) or solve
-// (X == 0: in place in B).
-template <typename Lo, typename Hi>
-CUresult tri_wide(const char *name, CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
-                  CBlasDiag diag, size_t m, size_t n, Hi alpha, CUdeviceptr A, size_t lda, CUdeviceptr B,
-                  size_t ldb, CUdeviceptr X, size_t ldx, bool solve, CUstream stream) {
-  if (CUresult r = tri_check(name, handle, side, uplo, trans, diag, m, n, lda, ldb, ldx, !solve);
-      r != CUDA_SUCCESS)
-    return r;
-  if (m == 0 || n == 0) return CUDA_SUCCESS;
-  hc::DeviceGuard g(handle->device);
-  HC_CHECK(g.status());
-  const long ka = side == CBlasLeft ? (long)m : (long)n;
-  const bool zero = hc::is_zero(alpha);  // result 0: neither A nor B read
-  Wide<Hi> wa{stream}, wb{stream}, wx{stream};
-  HC_CHECK(wa.alloc((size_t)(ka * ka)));
-  HC_CHECK(wb.alloc(m * n));
-  if (!zero) {
-    HC_CHECK((hc::convert2d<Lo, Hi>(stream, ka, ka, dptr<const Lo>(A), (long)lda, wa.p, ka)));
-    HC_CHECK((hc::convert2d<Lo, Hi>(stream, (long)m, (long)n, dptr<const Lo>(B), (long)ldb, wb.p, (long)m)));
-  }
-  Hi *out = wb.p;
-  const Hi z = hc::from_real<Hi>(0.0);
-  if (zero) {
-    HC_CHECK((hc::gemm<Hi>(stream, hc::OP_N, hc::OP_N, (long)m, (long)n, 0, z, nullptr, 1, nullptr, 1, z,
-                           nullptr, 1, wb.p, (long)m)));
-  } else if (solve) {
-    Wide<Hi> ws{stream};
-    HC_CHECK(ws.alloc((size_t)hc::trsm_ws_elems((char)side, (long)m, (long)n)));
-    HC_CHECK((hc::trsm<Hi>(stream, (char)side, (char)uplo, (char)trans, (char)diag, (long)m, (long)n, alpha,
-                           wa.p, ka, wb.p, (long)m, ws.p)));
-  } else {
-    HC_CHECK(wx.alloc(m * n));
-    HC_CHECK((hc::trmm<Hi>(stream, (char)side, (char)uplo, (char)trans, (char)diag, (long)m, (long)n, alpha,
-                           wa.p, ka, wb.p, (long)m, wx.p, (long)m)));
-    out = wx.p;
-  }
-  HC_CHECK((hc::convert2d<Hi, Lo>(stream, (long)m, (long)n, out, (long)m, dptr<Lo>(solve ? B : X),
-                                  (long)(solve ? ldb : ldx))));
-  return CUDA_SUCCESS;
-}
-
-hc::zd widen(cuComplex_t v) { return hc::make_zd((double)__real__ v, (double)__imag__ v); }
+hc::zf to_zf(cuComplex_t v) { return hc::make_zf(__real__ v, __imag__ v); }
 
 }  // namespace
 
@@ -342,43 +222,43 @@ CUresult cuZtrsm(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTrans
                       ldb, stream);
 }
 
-// Single precision triangular multiply and the single complex BLAS, on
-// widened copies (reference blas/strmm.c:374, blas/cgemm.c:226,
+// Single precision triangular multiply and the single complex BLAS on the
+// FP32 matrix cores (reference blas/strmm.c:374, blas/cgemm.c:226,
 // blas/cherk.c:190, blas/ctrmm.c:422, blas/ctrsm.c:248).
 CUresult cuStrmm2(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
                   CBlasDiag diag, size_t m, size_t n, float alpha, CUdeviceptr A, size_t lda,
                   CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx, CUstream stream) {
-  return tri_wide<float, double>("cuStrmm2", handle, side, uplo, trans, diag, m, n, (double)alpha, A, lda,
-                                 B, ldb, X, ldx, false, stream);
+  return trmm_api<float>("cuStrmm2", handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, X, ldx,
+                         stream);
 }
 
 CUresult cuCgemm2(CUBLAShandle handle, CBlasTranspose transA, CBlasTranspose transB, size_t m,
                   size_t n, size_t k, cuComplex_t alpha, CUdeviceptr A, size_t lda, CUdeviceptr B,
                   size_t ldb, cuComplex_t beta, CUdeviceptr C, size_t ldc, CUdeviceptr D, size_t ldd,
                   CUstream stream) {
-  return gemm_wide<hc::zf, zd>("cuCgemm2", handle, transA, transB, m, n, k, widen(alpha), A, lda, B, ldb,
-                               widen(beta), C, ldc, D, ldd, stream);
+  return gemm_api<hc::zf>("cuCgemm2", handle, transA, transB, m, n, k, to_zf(alpha), A, lda, B, ldb,
+                          to_zf(beta), C, ldc, D, ldd, stream);
 }
 
 CUresult cuCherk(CUBLAShandle handle, CBlasUplo uplo, CBlasTranspose trans, size_t n, size_t k,
                  float alpha, CUdeviceptr A, size_t lda, float beta, CUdeviceptr C, size_t ldc,
                  CUstream stream) {
-  return herk_wide<hc::zf, zd>("cuCherk", handle, uplo, trans, n, k, (double)alpha, A, lda, (double)beta,
-                               C, ldc, stream);
+  return herk_api<hc::zf>("cuCherk", handle, uplo, trans, n, k, (double)alpha, A, lda, (double)beta, C, ldc,
+                          stream);
 }
 
 CUresult cuCtrmm2(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
                   CBlasDiag diag, size_t m, size_t n, cuComplex_t alpha, CUdeviceptr A, size_t lda,
                   CUdeviceptr B, size_t ldb, CUdeviceptr X, size_t ldx, CUstream stream) {
-  return tri_wide<hc::zf, zd>("cuCtrmm2", handle, side, uplo, trans, diag, m, n, widen(alpha), A, lda, B,
-                              ldb, X, ldx, false, stream);
+  return trmm_api<hc::zf>("cuCtrmm2", handle, side, uplo, trans, diag, m, n, to_zf(alpha), A, lda, B, ldb, X,
+                          ldx, stream);
 }
 
 CUresult cuCtrsm(CUBLAShandle handle, CBlasSide side, CBlasUplo uplo, CBlasTranspose trans,
                  CBlasDiag diag, size_t m, size_t n, cuComplex_t alpha, CUdeviceptr A, size_t lda,
                  CUdeviceptr B, size_t ldb, CUstream stream) {
-  return tri_wide<hc::zf, zd>("cuCtrsm", handle, side, uplo, trans, diag, m, n, widen(alpha), A, lda, B,
-                              ldb, 0, 0, true, stream);
+  return trsm_api<hc::zf>("cuCtrsm", handle, side, uplo, trans, diag, m, n, to_zf(alpha), A, lda, B, ldb,
+                          stream);
 }
 
 }  // extern "C"

@@ -175,5 +175,15 @@ template hipError_t herk<float>(hipStream_t, char, char, long, long, double, con
                                 double, float *, long, const int *, bool);
 template hipError_t trsm<float>(hipStream_t, char, char, char, char, long, long, float,
                                 const float *, long, float *, long, float *);
+// single and single complex (FP32 matrix cores; diagonal blocks inverted in
+// double / double complex)
+template hipError_t trmm<float>(hipStream_t, char, char, char, char, long, long, float, const float *,
+                                long, const float *, long, float *, long);
+template hipError_t trmm<zf>(hipStream_t, char, char, char, char, long, long, zf, const zf *, long,
+                             const zf *, long, zf *, long);
+template hipError_t trsm<zf>(hipStream_t, char, char, char, char, long, long, zf, const zf *, long,
+                             zf *, long, zf *);
+template hipError_t herk<zf>(hipStream_t, char, char, long, long, double, const zf *, long, double,
+                             zf *, long, const int *, bool);
 
 }  // namespace hc

@@ -89,11 +89,12 @@ struct Lds {
 
 // Load the lower form of the n x n block at A: S(r, c) = A(r, c) (lower) or
 // conj(A(c, r)) (upper), r >= c.  Memory is read along its contiguous index.
-template <typename T, int NMAX, int THREADS>
-__device__ __forceinline__ void load_lower_form(T *S, bool lower, int n, const T *A, long lda) {
+template <typename T, int NMAX, int THREADS, typename ST = T>
+__device__ __forceinline__ void load_lower_form(T *S, bool lower, int n, const ST *A, long lda) {
   constexpr int LS = Lds<T, NMAX>::LS;
   // a: contiguous index in memory; the unreferenced triangle is read but not used
-  gather_block<NMAX, THREADS, 16>(n, A, lda, [&](int a, int b, T v) {
+  gather_block<NMAX, THREADS, 16>(n, A, lda, [&](int a, int b, ST x) {
+    const T v = convert<T>(x);
     if (lower) {
       if (a >= b) S[b * LS + a] = v;
     } else {
@@ -105,15 +106,15 @@ __device__ __forceinline__ void load_lower_form(T *S, bool lower, int n, const T
 
 // Store the lower form back into the caller's triangle (diagonal included
 // unless skip_diag).
-template <typename T, int NMAX, int THREADS>
-__device__ __forceinline__ void store_lower_form(const T *S, bool lower, int n, T *A, long lda, bool skip_diag) {
+template <typename T, int NMAX, int THREADS, typename ST = T>
+__device__ __forceinline__ void store_lower_form(const T *S, bool lower, int n, ST *A, long lda, bool skip_diag) {
   constexpr int LS = Lds<T, NMAX>::LS;
   for_block<NMAX, THREADS>(n, [&](int a, int b) {
     if (skip_diag && a == b) return;
     if (lower) {
-      if (a >= b) A[(long)b * lda + a] = S[b * LS + a];
+      if (a >= b) A[(long)b * lda + a] = convert<ST>(S[b * LS + a]);
     } else {
-      if (a <= b) A[(long)b * lda + a] = conjx(S[a * LS + b]);
+      if (a <= b) A[(long)b * lda + a] = convert<ST>(conjx(S[a * LS + b]));
     }
   });
 }
@@ -458,9 +459,10 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
 
 // Workgroup b inverts the diagonal block of order min(nb, ntot - b nb) at
 // (b nb, b nb) of A into the same place of B (one workgroup: nb = ntot).
-template <typename T, int NMAX>
+// T is the compute type, ST the storage type of A, B and W.
+template <typename T, typename ST, int NMAX>
 __global__ void __launch_bounds__(DIAG_THREADS)
-diag_trtri_kernel(int lower, int unit, int ntot, int nb, const T *A, long lda, T *B, long ldb, T *W,
+diag_trtri_kernel(int lower, int unit, int ntot, int nb, const ST *A, long lda, ST *B, long ldb, ST *W,
                   long ldw, const int *abort_flag) {
   if (abort_flag != nullptr && *abort_flag != 0) return;
   const long off = (long)blockIdx.x * nb;
@@ -471,7 +473,7 @@ diag_trtri_kernel(int lower, int unit, int ntot, int nb, const T *A, long lda, T
   __shared__ T S[NMAX * (NMAX + 1)];
   __shared__ T vec[2 * NMAX];
   __shared__ T dinv[NMAX];
-  load_lower_form<T, NMAX, DIAG_THREADS>(S, lower, n, A, lda);
+  load_lower_form<T, NMAX, DIAG_THREADS, ST>(S, lower, n, A, lda);
   __syncthreads();
   if (!unit) diag_reciprocals<T, NMAX>(S, dinv, n);
   __syncthreads();
@@ -480,23 +482,23 @@ diag_trtri_kernel(int lower, int unit, int ntot, int nb, const T *A, long lda, T
   __syncthreads();
   reg_to_lds<T, NMAX>(x, S, n);
   __syncthreads();
-  store_lower_form<T, NMAX, DIAG_THREADS>(S, lower, n, B, ldb, unit != 0);
-  if (W != nullptr) store_dense<T, NMAX, DIAG_THREADS>(S, false, n, unit != 0, W, ldw);
+  store_lower_form<T, NMAX, DIAG_THREADS, ST>(S, lower, n, B, ldb, unit != 0);
+  if (W != nullptr) store_dense<T, NMAX, DIAG_THREADS, ST>(S, false, n, unit != 0, W, ldw);
 }
 
 // Workgroup b squares the diagonal block of order min(nb, ntot - b nb) at
 // (b nb, b nb) of A into B + b bstride (ld ldb); one workgroup in place:
 // nb = ntot, B = A, ldb = lda.
-template <typename T, int NMAX>
+template <typename T, typename ST, int NMAX>
 __global__ void __launch_bounds__(LAUUM_THREADS)
-diag_lauum_kernel(int lower, int ntot, int nb, const T *A, long lda, T *B, long ldb, long bstride) {
+diag_lauum_kernel(int lower, int ntot, int nb, const ST *A, long lda, ST *B, long ldb, long bstride) {
   __shared__ T S[NMAX * (NMAX + 1)];
   const long off = (long)blockIdx.x * nb;
   const int n = (int)min((long)nb, (long)ntot - off);
-  load_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, A + off * lda + off, lda);
+  load_lower_form<T, NMAX, LAUUM_THREADS, ST>(S, lower, n, A + off * lda + off, lda);
   __syncthreads();
   lds_lauum<T, NMAX>(S, n);
-  store_lower_form<T, NMAX, LAUUM_THREADS>(S, lower, n, B + blockIdx.x * bstride, ldb, false);
+  store_lower_form<T, NMAX, LAUUM_THREADS, ST>(S, lower, n, B + blockIdx.x * bstride, ldb, false);
 }
 
 // D <- S on the triangle uplo (diagonal included) of an n x n block
@@ -615,6 +617,7 @@ __global__ void __launch_bounds__(256) logdet_final_kernel(const double *partial
 // Compute type of the diagonal-block kernels for a storage type.
 template <typename ST> struct Compute { using type = ST; };
 template <> struct Compute<float> { using type = double; };
+template <> struct Compute<zf> { using type = zd; };
 
 template <typename ST>
 hipError_t diag_potrf(hipStream_t s, char uplo, int n, ST *A, long lda, ST *W, long ldw, int *info,
@@ -628,47 +631,51 @@ hipError_t diag_potrf(hipStream_t s, char uplo, int n, ST *A, long lda, ST *W, l
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t diag_trtri(hipStream_t s, char uplo, char diag, int n, const T *A, long lda, T *B,
-                      long ldb, T *W, long ldw, const int *abort_flag) {
+template <typename ST>
+hipError_t diag_trtri(hipStream_t s, char uplo, char diag, int n, const ST *A, long lda, ST *B,
+                      long ldb, ST *W, long ldw, const int *abort_flag) {
+  using T = typename Compute<ST>::type;
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_trtri_kernel<T, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
+  hipLaunchKernelGGL((diag_trtri_kernel<T, ST, NMAX>), dim3(1), dim3(DIAG_THREADS), 0, s,
                      uplo == 'L' ? 1 : 0, diag == 'U' ? 1 : 0, n, n, A, lda, B, ldb, W, ldw,
                      abort_flag);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t diag_trtri_blocks(hipStream_t s, char uplo, char diag, long n, int nb, T *A, long lda) {
+template <typename ST>
+hipError_t diag_trtri_blocks(hipStream_t s, char uplo, char diag, long n, int nb, ST *A, long lda) {
+  using T = typename Compute<ST>::type;
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (nb <= 0 || nb > NMAX || n > (long)INT32_MAX) return hipErrorInvalidValue;
   const long blocks = (n + nb - 1) / nb;
-  hipLaunchKernelGGL((diag_trtri_kernel<T, NMAX>), dim3((unsigned)blocks), dim3(DIAG_THREADS), 0, s,
+  hipLaunchKernelGGL((diag_trtri_kernel<T, ST, NMAX>), dim3((unsigned)blocks), dim3(DIAG_THREADS), 0, s,
                      uplo == 'L' ? 1 : 0, diag == 'U' ? 1 : 0, (int)n, nb, A, lda, A, lda,
-                     static_cast<T *>(nullptr), 0L, static_cast<const int *>(nullptr));
+                     static_cast<ST *>(nullptr), 0L, static_cast<const int *>(nullptr));
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t diag_lauum(hipStream_t s, char uplo, int n, T *A, long lda) {
+template <typename ST>
+hipError_t diag_lauum(hipStream_t s, char uplo, int n, ST *A, long lda) {
+  using T = typename Compute<ST>::type;
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (n > NMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3(1), dim3(LAUUM_THREADS), 0, s,
-                     uplo == 'L' ? 1 : 0, n, n, static_cast<const T *>(A), lda, A, lda, 0L);
+  hipLaunchKernelGGL((diag_lauum_kernel<T, ST, NMAX>), dim3(1), dim3(LAUUM_THREADS), 0, s,
+                     uplo == 'L' ? 1 : 0, n, n, static_cast<const ST *>(A), lda, A, lda, 0L);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t diag_lauum_blocks(hipStream_t s, char uplo, long n, int nb, const T *A, long lda, T *W) {
+template <typename ST>
+hipError_t diag_lauum_blocks(hipStream_t s, char uplo, long n, int nb, const ST *A, long lda, ST *W) {
+  using T = typename Compute<ST>::type;
   constexpr int NMAX = diag_max<T>();
   if (n <= 0) return hipSuccess;
   if (nb <= 0 || nb > NMAX || n > (long)INT32_MAX) return hipErrorInvalidValue;
   const long blocks = (n + nb - 1) / nb;
-  hipLaunchKernelGGL((diag_lauum_kernel<T, NMAX>), dim3((unsigned)blocks), dim3(LAUUM_THREADS), 0, s,
+  hipLaunchKernelGGL((diag_lauum_kernel<T, ST, NMAX>), dim3((unsigned)blocks), dim3(LAUUM_THREADS), 0, s,
                      uplo == 'L' ? 1 : 0, (int)n, nb, A, lda, W, (long)nb, (long)nb * nb);
   return hipGetLastError();
 }
@@ -727,16 +734,13 @@ hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials
                                             const T *, long, T *);                              \
   template hipError_t diag_zero_scan<T>(hipStream_t, char, long, const T *, long, int *);       \
   template hipError_t logdet<T>(hipStream_t, const T *, long, long, double *, double *);
+// single / single-complex storage: computed in double / double complex
+// (the mixed-precision factorisation and the s / c routines); the
+// log-determinants take their logs in FP64
 HC_INST(double)
 HC_INST(zd)
+HC_INST(float)
+HC_INST(zf)
 #undef HC_INST
-// log-determinants of single / single-complex data (logs taken in FP64)
-template hipError_t logdet<float>(hipStream_t, const float *, long, long, double *, double *);
-template hipError_t logdet<zf>(hipStream_t, const zf *, long, long, double *, double *);
-// FP32 storage, FP64 computation (mixed-precision factorisation and solves)
-template hipError_t diag_potrf<float>(hipStream_t, char, int, float *, long, float *, long, int *,
-                                      long);
-template hipError_t trsm_diag_inverses<float>(hipStream_t, char, char, char, long, int,
-                                              const float *, long, float *);
 
 }  // namespace hc

@@ -1,4 +1,6 @@
-// Tile GEMM on CDNA4 matrix cores, double and double complex.
+// Tile GEMM on CDNA4 matrix cores: double and double complex on the FP64
+// matrix cores (v_mfma_f64_16x16x4_f64), single and single complex on the
+// FP32 ones (v_mfma_f32_16x16x4_f32, twice the rate).
 //
 // D <- alpha * op(A) * op(B) + beta * C, column-major, op in {N, T, C}.  One
 // kernel serves every BLAS-3 shape on the Cholesky path:
@@ -26,6 +28,41 @@ namespace hc {
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Matrix-core arithmetic per element type: the real type R (LDS planes and
+// fragments), the accumulator V and the 16 x 16 x 4 product.  The FP64 and
+// FP32 instructions take the same operand fragments (lane l: row l % 16,
+// k = l / 16) but return the 16 x 16 result in different lane orders:
+// register r of lane group g holds result row g + 4 r (FP64) or 4 g + r
+// (FP32), column l % 16.
+template <typename T> struct Prec;
+template <> struct Prec<double> {
+  using R = double;
+  using V = v4d;
+  static constexpr bool F32 = false;
+};
+template <> struct Prec<zd> : Prec<double> {};
+template <> struct Prec<float> {
+  using R = float;
+  using V = v4f;
+  static constexpr bool F32 = true;
+};
+template <> struct Prec<zf> : Prec<float> {};
+
+__device__ __forceinline__ v4d mma(double b, double a, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mma(float b, float a, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c, 0, 0, 0);
+}
+// result row (the D column here) of register r in lane group g
+template <typename T>
+__device__ __forceinline__ int res_row(int g, int r) { return Prec<T>::F32 ? 4 * g + r : g + 4 * r; }
+
+template <typename T> __device__ __forceinline__ T cmake(typename Prec<T>::R re, typename Prec<T>::R im);
+template <> __device__ __forceinline__ zd cmake<zd>(double re, double im) { return make_zd(re, im); }
+template <> __device__ __forceinline__ zf cmake<zf>(float re, float im) { return make_zf(re, im); }
 
 constexpr int KT = 16;         // K step
 constexpr int NTHREADS = 256;  // 4 waves
@@ -77,6 +114,14 @@ __device__ __forceinline__ void lds_put(double *s, int plane_stride, int idx, zd
   s[idx] = v.x;
   s[plane_stride + idx] = v.y;
 }
+__device__ __forceinline__ void lds_put(float *s, int plane_stride, int idx, float v) {
+  (void)plane_stride;
+  s[idx] = v;
+}
+__device__ __forceinline__ void lds_put(float *s, int plane_stride, int idx, zf v) {
+  s[idx] = v.x;
+  s[plane_stride + idx] = v.y;
+}
 
 // Per-thread global-load pattern for one operand tile of R x KT elements
 // (R = MT for op(A), NT for op(B)).  With the R index contiguous in memory
@@ -115,9 +160,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
 
   constexpr bool CPLX = is_complex<T>::value;
+  using R = typename Prec<T>::R;
+  using V = typename Prec<T>::V;
   constexpr int P = CPLX ? 2 : 1;
   constexpr int LSA = MT + PAD, LSB = NT + PAD;
-  constexpr int PSA = KT * LSA, PSB = KT * LSB;  // plane strides (doubles)
+  constexpr int PSA = KT * LSA, PSB = KT * LSB;  // plane strides (elements of R)
   constexpr int BUFA = P * PSA, BUFB = P * PSB;  // one LDS buffer
   // 2 x 2 waves; a 16-wide tile (skinny right-hand sides) uses 4 x 1
   constexpr int WAVES_N = NT >= 32 ? 2 : 1, WAVES_M = 4 / WAVES_N;
@@ -127,8 +174,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   constexpr int EA = PA::E, EB = PB::E;
   static_assert(EA >= 1 && EB >= 1 && BI >= 1 && BJ >= 1, "tile too small");
 
-  __shared__ double sA[2 * BUFA];
-  __shared__ double sB[2 * BUFB];
+  __shared__ R sA[2 * BUFA];
+  __shared__ R sB[2 * BUFB];
 
   // ---- tile coordinates ----
   // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2):
@@ -203,13 +250,13 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   const int lc = lane & 15, lg = lane >> 4;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
 
-  v4d acc[P][BI][BJ];
+  V acc[P][BI][BJ];
 #pragma unroll
   for (int q = 0; q < P; q++)
 #pragma unroll
     for (int i = 0; i < BI; i++)
 #pragma unroll
-      for (int j = 0; j < BJ; j++) acc[q][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < BJ; j++) acc[q][i][j] = (V){0, 0, 0, 0};
 
   // Real data with alpha = +-1 and beta != 0: the accumulators start from
   // (beta / alpha) C (exact scaling), read from clamped addresses right after
@@ -219,7 +266,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   // split K: there each split writes its raw partial sum and C enters once,
   // in the reduction.
   int npre = 0;
-  double cfac = 0.0;
+  R cfac = 0;
   if constexpr (!CPLX) {
     if (!is_zero(p.beta) && (p.alpha == 1.0 || p.alpha == -1.0) && p.part == nullptr) {
       cfac = p.beta / p.alpha;
@@ -236,7 +283,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
           for (int j = 0; j < BJ; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-              const int gn = min(n0 + wn0 + 16 * j + lg + 4 * r, p.n - 1);
+              const int gn = min(n0 + wn0 + 16 * j + res_row<T>(lg, r), p.n - 1);
               acc[0][i][j][r] = cfac * p.C[(long)gn * p.ldc + gm];
             }
         }
@@ -333,11 +380,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
         pb += b_kstride;
         load((kt + 1) * KT, pa, pb);
       }
-      const double *cA = sA + (kt & 1) * BUFA, *cB = sB + (kt & 1) * BUFB;
+      const R *cA = sA + (kt & 1) * BUFA, *cB = sB + (kt & 1) * BUFB;
 #pragma unroll
       for (int ks = 0; ks < KT; ks += 4) {
         const int kr = ks + lg;
-        double ar[BI], br[BJ];
+        R ar[BI], br[BJ];
 #pragma unroll
         for (int i = 0; i < BI; i++) ar[i] = cA[kr * LSA + wm0 + 16 * i + lc];
 #pragma unroll
@@ -347,9 +394,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
           for (int i = 0; i < BI; i++)
 #pragma unroll
             for (int j = 0; j < BJ; j++)
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ar[i], acc[0][i][j], 0, 0, 0);
+              acc[0][i][j] = mma(br[j], ar[i], acc[0][i][j]);
         } else {
-          double ai[BI], bi[BJ];
+          R ai[BI], bi[BJ];
 #pragma unroll
           for (int i = 0; i < BI; i++) ai[i] = cA[PSA + kr * LSA + wm0 + 16 * i + lc];
 #pragma unroll
@@ -358,10 +405,10 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
           for (int i = 0; i < BI; i++)
 #pragma unroll
             for (int j = 0; j < BJ; j++) {
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ar[i], acc[0][i][j], 0, 0, 0);
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bi[j], ai[i], acc[0][i][j], 0, 0, 0);
-              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[j], ar[i], acc[1][i][j], 0, 0, 0);
-              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ai[i], acc[1][i][j], 0, 0, 0);
+              acc[0][i][j] = mma(br[j], ar[i], acc[0][i][j]);
+              acc[0][i][j] = mma(-bi[j], ai[i], acc[0][i][j]);
+              acc[1][i][j] = mma(bi[j], ar[i], acc[1][i][j]);
+              acc[1][i][j] = mma(br[j], ai[i], acc[1][i][j]);
             }
         }
       }
@@ -375,8 +422,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   }
 
   // ---- epilogue: lane lc holds row m; register r of lane group lg holds
-  // column lg + 4 r (the 16x16x4 f64 result layout: block row lane/16 + 4 r,
-  // block column lane % 16, with rows of the block indexing N here) ----
+  // column res_row(lg, r) (the result rows of the 16 x 16 x 4 product index
+  // N here, its columns M) ----
   if (p.part != nullptr) {  // split K: raw partial sums
     T *part = p.part + (long)blockIdx.z * p.m * p.n;
 #pragma unroll
@@ -387,10 +434,10 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
       for (int j = 0; j < BJ; j++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int gn = n0 + wn0 + 16 * j + lg + 4 * r;
+          const int gn = n0 + wn0 + 16 * j + res_row<T>(lg, r);
           if (gn >= p.n) continue;
           T v;
-          if constexpr (CPLX) v = make_zd(acc[0][i][j][r], acc[1][i][j][r]);
+          if constexpr (CPLX) v = cmake<T>(acc[0][i][j][r], acc[1][i][j][r]);
           else v = acc[0][i][j][r];
           part[(long)gn * p.m + gm] = v;
         }
@@ -410,7 +457,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
       for (int j = 0; j < BJ; j++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int gnc = min(n0 + wn0 + 16 * j + lg + 4 * r, p.n - 1);
+          const int gnc = min(n0 + wn0 + 16 * j + res_row<T>(lg, r), p.n - 1);
           cv[j][r] = p.C[(long)gnc * p.ldc + gmc];
         }
     }
@@ -420,12 +467,12 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
     for (int j = 0; j < BJ; j++) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gn = n0 + wn0 + 16 * j + lg + 4 * r;
+        const int gn = n0 + wn0 + 16 * j + res_row<T>(lg, r);
         if (gn >= p.n) continue;
         if (p.tri_out == TRI_LOWER && gm < gn) continue;
         if (p.tri_out == TRI_UPPER && gm > gn) continue;
         T v;
-        if constexpr (CPLX) v = make_zd(acc[0][i][j][r], acc[1][i][j][r]);
+        if constexpr (CPLX) v = cmake<T>(acc[0][i][j][r], acc[1][i][j][r]);
         else v = acc[0][i][j][r];
         v = p.alpha * v;
         if (need_c && i * BJ + j >= npre) v = v + p.beta * cv[j][r];
@@ -524,42 +571,14 @@ __global__ void copy2d_kernel(int m, int n, const T *S, long lds, T *D, long ldd
   if (i < m && j < n) D[(long)j * ldd + i] = S[(long)j * lds + i];
 }
 
-__device__ __forceinline__ double cvt(float v, double *) { return (double)v; }
-__device__ __forceinline__ float cvt(double v, float *) { return (float)v; }
-__device__ __forceinline__ zd cvt(zf v, zd *) { return make_zd((double)v.x, (double)v.y); }
-__device__ __forceinline__ zf cvt(zd v, zf *) {
-  zf r;
-  r.x = (float)v.x;
-  r.y = (float)v.y;
-  return r;
-}
-
-template <typename S, typename D>
-__global__ void convert2d_kernel(int m, long n, const S *src, long lds, D *dst, long ldd) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (long j = blockIdx.y; j < n; j += gridDim.y)
-    dst[j * ldd + i] = cvt(src[j * lds + i], static_cast<D *>(nullptr));
-}
 
 }  // namespace
 
-template <typename S, typename D>
-hipError_t convert2d(hipStream_t s, long m, long n, const S *src, long lds, D *dst, long ldd) {
-  if (m <= 0 || n <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((m + 255) / 256), (unsigned)(n < 8192 ? n : 8192));
-  hipLaunchKernelGGL((convert2d_kernel<S, D>), grid, dim3(256), 0, s, (int)m, n, src, lds, dst, ldd);
-  return hipGetLastError();
-}
-template hipError_t convert2d<float, double>(hipStream_t, long, long, const float *, long, double *, long);
-template hipError_t convert2d<double, float>(hipStream_t, long, long, const double *, long, float *, long);
-template hipError_t convert2d<zf, zd>(hipStream_t, long, long, const zf *, long, zd *, long);
-template hipError_t convert2d<zd, zf>(hipStream_t, long, long, const zd *, long, zf *, long);
 
 template <typename T>
-hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
-                long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
-                const GemmOpts &o) {
+hipError_t gemm_tile(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
+                     long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
+                     const GemmOpts &o) {
   if (m <= 0 || n <= 0) return hipSuccess;
   KParams<T> p;
   p.m = (int)m; p.n = (int)n; p.k = (int)(k > 0 ? k : 0);
@@ -571,14 +590,15 @@ hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, 
   p.kchunk = 0;
   p.part = nullptr;
 
-  // Tile choice: the large tile when it still yields about one workgroup per
-  // CU (256 CUs), otherwise the small one; in-place callers may demand a tile
-  // that spans a whole panel in M or N.
+  // Tile choice: the large tile when the launch has at least the measured
+  // number of them (family D / Z of the tuning table; about one workgroup per
+  // CU by default), otherwise the small one; in-place callers may demand a
+  // tile that spans a whole panel in M or N.
   constexpr bool CPLX = is_complex<T>::value;
   constexpr int BIG = CPLX ? 64 : 128, SMALL = CPLX ? 32 : 64;
   const long tb_m = (m + BIG - 1) / BIG, tb_n = (n + BIG - 1) / BIG;
   const long tiles_big = o.tri_grid ? tb_m * (tb_m + 1) / 2 : tb_m * tb_n;
-  int mt = tiles_big >= 240 ? BIG : SMALL, nt = mt;
+  int mt = tiles_big >= tuned_tile_min(CPLX ? 'Z' : 'D', 240) ? BIG : SMALL, nt = mt;
   // real data with at most 16 columns and plenty of row tiles: 128 x 16 tiles
   if (!CPLX && tb == OP_N && n <= 16 && o.tri_b == TRI_NONE && o.min_nt <= 16 && !o.tri_grid &&
       (m + 127) / 128 >= 64) {
@@ -616,13 +636,26 @@ hipError_t copy2d(hipStream_t s, long m, long n, const T *S, long lds, T *D, lon
   return hipGetLastError();
 }
 
-template hipError_t gemm<double>(hipStream_t, int, int, long, long, long, double, const double *,
-                                 long, const double *, long, double, const double *, long,
-                                 double *, long, const GemmOpts &);
-template hipError_t gemm<zd>(hipStream_t, int, int, long, long, long, zd, const zd *, long,
-                             const zd *, long, zd, const zd *, long, zd *, long,
-                             const GemmOpts &);
+// gemm<T> is this kernel for double, double complex and single complex; for
+// single data gemm<float> (gemm_lp.hip) runs its own FP32 kernel and hands
+// triangular operands to gemm_tile<float>.
+template <typename T>
+hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A, long lda,
+                const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd, const GemmOpts &o) {
+  return gemm_tile<T>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, D, ldd, o);
+}
+
+#define HC_GEMM_INST(F, T)                                                                          \
+  template hipError_t F<T>(hipStream_t, int, int, long, long, long, T, const T *, long, const T *, \
+                           long, T, const T *, long, T *, long, const GemmOpts &);
+HC_GEMM_INST(gemm, double)
+HC_GEMM_INST(gemm, zd)
+HC_GEMM_INST(gemm, zf)
+HC_GEMM_INST(gemm_tile, float)
+#undef HC_GEMM_INST
 template hipError_t copy2d<double>(hipStream_t, long, long, const double *, long, double *, long);
 template hipError_t copy2d<zd>(hipStream_t, long, long, const zd *, long, zd *, long);
+template hipError_t copy2d<float>(hipStream_t, long, long, const float *, long, float *, long);
+template hipError_t copy2d<zf>(hipStream_t, long, long, const zf *, long, zf *, long);
 
 }  // namespace hc

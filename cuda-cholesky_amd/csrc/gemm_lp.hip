@@ -352,22 +352,14 @@ struct Bf16Params {
 
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
-// F16: the 16-bit copies are IEEE binary16 (v_mfma_f32_16x16x32_f16) instead
-// of bfloat16; the two formats share the layout and the sign bit.
-template <int MT, int NT, bool TG, bool F16>
-__global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p) {
-  if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
-  constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
-  constexpr int CA = MT * BK / 8 / NTHREADS;  // 16-byte chunks per thread
-  constexpr int CB = NT * BK / 8 / NTHREADS;
-  static_assert(CA >= 1 && CB >= 1, "tile too small");
-  __shared__ __attribute__((aligned(16))) u16 sA[2][MT * BLK];
-  __shared__ __attribute__((aligned(16))) u16 sB[2][NT * BLK];
-
+// Tile (bm, bn) of this workgroup: XCD-contiguous, in groups of GROUP tile
+// rows; on a triangular grid (TG) the lower (tri_out TRI_LOWER) or upper
+// triangle of tiles, numbered row by row.
+template <bool TG>
+__device__ __forceinline__ void tile16_coords(int tri_out, int &bm, int &bn) {
   const int nwg = TG ? (int)gridDim.x : (int)(gridDim.x * gridDim.y);
   const int wid = TG ? (int)blockIdx.x : (int)(blockIdx.x + blockIdx.y * gridDim.x);
   const int logical = xcd_contiguous(wid, nwg);
-  int bm, bn;
   if constexpr (TG) {
     const int id = logical;
     int r = (int)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
@@ -389,7 +381,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
       tc = first + cl;
       tr = tc + o;
     }
-    if (p.tri_out == TRI_LOWER) { bm = tr; bn = tc; } else { bm = tc; bn = tr; }
+    if (tri_out == TRI_LOWER) { bm = tr; bn = tc; } else { bm = tc; bn = tr; }
   } else {
     const int gx = (int)gridDim.x, gy = (int)gridDim.y;
     const int per = GROUP * gy, first = (logical / per) * GROUP, gs = min(GROUP, gx - first);
@@ -397,6 +389,22 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
     bm = first + o % gs;
     bn = o / gs;
   }
+}
+
+// F16: the 16-bit copies are IEEE binary16 (v_mfma_f32_16x16x32_f16) instead
+// of bfloat16; the two formats share the layout and the sign bit.
+template <int MT, int NT, bool TG, bool F16>
+__global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p) {
+  if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+  constexpr int WM = MT / 2, WN = NT / 2, BI = WM / 16, BJ = WN / 16;
+  constexpr int CA = MT * BK / 8 / NTHREADS;  // 16-byte chunks per thread
+  constexpr int CB = NT * BK / 8 / NTHREADS;
+  static_assert(CA >= 1 && CB >= 1, "tile too small");
+  __shared__ __attribute__((aligned(16))) u16 sA[2][MT * BLK];
+  __shared__ __attribute__((aligned(16))) u16 sB[2][NT * BLK];
+
+  int bm, bn;
+  tile16_coords<TG>(p.tri_out, bm, bn);
   const int m0 = bm * MT, n0 = bn * NT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lc = lane & 15, lg = lane >> 4;
@@ -551,6 +559,160 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_bf16_nt_kernel(Bf16Params p)
   }
 }
 
+// The same product on 256 x 256 tiles for large updates: 8 waves (a 2 x 4
+// grid, each wave 128 x 64 of D), so each staged operand byte feeds twice the
+// products of the 128 x 128 kernel (half the L2 traffic per flop, and 12
+// fragment reads per 32 products per wave instead of 8 per 16).  LDS holds
+// two K steps of both operands (2 x 2 x 256 x BLK x 2 bytes = 144 KiB: one
+// workgroup per CU); one register set prefetches the next K step while the
+// current one is computed.
+constexpr int T256_THREADS = 512;
+
+template <bool TG, bool F16>
+__global__ void __launch_bounds__(T256_THREADS, 1) gemm16_nt_256_kernel(Bf16Params p) {
+  if (p.abort_flag != nullptr && *p.abort_flag != 0) return;
+  constexpr int MT = 256, NT = 256, WGM = 2;
+  constexpr int WM = MT / WGM, WN = NT / (T256_THREADS / 64 / WGM), BI = WM / 16, BJ = WN / 16;
+  constexpr int CA = MT * BK / 8 / T256_THREADS;  // 16-byte chunks per thread
+  constexpr int CB = NT * BK / 8 / T256_THREADS;
+  static_assert(BI == 8 && BJ == 4 && CA == 4 && CB == 4, "256 x 256 tile, 8 waves of 128 x 64");
+  __shared__ __attribute__((aligned(16))) u16 sA[2][MT * BLK];
+  __shared__ __attribute__((aligned(16))) u16 sB[2][NT * BLK];
+
+  int bm, bn;
+  tile16_coords<TG>(p.tri_out, bm, bn);
+  const int m0 = bm * MT, n0 = bn * NT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lc = lane & 15, lg = lane >> 4;
+  const int wm0 = (wave % WGM) * WM, wn0 = (wave / WGM) * WN;
+
+  // accumulators seeded with (beta / |alpha|) C, sign of alpha in the A
+  // fragments, |alpha| applied at the end (a power of two: exact), as in
+  // gemm_bf16_nt_kernel
+  const float aabs = fabsf(p.alpha);
+  const bool use_beta = p.beta != 0.0f;
+  v4f acc[BI][BJ];
+  auto seed_acc = [&]() __attribute__((always_inline)) {
+    if (use_beta) {
+      const float cs = p.beta / aabs;
+#pragma unroll
+      for (int i = 0; i < BI; i++) {
+        const int gm = min(m0 + wm0 + 16 * i + lc, p.m - 1);
+#pragma unroll
+        for (int j = 0; j < BJ; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int gn = min(n0 + wn0 + 16 * j + 4 * lg + r, p.n - 1);
+            acc[i][j][r] = cs * p.C[(long)gn * p.ldc + gm];
+          }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < BI; i++)
+#pragma unroll
+        for (int j = 0; j < BJ; j++) acc[i][j] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+    }
+  };
+  const unsigned sgn = p.alpha < 0.0f ? 0x80008000u : 0u;
+
+  const int kend = p.k;
+  if (kend > 0) {
+    v4u ra[CA], rb[CB];
+    auto load = [&](int kb) __attribute__((always_inline)) {
+#pragma unroll
+      for (int q = 0; q < CA; q++) {
+        const int e = tid + q * T256_THREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        const int gr = m0 + row;
+        const v4u *src = reinterpret_cast<const v4u *>(p.A + (long)min(gr, p.m - 1) * p.lda + kb + kc);
+        const unsigned msk = gr < p.m ? 0xffffffffu : 0u;  // rows past m read as zero
+        ra[q] = *src & (v4u){msk, msk, msk, msk};
+      }
+#pragma unroll
+      for (int q = 0; q < CB; q++) {
+        const int e = tid + q * T256_THREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        const int gc = n0 + row;
+        const v4u *src = reinterpret_cast<const v4u *>(p.B + (long)min(gc, p.n - 1) * p.ldb + kb + kc);
+        const unsigned msk = gc < p.n ? 0xffffffffu : 0u;
+        rb[q] = *src & (v4u){msk, msk, msk, msk};
+      }
+    };
+    auto store = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+      for (int q = 0; q < CA; q++) {
+        const int e = tid + q * T256_THREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        *reinterpret_cast<v4u *>(&sA[buf][row * BLK + kc]) = ra[q];
+      }
+#pragma unroll
+      for (int q = 0; q < CB; q++) {
+        const int e = tid + q * T256_THREADS, row = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        *reinterpret_cast<v4u *>(&sB[buf][row * BLK + kc]) = rb[q];
+      }
+    };
+    auto compute = [&](int buf) __attribute__((always_inline)) {
+      const u16 *cA = sA[buf], *cB = sB[buf];
+#pragma unroll
+      for (int h = 0; h < BK; h += 32) {
+        v8bf fb[BJ];
+#pragma unroll
+        for (int j = 0; j < BJ; j++)
+          fb[j] = *reinterpret_cast<const v8bf *>(&cB[(wn0 + 16 * j + lc) * BLK + h + 8 * lg]);
+#pragma unroll
+        for (int i = 0; i < BI; i++) {
+          v4u x = *reinterpret_cast<const v4u *>(&cA[(wm0 + 16 * i + lc) * BLK + h + 8 * lg]);
+          x ^= (v4u){sgn, sgn, sgn, sgn};
+          const v8bf fa = __builtin_bit_cast(v8bf, x);
+#pragma unroll
+          for (int j = 0; j < BJ; j++) {
+            if constexpr (F16)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, fb[j]),
+                                                                 __builtin_bit_cast(v8h, fa), acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa, acc[i][j], 0, 0, 0);
+          }
+        }
+      }
+    };
+    const int ntiles = kend / BK;
+    load(0);
+    seed_acc();
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < ntiles; kt++) {
+      // buffer (kt + 1) & 1 was last read in step kt - 1, before the barrier
+      const bool more = kt + 1 < ntiles;
+      if (more) load((kt + 1) * BK);
+      compute(kt & 1);
+      if (more) store((kt + 1) & 1);
+      __syncthreads();
+    }
+  } else {
+    seed_acc();
+  }
+
+  if (aabs != 1.0f) {
+#pragma unroll
+    for (int i = 0; i < BI; i++)
+#pragma unroll
+      for (int j = 0; j < BJ; j++) acc[i][j] *= aabs;
+  }
+#pragma unroll
+  for (int i = 0; i < BI; i++) {
+    const int gm = m0 + wm0 + 16 * i + lc;
+    if (gm >= p.m) continue;
+#pragma unroll
+    for (int j = 0; j < BJ; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gn = n0 + wn0 + 16 * j + 4 * lg + r;
+        if (gn >= p.n) continue;
+        if (p.tri_out == TRI_LOWER && gm < gn) continue;
+        if (p.tri_out == TRI_UPPER && gm > gn) continue;
+        p.D[(long)gn * p.ldd + gm] = acc[i][j][r];
+      }
+    }
+  }
+}
+
 // T (bf16, row r at T + r * ldt, k contiguous) <- P (FP32, column-major, ld
 // ldp) for rows 0..m-1 and k < kp, zeros for kp <= k < ldt.
 __global__ void panel_to_bf16_kernel(int m, int kp, const float *P, long ldp, u16 *T, long ldt) {
@@ -660,11 +822,27 @@ hipError_t panel_to_f16(hipStream_t s, long m, long kp, const float *P, long ldp
   return hipGetLastError();
 }
 
+// Tile choice: 256 x 256 (one 8-wave workgroup per CU) when the grid has at
+// least the measured number of such tiles (family W; by default two per CU),
+// 128 x 128 when it has enough of those (family H; about one per CU), else
+// 64 x 64.
 template <bool F16>
 hipError_t launch_nt16(hipStream_t s, const Bf16Params &p, long m, long n, bool tri_grid) {
   const long tm = (m + 127) / 128, tn = (n + 127) / 128;
   const long tiles = tri_grid ? tm * (tm + 1) / 2 : tm * tn;
-  const bool big = tiles >= 240;
+  const long um = (m + 255) / 256, un = (n + 255) / 256;
+  const long tiles256 = tri_grid ? um * (um + 1) / 2 : um * un;
+  if (tiles256 >= tuned_tile_min('W', 512)) {
+    if (tri_grid) {
+      if (m != n) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((gemm16_nt_256_kernel<true, F16>), dim3((unsigned)tiles256), dim3(T256_THREADS), 0, s, p);
+    } else {
+      hipLaunchKernelGGL((gemm16_nt_256_kernel<false, F16>), dim3((unsigned)um, (unsigned)un), dim3(T256_THREADS),
+                         0, s, p);
+    }
+    return hipGetLastError();
+  }
+  const bool big = tiles >= tuned_tile_min('H', 240);
   if (tri_grid) {
     if (m != n) return hipErrorInvalidValue;
     if (big) {
@@ -712,7 +890,8 @@ hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, fl
                        const float *A, long lda, const float *B, long ldb, float beta,
                        const float *C, long ldc, float *D, long ldd, const GemmOpts &o) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (o.tri_a != TRI_NONE || o.tri_b != TRI_NONE) return hipErrorInvalidValue;
+  if (o.tri_a != TRI_NONE || o.tri_b != TRI_NONE)
+    return gemm_tile<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, D, ldd, o);
   LpParams p;
   p.m = (int)m; p.n = (int)n; p.k = (int)(k > 0 ? k : 0);
   p.alpha = alpha; p.beta = beta;
@@ -721,7 +900,7 @@ hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, fl
   p.abort_flag = o.abort_flag;
   const long tb_m = (m + 127) / 128, tb_n = (n + 127) / 128;
   const long tiles_big = o.tri_grid ? tb_m * (tb_m + 1) / 2 : tb_m * tb_n;
-  int mt = tiles_big >= 240 ? 128 : 64, nt = mt;
+  int mt = tiles_big >= tuned_tile_min('S', 240) ? 128 : 64, nt = mt;
   if (o.min_mt > mt) mt = 128;
   if (o.min_nt > nt) nt = 128;
   if (o.min_mt > 128 || o.min_nt > 128) return hipErrorInvalidValue;

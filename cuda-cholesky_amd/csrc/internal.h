@@ -52,12 +52,25 @@ __host__ __device__ inline zd operator/(zd a, zd b) {
   const double r = b.x / b.y, d = b.y + b.x * r;
   return make_zd((a.x * r + a.y) / d, (a.y * r - a.x) / d);
 }
-// Single complex, layout-compatible with C99 `float complex`: storage only
-// (the single-complex entry points compute in double complex).
+// Single complex, layout-compatible with C99 `float complex`.  The level-3
+// products compute in it (FP32 matrix cores); the diagonal-block kernels load
+// it, compute in double complex and store it back.
 struct __attribute__((aligned(8))) zf {
   float x, y;
 };
+__host__ __device__ inline zf make_zf(float re, float im) { zf r; r.x = re; r.y = im; return r; }
 __host__ __device__ inline double realx(zf a) { return a.x; }
+__host__ __device__ inline zf conjx(zf a) { return make_zf(a.x, -a.y); }
+__host__ __device__ inline zf operator+(zf a, zf b) { return make_zf(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline zf operator-(zf a, zf b) { return make_zf(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline zf operator-(zf a) { return make_zf(-a.x, -a.y); }
+__host__ __device__ inline zf operator*(zf a, zf b) {
+  return make_zf(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline zf operator*(float a, zf b) { return make_zf(a * b.x, a * b.y); }
+__host__ __device__ inline bool is_zero(zf a) { return a.x == 0.0f && a.y == 0.0f; }
+__host__ __device__ inline bool is_one(zf a) { return a.x == 1.0f && a.y == 0.0f; }
+__host__ __device__ inline bool is_one(float a) { return a == 1.0f; }
 
 __host__ __device__ inline float conjx(float a) { return a; }
 __host__ __device__ inline double realx(float a) { return a; }
@@ -71,14 +84,19 @@ template <typename T> __host__ __device__ inline T from_real(double v);
 template <> __host__ __device__ inline double from_real<double>(double v) { return v; }
 template <> __host__ __device__ inline zd from_real<zd>(double v) { return make_zd(v, 0.0); }
 template <> __host__ __device__ inline float from_real<float>(double v) { return (float)v; }
+template <> __host__ __device__ inline zf from_real<zf>(double v) { return make_zf((float)v, 0.0f); }
 
 // Storage <-> compute conversion (the mixed-precision path keeps FP32 data
 // and computes the diagonal blocks in FP64).
 template <typename D, typename S> __host__ __device__ inline D convert(S v) { return (D)v; }
 template <> __host__ __device__ inline zd convert<zd, zd>(zd v) { return v; }
+template <> __host__ __device__ inline zd convert<zd, zf>(zf v) { return make_zd((double)v.x, (double)v.y); }
+template <> __host__ __device__ inline zf convert<zf, zd>(zd v) { return make_zf((float)v.x, (float)v.y); }
+template <> __host__ __device__ inline zf convert<zf, zf>(zf v) { return v; }
 
 template <typename T> struct is_complex { static constexpr bool value = false; };
 template <> struct is_complex<zd> { static constexpr bool value = true; };
+template <> struct is_complex<zf> { static constexpr bool value = true; };
 
 // ---------------------------------------------------------------------------
 // operand codes
@@ -113,21 +131,23 @@ template <typename T>
 hipError_t gemm(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
                 long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
                 const GemmOpts &o = GemmOpts());
-// FP32 storage (gemm_lp.hip): op in {N, T}; no triangular operand masks.
+// FP32 storage (gemm_lp.hip): op in {N, T} on the FP32 / BF16 kernel;
+// products with a triangular operand run on gemm_tile<float>.
 template <>
 hipError_t gemm<float>(hipStream_t s, int ta, int tb, long m, long n, long k, float alpha,
                        const float *A, long lda, const float *B, long ldb, float beta,
                        const float *C, long ldc, float *D, long ldd, const GemmOpts &o);
+// The tile kernel of gemm.hip for any element type (double, double complex,
+// float, single complex); every GemmOpts option.
+template <typename T>
+hipError_t gemm_tile(hipStream_t s, int ta, int tb, long m, long n, long k, T alpha, const T *A,
+                     long lda, const T *B, long ldb, T beta, const T *C, long ldc, T *D, long ldd,
+                     const GemmOpts &o = GemmOpts());
 
 // 2-D copy D <- S (m x n)
 template <typename T>
 hipError_t copy2d(hipStream_t s, long m, long n, const T *S, long lds, T *D, long ldd);
 
-// D = S element-wise with a change of precision, m x n column-major: float <->
-// double and single <-> double complex (the single and single-complex entry
-// points run the double / double-complex paths on a widened copy).
-template <typename S, typename D>
-hipError_t convert2d(hipStream_t s, long m, long n, const S *src, long lds, D *dst, long ldd);
 
 // ---------------------------------------------------------------------------
 // level-3 building blocks (blas_ops.cpp)
@@ -308,6 +328,13 @@ hipError_t basis_put(hipStream_t s, long n, long nrhs, int nv, int slot, const d
                      const double *scale, double *V);
 hipError_t basis_combine(hipStream_t s, long n, long nrhs, int nv, int nvec, const double *V,
                          const double *y, double *X, long ldx);
+
+// Tile choice of a GEMM kernel family: a launch takes the family's larger
+// tile when it has at least this many of them (tuning.cpp; measured by the
+// autotuner, default dflt).  Families: D (double tile kernel, 128 vs 64),
+// Z (complex tile kernel, 64 vs 32), S (FP32 kernel, 128 vs 64), H (16-bit
+// kernel, 128 vs 64), W (16-bit kernel, 256 vs 128).
+long tuned_tile_min(char family, long dflt);
 
 // Measured outer block size for precision D / Z / S / B at order n, 0 when
 // the tuning table has no entry (tuning.cpp).

@@ -464,81 +464,6 @@ CUresult logdet_api(const char *name, CULAPACKhandle h, CUdeviceptr x, size_t in
   return CUDA_SUCCESS;
 }
 
-// ---- single and single complex on a widened copy ----
-//
-// The checks run first with the caller's arguments (the indices of the double
-// twins); then the n x n block is widened on the handle's main stream into a
-// stream-ordered copy (ld n), `run` applies the double / double-complex
-// routine to it, and the block is narrowed back (the triangle not referenced
-// round-trips exactly).  Synchronous, like the routines themselves.
-template <typename Lo, typename Hi, typename Run>
-CUresult on_wide(CULAPACKhandle h, size_t n, CUdeviceptr A, size_t lda, Run run) {
-  hc::DeviceGuard g(h->device);
-  HC_CHECK(g.status());
-  Hi *w = nullptr;
-  HC_CHECK(hipMallocAsync(reinterpret_cast<void **>(&w), n * n * sizeof(Hi), h->stream));
-  CUresult r = hc::to_cu(hc::convert2d<Lo, Hi>(h->stream, (long)n, (long)n, dptr<const Lo>(A), (long)lda, w,
-                                                (long)n));
-  if (r == CUDA_SUCCESS) r = run(reinterpret_cast<CUdeviceptr>(w));
-  if (r == CUDA_SUCCESS)
-    r = hc::to_cu(hc::convert2d<Hi, Lo>(h->stream, (long)n, (long)n, w, (long)n, dptr<Lo>(A), (long)lda));
-  (void)hipFreeAsync(w, h->stream);
-  const CUresult r2 = hc::to_cu(hipStreamSynchronize(h->stream));
-  return r != CUDA_SUCCESS ? r : r2;
-}
-
-enum class Lk { potrf, trtri, lauum, potri };
-
-template <typename Lo, typename Hi>
-CUresult lapack_wide(const char *name, Lk kind, CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n,
-                     CUdeviceptr A, size_t lda, long *info) {
-  // the checks of potrf_api / trtri_api / lauum_api / potri_api
-  const bool tri = kind == Lk::trtri;
-  if (info == nullptr) return bad_arg(name, tri ? 7 : 6);
-  *info = 0;
-  if (h == nullptr) { *info = -1; return bad_arg(name, 1); }
-  if (!hc::valid_uplo(uplo)) { *info = -2; return bad_arg(name, 2); }
-  if (tri && !hc::valid_diag(diag)) { *info = -3; return bad_arg(name, 3); }
-  if (lda < n) { *info = tri ? -5 : -4; return bad_arg(name, tri ? 5 : 4); }
-  if (n == 0) return CUDA_SUCCESS;
-  return on_wide<Lo, Hi>(h, n, A, lda, [&](CUdeviceptr w) {
-    switch (kind) {
-      case Lk::potrf: return potrf_api<Hi>(name, h, uplo, n, w, n, info);
-      case Lk::trtri: return trtri_api<Hi>(name, h, uplo, diag, n, w, n, info);
-      case Lk::lauum: return lauum_api<Hi>(name, h, uplo, n, w, n, info);
-      default: return potri_api<Hi>(name, h, uplo, n, w, n, info);
-    }
-  });
-}
-
-template <typename Lo, typename Hi>
-CUresult trti22_wide(const char *name, CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n,
-                     CUdeviceptr A, size_t lda, CUdeviceptr B, size_t ldb, CUdeviceptr info, CUstream stream) {
-  // the checks of trti22_api
-  if (h == nullptr) return bad_arg(name, 1);
-  if (!hc::valid_uplo(uplo)) return bad_arg(name, 2);
-  if (!hc::valid_diag(diag)) return bad_arg(name, 3);
-  if (n > (size_t)hc::diag_max<Hi>()) return CUDA_ERROR_INVALID_VALUE;
-  if (lda < n) return bad_arg(name, 6);
-  if (ldb < n) return bad_arg(name, 8);
-  if (n == 0) return trti22_api<Hi>(name, h, uplo, diag, 0, A, lda, B, ldb, info, stream);
-  hc::DeviceGuard g(h->device);
-  HC_CHECK(g.status());
-  Hi *w = nullptr;  // widened A, then widened B (its untouched part round-trips)
-  HC_CHECK(hipMallocAsync(reinterpret_cast<void **>(&w), 2 * n * n * sizeof(Hi), stream));
-  Hi *wb = w + n * n;
-  CUresult r = hc::to_cu(hc::convert2d<Lo, Hi>(stream, (long)n, (long)n, dptr<const Lo>(A), (long)lda, w, (long)n));
-  if (r == CUDA_SUCCESS)
-    r = hc::to_cu(hc::convert2d<Lo, Hi>(stream, (long)n, (long)n, dptr<const Lo>(B), (long)ldb, wb, (long)n));
-  if (r == CUDA_SUCCESS)
-    r = trti22_api<Hi>(name, h, uplo, diag, n, reinterpret_cast<CUdeviceptr>(w), n,
-                       reinterpret_cast<CUdeviceptr>(wb), n, info, stream);
-  if (r == CUDA_SUCCESS)
-    r = hc::to_cu(hc::convert2d<Hi, Lo>(stream, (long)n, (long)n, wb, (long)n, dptr<Lo>(B), (long)ldb));
-  (void)hipFreeAsync(w, stream);
-  return r;
-}
-
 }  // namespace
 
 namespace hc {
@@ -623,40 +548,41 @@ CUresult cuZlogdet(CULAPACKhandle h, CUdeviceptr x, size_t incx, size_t n, doubl
   return logdet_api<zd>("cuZlogdet", h, x, incx, n, result, stream);
 }
 
-// Single precision and single complex on widened copies (reference
+// Single precision and single complex: level-3 products on the FP32 matrix
+// cores, diagonal blocks computed in double / double complex (reference
 // lapack/cpotrf.c:174, lapack/strtri.c:301,369, lapack/ctrtri.c:301,326,
 // lapack/slauum.c:197, lapack/clauum.c:154, lapack/spotri.c:25,
 // lapack/cpotri.c:25, lapack/slogdet.c:37, lapack/clogdet.c:38).
 CUresult cuCpotrf(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
-  return lapack_wide<hc::zf, zd>("cuCpotrf", Lk::potrf, h, uplo, CBlasNonUnit, n, A, lda, info);
+  return potrf_api<hc::zf>("cuCpotrf", h, uplo, n, A, lda, info);
 }
 CUresult cuStrtri(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A, size_t lda,
                   long *info) {
-  return lapack_wide<float, double>("cuStrtri", Lk::trtri, h, uplo, diag, n, A, lda, info);
+  return trtri_api<float>("cuStrtri", h, uplo, diag, n, A, lda, info);
 }
 CUresult cuCtrtri(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A, size_t lda,
                   long *info) {
-  return lapack_wide<hc::zf, zd>("cuCtrtri", Lk::trtri, h, uplo, diag, n, A, lda, info);
+  return trtri_api<hc::zf>("cuCtrtri", h, uplo, diag, n, A, lda, info);
 }
 CUresult cuSlauum(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
-  return lapack_wide<float, double>("cuSlauum", Lk::lauum, h, uplo, CBlasNonUnit, n, A, lda, info);
+  return lauum_api<float>("cuSlauum", h, uplo, n, A, lda, info);
 }
 CUresult cuClauum(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
-  return lapack_wide<hc::zf, zd>("cuClauum", Lk::lauum, h, uplo, CBlasNonUnit, n, A, lda, info);
+  return lauum_api<hc::zf>("cuClauum", h, uplo, n, A, lda, info);
 }
 CUresult cuSpotri(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
-  return lapack_wide<float, double>("cuSpotri", Lk::potri, h, uplo, CBlasNonUnit, n, A, lda, info);
+  return potri_api<float>("cuSpotri", h, uplo, n, A, lda, info);
 }
 CUresult cuCpotri(CULAPACKhandle h, CBlasUplo uplo, size_t n, CUdeviceptr A, size_t lda, long *info) {
-  return lapack_wide<hc::zf, zd>("cuCpotri", Lk::potri, h, uplo, CBlasNonUnit, n, A, lda, info);
+  return potri_api<hc::zf>("cuCpotri", h, uplo, n, A, lda, info);
 }
 CUresult cuStrti22(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A, size_t lda,
                    CUdeviceptr B, size_t ldb, CUdeviceptr info, CUstream stream) {
-  return trti22_wide<float, double>("cuStrti22", h, uplo, diag, n, A, lda, B, ldb, info, stream);
+  return trti22_api<float>("cuStrti22", h, uplo, diag, n, A, lda, B, ldb, info, stream);
 }
 CUresult cuCtrti22(CULAPACKhandle h, CBlasUplo uplo, CBlasDiag diag, size_t n, CUdeviceptr A, size_t lda,
                    CUdeviceptr B, size_t ldb, CUdeviceptr info, CUstream stream) {
-  return trti22_wide<hc::zf, zd>("cuCtrti22", h, uplo, diag, n, A, lda, B, ldb, info, stream);
+  return trti22_api<hc::zf>("cuCtrti22", h, uplo, diag, n, A, lda, B, ldb, info, stream);
 }
 CUresult cuSlogdet(CULAPACKhandle h, CUdeviceptr x, size_t incx, size_t n, float *result, CUstream stream) {
   if (result == nullptr) return bad_arg("cuSlogdet", 5);

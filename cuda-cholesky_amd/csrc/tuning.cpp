@@ -6,10 +6,13 @@
 // one line per entry:
 //     potrf_nb <precision> <n_max> <nb>
 //     batched_per_cu <precision> <inverse> <workgroups per CU>
+//     gemm_tile <family> <min_tiles>
 // precision: D (FP64), Z (complex FP64), S (FP32 products / storage), B
 // (BF16 products).  For an order n the first potrf_nb line of that precision
-// with n <= n_max applies.  The table is read once, from $CUCHOLESKY_TUNING or
-// else tuning.txt beside the library; without it the built-in defaults hold.
+// with n <= n_max applies.  gemm_tile: a launch of kernel family F takes its
+// larger tile when it has at least min_tiles of them (families below).  The
+// table is read once, from $CUCHOLESKY_TUNING or else tuning.txt beside the
+// library; without it the built-in defaults hold.
 #include "internal.h"
 
 #include <dlfcn.h>
@@ -38,6 +41,9 @@ std::vector<Entry> &table() {
 // batched_per_cu entries, indexed [precision D/S][inverse]; 0 = none
 int g_batched_per_cu[2][2] = {{0, 0}, {0, 0}};
 
+// gemm_tile entries by family letter; 0 = none
+long g_tile_min[26] = {};
+
 void load() {
   std::string path;
   if (const char *e = std::getenv("CUCHOLESKY_TUNING")) {
@@ -58,6 +64,11 @@ void load() {
   char line[256];
   while (std::fgets(line, sizeof line, f) != nullptr) {
     if (line[0] == '#') continue;
+    long tmin = 0;
+    if (std::sscanf(line, "gemm_tile %7s %ld", prec, &tmin) == 2) {
+      if (prec[0] >= 'A' && prec[0] <= 'Z' && tmin > 0) g_tile_min[prec[0] - 'A'] = tmin;
+      continue;
+    }
     if (std::sscanf(line, "%63s %7s %ld %ld", key, prec, &n_max, &nb) != 4) continue;
     if (std::strcmp(key, "potrf_nb") == 0 && nb > 0 && n_max > 0)
       table().push_back(Entry{prec[0], n_max, nb});
@@ -82,6 +93,22 @@ long tuned_potrf_nb(char prec, long n) {
 int tuned_batched_per_cu(char prec, bool inverse) {
   std::call_once(g_once, load);
   return g_batched_per_cu[prec == 'S' ? 1 : 0][inverse ? 1 : 0];
+}
+
+long tuned_tile_min(char family, long dflt) {
+  if (family < 'A' || family > 'Z') return dflt;
+  // $CUCHOLESKY_TILE_MIN_<family> (read on every call: the autotuner changes
+  // it between measurements), else the table, else the built-in default
+  char name[32] = "CUCHOLESKY_TILE_MIN_";
+  name[20] = family;
+  name[21] = '\0';
+  if (const char *e = std::getenv(name)) {
+    const long v = std::atol(e);
+    if (v > 0) return v;
+  }
+  std::call_once(g_once, load);
+  const long t = g_tile_min[family - 'A'];
+  return t > 0 ? t : dflt;
 }
 
 }  // namespace hc

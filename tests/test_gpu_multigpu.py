@@ -521,15 +521,15 @@ def test_sc_potrf_potri_trtri_lauum(gpu, mlapack, cplx, uplo):
     X = F.copy(order="F")
     st, info = getattr(gpu, f"cuMultiGPU{p}potri")(mlapack, ord(uplo), n, X, n)
     assert st == 0 and info == 0
-    assert O.inverse_residual(w64(A), w64(X), uplo) <= 100 * EPS32
+    assert O.inverse_residual(w64(A), w64(X), uplo) <= 10 * n * EPS32
     T = F.copy(order="F")
     st, info = getattr(gpu, f"cuMultiGPU{p}trtri")(mlapack, ord(uplo), gpu.NonUnit, n, T, n)
     assert st == 0 and info == 0
     ref, _ = (O.ztrtri if cplx else O.dtrtri)(uplo, "N", w64(F))
     mask = tri(np.ones((n, n)), uplo).astype(bool)
-    assert rel(w64(T)[mask], ref[mask]) <= 4 * EPS32
+    assert rel(w64(T)[mask], ref[mask]) <= 10 * n * EPS32
     Lm = F.copy(order="F")
     st, info = getattr(gpu, f"cuMultiGPU{p}lauum")(mlapack, ord(uplo), n, Lm, n)
     assert st == 0 and info == 0
     refl, _ = (O.zlauum if cplx else O.dlauum)(uplo, w64(F))
-    assert rel(tri(w64(Lm), uplo), tri(refl, uplo)) <= 4 * EPS32
+    assert rel(tri(w64(Lm), uplo), tri(refl, uplo)) <= 2 * (n + 4) * EPS32

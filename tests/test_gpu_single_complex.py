@@ -1,14 +1,20 @@
-"""GPU parity of the single-precision and single-complex entry points that run
-on widened copies (cuStrmm2, cuStrtri, cuStrti22, cuSlauum, cuSpotri,
-cuSlogdet and the c surface: cuCgemm2, cuCherk, cuCtrmm2, cuCtrsm, cuCpotrf,
-cuCtrtri, cuCtrti22, cuClauum, cuCpotri, cuClogdet).
+"""GPU parity of the single-precision and single-complex entry points
+(cuStrmm2, cuStrtri, cuStrti22, cuSlauum, cuSpotri, cuSlogdet and the c
+surface: cuCgemm2, cuCherk, cuCtrmm2, cuCtrsm, cuCpotrf, cuCtrtri,
+cuCtrti22, cuClauum, cuCpotri, cuClogdet).
 
 The reference is the oracle's double / double-complex routine applied to the
-same FP32 inputs widened exactly; the library computes in double precision
-and rounds once, so results agree to a few units of FP32 rounding (bounds
-below in eps32 = 2^-23 units).  Padding rows and the triangle a routine does
-not reference come back bit-identical, and info / argument behaviour follows
-the double twins.
+same FP32 inputs widened exactly.  The library computes the level-3 products
+in FP32 / single complex (FP32 matrix cores) and the diagonal blocks in
+double / double complex, so the bounds are the usual error bounds in units of
+eps32 = 2^-23 that grow with the inner dimension k of a product
+(|D - ref| <= 2 (k + 4) eps32 (|alpha| |A| |B| + |beta| |C|) per element, as
+the reference's own BLAS tests bound each element) and with the order n of a
+factorisation, inverse or solve (10 n eps32, relative).  A routine small
+enough to stay inside one diagonal block (trti22) is still computed in double
+and rounded once (4 eps32).  Padding rows and the triangle a routine does not
+reference come back bit-identical, and info / argument behaviour follows the
+double twins.
 """
 import itertools
 
@@ -85,7 +91,8 @@ def test_cgemm2(gpu, blas, ta, tb, m, n, k):
     D = gpu.from_device(dD)
     ref = wide(C)
     O.zgemm(ta, tb, m, n, k, al, wide(A), wide(B), be, ref)
-    bound = 4 * EPS32 * (abs(al) * np.abs(opm(wide(A), ta)) @ np.abs(opm(wide(B), tb)) + abs(be) * np.abs(wide(C)))
+    bound = 2 * (k + 4) * EPS32 * (abs(al) * np.abs(opm(wide(A), ta)) @ np.abs(opm(wide(B), tb))
+                                   + abs(be) * np.abs(wide(C)))
     assert np.all(np.abs(D[:m] - ref) <= bound)
     assert np.all(D[m:] == 9.0)
 
@@ -117,7 +124,7 @@ def test_cherk(gpu, blas, uplo, trans):
     ref = wide(C)
     O.zherk(uplo, trans, n, k, 0.75, wide(A), 1.5, ref)
     M = opm(wide(A), trans)
-    bound = 4 * EPS32 * (0.75 * np.abs(M) @ np.abs(M).T + 1.5 * np.abs(wide(C)))
+    bound = 2 * (k + 4) * EPS32 * (0.75 * np.abs(M) @ np.abs(M).T + 1.5 * np.abs(wide(C)))
     mask = tri(np.ones((n, n)), uplo).astype(bool)
     assert np.all(np.abs(R[:n] - ref)[mask] <= bound[mask])
     np.testing.assert_array_equal(R[:n][other_mask(n, uplo)], C[other_mask(n, uplo)])
@@ -153,7 +160,7 @@ def run_trmm(gpu, blas, side, uplo, trans, diag, m, n, cplx):
     if diag == "U":
         Tm = Tm - np.diag(np.diag(Tm)) + np.eye(k)
     absprod = np.abs(Tm) @ np.abs(wide(B)) if side == "L" else np.abs(wide(B)) @ np.abs(Tm)
-    assert np.all(np.abs(X[:m] - ref) <= 4 * EPS32 * abs(al) * absprod)
+    assert np.all(np.abs(X[:m] - ref) <= 2 * (k + 4) * EPS32 * abs(al) * absprod)
     assert np.all(X[m:] == 4.0)
 
 
@@ -193,7 +200,7 @@ def test_ctrsm(gpu, blas, side, uplo, trans, diag):
     X = gpu.from_device(dB)
     ref = wide(B)
     O.ztrsm(side, uplo, trans, diag, m, n, al, wide(T), ref)
-    assert rel(X[:m], ref) <= 4 * EPS32
+    assert rel(X[:m], ref) <= 10 * k * EPS32
     assert np.all(X[m:] == 6.0)
 
 
@@ -214,16 +221,16 @@ def test_cpotrf_cpotri(gpu, lapack, uplo, n):
     assert st == 0 and info == 0
     F = gpu.from_device(dA)
     ref, _ = O.zpotrf(uplo, wide(A))
-    assert rel(tri(F[:n], uplo), tri(ref, uplo)) <= 4 * EPS32
-    assert O.potrf_backward_error(wide(A), wide(F[:n]), uplo) <= 8 * EPS32
+    assert rel(tri(F[:n], uplo), tri(ref, uplo)) <= 10 * n * EPS32
+    assert O.potrf_backward_error(wide(A), wide(F[:n]), uplo) <= 10 * n * EPS32
     np.testing.assert_array_equal(F[:n][other_mask(n, uplo)], A[other_mask(n, uplo)])
     assert np.all(F[n:] == 8.0)
     st, info = gpu.cuCpotri(lapack, ord(uplo), n, gpu.ptr(dA), n + 2)
     assert st == 0 and info == 0
     X = gpu.from_device(dA)
     refi, _ = O.zpotri(uplo, wide(F[:n]))
-    assert rel(O.herm_from(wide(X[:n]), uplo), O.herm_from(refi, uplo)) <= 4 * EPS32
-    assert O.inverse_residual(wide(A), wide(X[:n]), uplo) <= 100 * EPS32
+    assert rel(O.herm_from(wide(X[:n]), uplo), O.herm_from(refi, uplo)) <= 10 * n * EPS32
+    assert O.inverse_residual(wide(A), wide(X[:n]), uplo) <= 10 * n * EPS32
 
 
 @pytest.mark.parametrize("uplo", "LU")
@@ -255,7 +262,7 @@ def test_trtri(gpu, lapack, cplx, uplo, diag):
     if diag == "U":
         np.fill_diagonal(mask, False)
         assert np.all(np.diag(X) == np.diag(T))
-    assert rel(wide(X)[mask], ref[mask]) <= 4 * EPS32
+    assert rel(wide(X)[mask], ref[mask]) <= 10 * n * EPS32
     np.testing.assert_array_equal(X[other_mask(n, uplo)], T[other_mask(n, uplo)])
 
 
@@ -280,7 +287,7 @@ def test_lauum(gpu, lapack, cplx, uplo):
     assert st == 0 and info == 0
     X = gpu.from_device(dT)
     ref, _ = (O.zlauum if cplx else O.dlauum)(uplo, wide(T))
-    assert rel(tri(wide(X), uplo), tri(ref, uplo)) <= 4 * EPS32
+    assert rel(tri(wide(X), uplo), tri(ref, uplo)) <= 2 * (n + 4) * EPS32
     np.testing.assert_array_equal(X[other_mask(n, uplo)], T[other_mask(n, uplo)])
 
 
@@ -296,8 +303,8 @@ def test_spotrf_spotri(gpu, lapack, uplo):
     assert st == 0 and info == 0
     X = gpu.from_device(dA)
     refi, _ = O.dpotri(uplo, wide(F))
-    assert rel(O.herm_from(wide(X), uplo), O.herm_from(refi, uplo)) <= 4 * EPS32
-    assert O.inverse_residual(wide(A), wide(X), uplo) <= 100 * EPS32
+    assert rel(O.herm_from(wide(X), uplo), O.herm_from(refi, uplo)) <= 10 * n * EPS32
+    assert O.inverse_residual(wide(A), wide(X), uplo) <= 10 * n * EPS32
 
 
 @pytest.mark.parametrize("cplx", [False, True])
