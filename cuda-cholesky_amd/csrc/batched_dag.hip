@@ -35,6 +35,7 @@
 // of column j of X (they finish top-down); ttot / ltot count TRTRI / LAUUM
 // tiles; failed holds the 1-based first non-positive pivot.
 #include "internal.h"
+#include "publish.h"
 #include "tile16.h"
 #include "batched_dag.h"
 
@@ -88,14 +89,14 @@ __device__ int read_uniform(const int *p, int *s_word) {
   return *s_word;
 }
 
-// Publish everything this workgroup wrote, then add 1 to each counter.  Every
-// wave releases its own stores at agent scope (waits for them and writes the
-// XCD's L2 back) before the barrier: the barrier itself only waits for LDS
-// traffic, so a release by thread 0 alone would not cover the other waves'
-// global stores still in flight.
+// Publish everything this workgroup wrote, then add 1 to each counter: every
+// wave waits for its own global stores to reach the L2 before the barrier
+// (the barrier itself only waits for LDS traffic), then wave 0 writes the L2
+// back once and thread 0 updates the counters.
 __device__ void signal(int *c0, int *c1) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  publish_wait_own();
   __syncthreads();
+  if (threadIdx.x < 64) publish_release();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(c0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (c1 != nullptr) __hip_atomic_fetch_add(c1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

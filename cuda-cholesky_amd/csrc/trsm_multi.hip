@@ -20,8 +20,8 @@
 //     order the consumers' lanes read them (one 16-byte vector per lane and
 //     k-group): X in its own layout puts the 16 columns of a block ldx apart,
 //     and gathering them cost more than the rest of a chain step.  All waves
-//     store, one barrier, then a single wave publishes (one release fence and
-//     the flag).
+//     store and wait for their stores, one barrier, then a single wave
+//     publishes (one L2 write-back and the flag).
 //   * Waits are bounded: a wait that outlasts kSpinLimit polls sets the
 //     error word sync[0] and stops waiting, so every wave exits; the host
 //     turns a set error word into a failed call.
@@ -30,6 +30,7 @@
 // (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32); eight waves take 16 rows
 // of the block each.
 #include "internal.h"
+#include "publish.h"
 
 namespace hc {
 namespace {
@@ -369,12 +370,15 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
       if (k < ni && col < nrhs) X[(long)col * ldx + ri + k] = acc.v[r];
     }
   }
-  // publish: every wave releases its own stores at agent scope (the barrier
-  // alone waits only for LDS traffic, not for other waves' global stores),
-  // then the barrier, then the flag
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // publish: every wave waits for its own global stores to reach the L2 (the
+  // barrier alone waits only for LDS traffic), then wave 0 writes the L2 back
+  // once and sets the flag
+  publish_wait_own();
   __syncthreads();
-  if (wave == 0 && lane == 0) __hip_atomic_store(ready + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (wave == 0) {
+    publish_release();
+    if (lane == 0) __hip_atomic_store(ready + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Clears the tickets and ready flags (sync[1 ..]); the error word is left.
