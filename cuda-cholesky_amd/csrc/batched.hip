@@ -33,6 +33,7 @@ namespace {
 
 using t16::LEAF;
 using t16::TileAcc;
+using t16::kNegA;
 
 constexpr int BT_THREADS = 256;                // 4 waves per matrix
 constexpr int BT_NW = BT_THREADS / 64;
@@ -70,14 +71,12 @@ template <typename ST, bool LOWER>
 __device__ __forceinline__ int leaf_step(double (&a)[LEAF], const View<ST, LOWER> &v, int g0, double *Li,
                                          double &logpart) {
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  double unused = 0.0;
-  const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
+  double x[4], rpiv;
+  const int b = t16::leaf_factor_invert<double>(a, x, rpiv);
   if (b >= 0) return g0 + b;
-  // log of this lane's own pivot L_rr^2, all rows at once (off the pivot
-  // chain); summed over the 16 rows once per matrix
-  double dg = a[0];
-  t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(r == decltype(C)::value, a[decltype(C)::value], dg); });
-  if (q == 0 && g0 + r < v.n) logpart += 2.0 * log(dg);
+  // log of this lane's own pivot L_rr^2 = -2 log(1 / L_rr), all rows at once
+  // (off the pivot chain); summed over the 16 rows once per matrix
+  if (q == 0 && g0 + r < v.n) logpart -= 2.0 * log(rpiv);
 #pragma unroll
   for (int j = 0; j < 4; j++) {  // lane group q writes columns q + 4 j of its row
     // a value select (a conditional over the array elements themselves
@@ -85,8 +84,6 @@ __device__ __forceinline__ int leaf_step(double (&a)[LEAF], const View<ST, LOWER
     const double f = t16::sel(q == 0, a[4 * j], t16::sel(q == 1, a[4 * j + 1], t16::sel(q == 2, a[4 * j + 2], a[4 * j + 3])));
     v.put(g0 + r, g0 + q + 4 * j, f);
   }
-  double x[4];
-  t16::leaf_inverse_cols<double>(a, x);
 #pragma unroll
   for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
   return -1;
@@ -334,33 +331,31 @@ __device__ __forceinline__ int slot_code(int codes, int s) {
 }
 
 // T -= L_J L_I^T with the panel tiles L_I, L_J in LDS (P_I, P_J); lo is the
-// lane's fragment offset at k-step 0.
+// lane's fragment offset at k-step 0.  The minus sign is the instruction's
+// negate-A modifier, not a VALU negation of each operand.
 __device__ __forceinline__ void tile_update(v4d &T, const double *PI, const double *PJ, int lo) {
 #pragma unroll
   for (int r = 0; r < 4; r++)
-    T = __builtin_amdgcn_mfma_f64_16x16x4f64(-PJ[r * 4 * LL + lo], PI[r * 4 * LL + lo], T, 0, 0, 0);
+    T = __builtin_amdgcn_mfma_f64_16x16x4f64(PJ[r * 4 * LL + lo], PI[r * 4 * LL + lo], T, 0, 0, kNegA);
 }
 
 // The leaf wave's step: factor the leaf at g0 (rows in a[], lane r + 16 q
-// holds row r), write L_pp to the matrix, L_rr (rows below n) to piv[g0 + r]
-// and inv(L_pp) to Li; returns the failing pivot's column (g0 + k) or -1.
+// holds row r), write L_pp to the matrix, 1 / L_rr (rows below n) to
+// rpiv[g0 + r] and inv(L_pp) to Li; returns the failing pivot's column
+// (g0 + k) or -1.
 template <typename ST, bool LOWER>
 __device__ __forceinline__ int leaf_step_piv(double (&a)[LEAF], const View<ST, LOWER> &v, int g0, double *Li,
-                                             double *piv) {
+                                             double *rpiv) {
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  double unused = 0.0;
-  const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
+  double x[4], rp;
+  const int b = t16::leaf_factor_invert<double>(a, x, rp);
   if (b >= 0) return g0 + b;
-  double dg = a[0];
-  t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(r == decltype(C)::value, a[decltype(C)::value], dg); });
-  if (q == 0 && g0 + r < v.n) piv[g0 + r] = dg;
+  if (q == 0 && g0 + r < v.n) rpiv[g0 + r] = rp;
 #pragma unroll
   for (int j = 0; j < 4; j++) {  // lane group q writes columns q + 4 j of its row
     const double f = t16::sel(q == 0, a[4 * j], t16::sel(q == 1, a[4 * j + 1], t16::sel(q == 2, a[4 * j + 2], a[4 * j + 3])));
     v.put(g0 + r, g0 + q + 4 * j, f);
   }
-  double x[4];
-  t16::leaf_inverse_cols<double>(a, x);
 #pragma unroll
   for (int j = 0; j < 4; j++) Li[(q + 4 * j) * LL + r] = x[j];
   return -1;
@@ -401,7 +396,7 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
   __shared__ double P[NPMAX * LEAF * LL];  // panel tiles L_Ip
   __shared__ double Li[LEAF * LL];         // inv(L_pp)
   __shared__ double Dg[LEAF * LL];         // tile (D, D) on its way to the leaf wave
-  __shared__ double piv[NPMAX * LEAF];     // L_rr
+  __shared__ double rpiv[NPMAX * LEAF];    // 1 / L_rr
   __shared__ double red[NW];
   __shared__ int flag, dready;             // failing column or -1; D once tile (D, D) is in Dg
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lk = lane >> 4;
@@ -448,7 +443,7 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
           if (ready) {
             double a[LEAF];
             t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
-            b = leaf_step_piv<ST, LOWER>(a, v, D * LEAF, Li, piv);
+            b = leaf_step_piv<ST, LOWER>(a, v, D * LEAF, Li, rpiv);
           }
           if (lane == 0) flag = b;
         }
@@ -504,6 +499,12 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
           bad = flag;
           if (bad != -1) break;
           // panel of column p: T_Ip <- inv(L_pp) T_Ip, to LDS and to the matrix
+          // this lane's fragments of inv(L_pp), read once for the wave's panel
+          // tiles in the largest class (the smaller ones have no registers
+          // to spare and re-read them)
+          double lf[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) lf[r] = CLS == 2 ? Li[r * 4 * LL + lo] : 0.0;
           t16::static_for<0, S>([&](auto Sc) {
             constexpr int s = decltype(Sc)::value;
             const int c = slot_code(codes, s);
@@ -511,7 +512,7 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
               v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
               for (int r = 0; r < 4; r++)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[r * 4 * LL + lo], T[s][r], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CLS == 2 ? lf[r] : Li[r * 4 * LL + lo], T[s][r], acc, 0, 0, 0);
               T[s] = acc;
               const int I = c & 255;
               double *Pt = P + I * LEAF * LL;
@@ -552,10 +553,10 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
         __syncthreads();
       }
     }
-    // log-determinant: 2 sum log L_rr over the pivots, one per thread
+    // log-determinant: 2 sum log L_rr = -2 sum log(1 / L_rr), one pivot per thread
     double lg = 0.0;
     if (bad == -1)
-      for (int r = tid; r < n; r += NW * 64) lg += log(piv[r]);
+      for (int r = tid; r < n; r += NW * 64) lg -= log(rpiv[r]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
     if (lane == 0) red[wave] = lg;
@@ -568,7 +569,7 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
       info[idx] = bad >= 0 ? bad + 1 : (bad == -2 ? -1 : 0);
       if (logdet != nullptr) logdet[idx] = bad != -1 ? __builtin_nan("") : 2.0 * sum;
     }
-    __syncthreads();  // flag, piv and LDS tiles are reused by the next matrix
+    __syncthreads();  // flag, rpiv and LDS tiles are reused by the next matrix
   }
 }
 

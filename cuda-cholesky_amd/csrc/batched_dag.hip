@@ -298,13 +298,11 @@ __device__ int lds_potrf64(double *P, double *Li, int *flag, int kvalid, double 
     if (wave == 0) {
       double a[LEAF];  // row li of the leaf
       t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = P[(g0 + decltype(C)::value) * LD + g0 + li]; });
-      double unused = 0.0;
-      const int b = t16::leaf_factor_rows<double>(a, 0, &unused);
+      double x[4], rpiv;
+      const int b = t16::leaf_factor_invert<double>(a, x, rpiv);
       if (b < 0) {
-        // log of this lane's own pivot L_rr^2, off the pivot chain
-        double dg = a[0];
-        t16::static_for<1, LEAF>([&](auto C) { dg = t16::sel(li == decltype(C)::value, a[decltype(C)::value], dg); });
-        if (lk == 0 && g0 + li < kvalid) logsum += 2.0 * log(dg);
+        // log of this lane's own pivot L_rr^2 = -2 log(1 / L_rr), off the pivot chain
+        if (lk == 0 && g0 + li < kvalid) logsum -= 2.0 * log(rpiv);
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) {  // lane group lk writes columns lk + 4 jj of row li
           const double f = t16::sel(lk == 0, a[4 * jj], t16::sel(lk == 1, a[4 * jj + 1],
@@ -312,8 +310,6 @@ __device__ int lds_potrf64(double *P, double *Li, int *flag, int kvalid, double 
           const int c = lk + 4 * jj;
           P[(g0 + c) * LD + g0 + li] = c <= li ? f : 0.0;
         }
-        double x[4];
-        t16::leaf_inverse_cols<double>(a, x);
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) Li[(lk + 4 * jj) * LL + li] = x[jj];
       }

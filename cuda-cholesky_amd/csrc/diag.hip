@@ -315,8 +315,18 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
   T a[LEAF];  // row r (row layout: the four lane groups hold the same row)
 #pragma unroll
   for (int c = 0; c < LEAF; c++) a[c] = S[c * ls + r];
-  double unused = 0.0;
-  const int bad = t16::leaf_factor_rows<T>(a, 0, &unused);
+  T x[4];
+  int bad;
+  if constexpr (is_complex<T>::value) {
+    // the complex kernel is already at its register limit: the fused pass
+    // keeps more values live, so factor and invert one after the other
+    double unused = 0.0;
+    bad = t16::leaf_factor_rows<T>(a, 0, &unused);
+    if (bad < 0) t16::leaf_inverse_cols<T>(a, x);
+  } else {
+    double rpiv;
+    bad = t16::leaf_factor_invert<T>(a, x, rpiv);
+  }
   if (bad >= 0) return bad;
   // lane group q writes columns q + 4 j of its row
   auto pick = [&](const T *v, int j) {
@@ -331,8 +341,6 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
       else A[(long)gr * lda + gc] = convert<ST>(conjx(v));
     }
   }
-  T x[4];
-  t16::leaf_inverse_cols<T>(a, x);
 #pragma unroll
   for (int j = 0; j < 4; j++) S[(q + 4 * j) * ls + r] = x[j];
   return -1;

@@ -22,6 +22,11 @@ constexpr int LEAF = 16;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Last argument of __builtin_amdgcn_mfma_f64_*: on gfx950 the FP64 MFMA's
+// BLGP field holds negate modifiers (bit 0: A, bit 1: B, bit 2: C), so
+// acc - A B is one instruction with no VALU negation of the operand.
+constexpr int kNegA = 1;
+
 __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
 __device__ __forceinline__ zd sel(bool c, zd a, zd b) { return make_zd(c ? a.x : b.x, c ? a.y : b.y); }
 
@@ -205,6 +210,55 @@ __device__ __forceinline__ void leaf_inverse_cols(const T a[LEAF], T x[4]) {
   });
 #pragma unroll
   for (int j = 0; j < 4; j++) x[j] = (e[j] - acc[j]) * dinv;
+}
+
+// leaf_factor_rows followed by the inverse, with fewer instructions: the leaf
+// wave issues every one of them for all 64 lanes, so the count, not the
+// pivot chain's latency, sets the leaf's time.  With L = L1 D (L1 unit lower,
+// D = diag(L_kk)), inv(L) = D^-1 inv(L1), and the rows of Y = inv(L1) follow
+// from Y(k, c) = delta_kc - sum_{j<k} L1(k, j) Y(j, c) with no division:
+// at step k, row k of Y is final in lane k, is broadcast, and every lane
+// r > k subtracts L1(r, k) = L(r, k) / L_kk times it (two instructions per
+// column, against four for the substitution with L).  X = D^-1 Y and L1 use
+// 1 / L_kk = rsqrt(d_k), which the factorisation computes anyway, so no
+// divide remains.  On return a[c] (c <= r) is L(r, c) as from
+// leaf_factor_rows, x[j] = X(r, q + 4 j) (zeros above the diagonal) and
+// rpiv = 1 / L(r, r) of the lane's row r.  Returns the first non-positive /
+// NaN pivot (0-based) or -1; on failure x and rpiv are unspecified.
+template <typename T>
+__device__ __forceinline__ int leaf_factor_invert(T a[LEAF], T x[4], double &rpiv) {
+  int r = threadIdx.x & 15, q = (threadIdx.x >> 4) & 3;
+  asm volatile("" : "+v"(r), "+v"(q));
+  int bad = -1;
+  double rs_own = 1.0;
+  static_for<0, LEAF>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    const double d0 = realx(row_bcast<k>(a[k]));  // A(k, k) after k rank-1 updates
+    const bool ok = d0 > 0.0;                     // false for NaN
+    bad = (bad < 0 && !ok) ? k : bad;
+    const double d = ok ? d0 : 1.0;
+    const double rs = rsqrt_nr(d);
+    const T ld = (rs * rs) * a[k];                // A(r, k) / d
+    static_for<k + 1, LEAF>([&](auto C) {         // rows r < c only touch unused entries
+      constexpr int c = decltype(C)::value;
+      a[c] = a[c] - ld * conjx(row_bcast<c>(a[k]));
+    });
+    rs_own = r == k ? rs : rs_own;
+    a[k] = sel(r == k, from_real<T>(d * rs), rs * a[k]);  // entries r < k are unused
+  });
+  T y[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) y[j] = sel(r == q + 4 * j, from_real<T>(1.0), from_real<T>(0.0));
+  static_for<0, LEAF>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    const T l1 = sel(r > k, row_bcast<k>(rs_own) * a[k], from_real<T>(0.0));  // L1(r, k)
+#pragma unroll
+    for (int j = 0; j < 4; j++) y[j] = y[j] - l1 * row_bcast<k>(y[j]);  // row k of Y is final in lane k
+  });
+#pragma unroll
+  for (int j = 0; j < 4; j++) x[j] = rs_own * y[j];
+  rpiv = rs_own;
+  return bad;
 }
 
 }  // namespace t16
