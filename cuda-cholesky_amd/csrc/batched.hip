@@ -296,28 +296,40 @@ potrf_batched_kernel(int count, const int *nvec, const long *offs, const int *or
 // One workgroup factors one matrix (n <= 16 NPMAX) at a time and holds the
 // matrix's whole lower tile triangle in registers, so that its HBM traffic is
 // one read of the triangle and one write of L.  The critical path of a
-// matrix is its chain of 16-column leaves (each one's factor and inverse
-// needs the previous one's panel), so the roles are cut around it:
-//   * the leaf wave (the last) holds no tiles: at step p it waits (LDS flag)
-//     for tile (p + 1, p + 1), factors and inverts leaf p + 1 and records its
-//     pivots (their logs are taken once per matrix, off the chain);
-//   * the NW - 1 tile waves hold every tile.  Off-diagonal tile (I, J),
-//     I > J, numbered column by column (u = J (np - 1) - J (J - 1) / 2 +
-//     I - J - 1, np = ceil(n / 16)), is in wave u % (NW - 1), slot
-//     u / (NW - 1): the trailing tiles of any step and the panel tiles of any
-//     column are consecutive numbers, dealt evenly.  Diagonal tile (J, J) is
-//     in wave J % (NW - 1), diagonal slot J / (NW - 1).
+// matrix is its chain of 16-column leaves: leaf p + 1 needs tile (p + 1, p + 1)
+// after its update with panel tile (p + 1, p), which needs inv(L_pp) from
+// leaf p.  The roles are cut so that nothing else lies on that chain:
+//   * the leaf wave (the last) holds no tiles: it waits (LDS flag `dready`)
+//     for tile (D, D), factors and inverts leaf D, records its pivots (their
+//     logs are taken once per matrix, off the chain), publishes inv(L_DD)
+//     (flag `lready`) and waits for the next tile.  It takes part in no
+//     barrier inside a matrix;
+//   * the NW - 1 tile waves hold every tile.  The sub-diagonal tile
+//     (J + 1, J) and the diagonal tile (J + 1, J + 1) are in the same wave,
+//     (J + 1) % (NW - 1): at step p that wave forms its panel tile (p + 1, p)
+//     first, updates tile (p + 1, p + 1) with it straight from registers and
+//     hands it to the leaf wave before it forms its other panel tiles, so leaf
+//     p + 1 starts one panel tile and one update after leaf p has finished.
+//     The other off-diagonal tiles follow round-robin: in the sequence g =
+//     0, 1, ... of the np - 1 sub-diagonal tiles (g: tile (g + 1, g)) and then
+//     the tiles I >= J + 2 column by column, tile g is in wave
+//     (g + 1) % (NW - 1), slot g / (NW - 1) (np = ceil(n / 16)).  Diagonal tile
+//     (J, J) is in wave J % (NW - 1), diagonal slot J / (NW - 1).
 // A tile is held as T = A_IJ^T in the MFMA result layout (lane l, register
 // r: T(l / 16 + 4 r, l % 16) = A_IJ(l % 16, l / 16 + 4 r)), which is also the
 // operand fragment of A_IJ for k-step r: tiles enter the matrix cores with no
-// layout change.  Per step p, with D = p + 1:
-//   panel:    T_Ip <- inv(L_pp) T_Ip = L_Ip^T (I > p), to LDS (element
-//             (row, k) at P_I[k LL + row]) and to the matrix;
-//   barrier;
-//   the owner of (D, D) updates it first, hands it to the leaf wave through
-//   LDS, then every tile wave updates its trailing tiles T_IJ -= L_Jp L_Ip^T
-//   (p < J <= I) while the leaf is factored;
-//   barrier.
+// layout change.  Per step p, in the tile waves:
+//   wait for inv(L_pp) (flag `lready`);
+//   panel:    T_Ip <- inv(L_pp) T_Ip = L_Ip^T (I > p), to LDS buffer p % 2
+//             (element (row, k) at P_I[k LL + row]) and to the matrix;
+//   tile-wave barrier (an LDS counter: the leaf wave is not held);
+//   trailing: T_IJ -= L_Jp L_Ip^T (p < J <= I, (p + 1, p + 1) excepted).
+// The panel buffers, inv(L_pp) and the hand-off tile alternate or are
+// consumed before they are written again: P and inv(L) by step parity, the
+// hand-off tile (D, D) is read by the leaf wave before leaf D publishes, and
+// the next hand-off waits for that.  Every wait is bounded (4 s of the
+// 100 MHz real-time counter); an abandoned one makes the matrix report
+// info = -1.
 // ---------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -381,6 +393,23 @@ struct RegClass<2> {
   static constexpr int NPMAX = 16, NW = 8, OCC = 2;
 };
 
+// Bounded wait of a whole wave until `pred()` holds (4 s of the 100 MHz
+// real-time counter); false when abandoned.
+template <typename F>
+__device__ __forceinline__ bool wave_wait(F pred) {
+  if (pred()) return true;
+  const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+  while (!pred()) {
+    __builtin_amdgcn_s_sleep(1);
+    if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ int lds_acquire(const int *f) {
+  return __builtin_amdgcn_readfirstlane(__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
 // cnt[c]: matrices of class c; `order` lists class 2, then 1, then 0.
 template <typename ST, bool LOWER, int CLS, int NPMAX = RegClass<CLS>::NPMAX, int NW = RegClass<CLS>::NW,
           int OCC = RegClass<CLS>::OCC>
@@ -392,17 +421,30 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
   constexpr int NWT = NW - 1;                                            // tile waves
   constexpr int NO = NPMAX * (NPMAX - 1) / 2, S = (NO + NWT - 1) / NWT;  // off-diagonal tiles, slots
   constexpr int SD = (NPMAX + NWT - 1) / NWT;                           // diagonal slots
+  constexpr int PB = NPMAX * LEAF * LL;                                 // one panel buffer
   static_assert(S <= 64, "slot codes live in one lane each");
-  __shared__ double P[NPMAX * LEAF * LL];  // panel tiles L_Ip
-  __shared__ double Li[LEAF * LL];         // inv(L_pp)
-  __shared__ double Dg[LEAF * LL];         // tile (D, D) on its way to the leaf wave
-  __shared__ double rpiv[NPMAX * LEAF];    // 1 / L_rr
+  __shared__ double P[2 * PB];           // panel tiles L_Ip, by step parity
+  __shared__ double Li[2 * LEAF * LL];   // inv(L_pp), by step parity
+  __shared__ double Dg[LEAF * LL];       // tile (D, D) on its way to the leaf wave
+  __shared__ double rpiv[NPMAX * LEAF];  // 1 / L_rr
   __shared__ double red[NW];
-  __shared__ int flag, dready;             // failing column or -1; D once tile (D, D) is in Dg
+  // flag: failing column, -1 (none) or -2 (the leaf wave abandoned a wait);
+  // fleaf: the leaf that failed (set once, before lready; a large value
+  // while none has), so that every tile wave decides at the same step;
+  // dready: D once tile (D, D) is in Dg; lready: D once leaf D is done;
+  // tbar: tile-wave barrier arrivals; werr: a tile wave abandoned a wait
+  __shared__ int flag, fleaf, dready, lready, tbar, werr;
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int lo = lk * LL + li;  // this lane's operand fragment in an LDS tile (k-step 0)
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  if (tid == 0) dready = -1;
+  if (tid == 0) {
+    flag = -1;
+    fleaf = 1 << 30;
+    dready = -1;
+    lready = -1;
+    tbar = 0;
+    werr = 0;
+  }
   __syncthreads();
   for (int t = blockIdx.x; t < count; t += gridDim.x) {
     const int idx = order[t];
@@ -417,51 +459,41 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
       continue;
     }
     const int np = (n + LEAF - 1) / LEAF;
-    int bad = -1;
     if (wave == NWT) {
       // ---- leaf wave ----
-      for (int p = -1; p < np; p++) {
-        if (p >= 0) {
-          bad = flag;
-          if (bad != -1) break;
-          __syncthreads();  // the panel of column p is in LDS
+      for (int D = 0; D < np; D++) {
+        int b = -2;
+        if (wave_wait([&] { return lds_acquire(&dready) == D; })) {
+          double a[LEAF];
+          t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
+          b = leaf_step_piv<ST, LOWER>(a, v, D * LEAF, Li + (D & 1) * LEAF * LL, rpiv);
         }
-        const int D = p + 1;
-        if (D < np) {
-          // bounded wait (4 s of the 100 MHz real-time counter); abandoned,
-          // the matrix reports info = -1
-          const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-          bool ready = true;
-          while (__hip_atomic_load(&dready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != D) {
-            __builtin_amdgcn_s_sleep(1);
-            if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {
-              ready = false;
-              break;
-            }
+        if (lane == 0) {
+          if (b != -1) {
+            flag = b;
+            fleaf = D;
           }
-          int b = -2;
-          if (ready) {
-            double a[LEAF];
-            t16::static_for<0, LEAF>([&](auto C) { a[decltype(C)::value] = Dg[decltype(C)::value * LL + li]; });
-            b = leaf_step_piv<ST, LOWER>(a, v, D * LEAF, Li, rpiv);
-          }
-          if (lane == 0) flag = b;
+          __hip_atomic_store(&lready, D, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        __syncthreads();
+        if (b != -1) break;
       }
     } else {
       // ---- tile waves ----
       const int no = np * (np - 1) / 2;
       int codes = -1;  // lane s < S: the code of this wave's off-diagonal slot s
       {
-        const int u = lane * NWT + wave;
-        if (lane < S && u < no) {
-          int J = 0, start = 0;  // start: number of tile (J + 1, J)
-          while (u >= start + np - 1 - J) {
-            start += np - 1 - J;
-            J++;
+        const int g = (wave + NWT - 1) % NWT + NWT * lane;
+        if (lane < S && g < no) {
+          if (g < np - 1) {
+            codes = (g + 1) | (g << 8);  // sub-diagonal tile (g + 1, g)
+          } else {
+            int k = g - (np - 1), J = 0;  // k-th tile with I >= J + 2, column by column
+            while (k >= np - 2 - J) {
+              k -= np - 2 - J;
+              J++;
+            }
+            codes = (J + 2 + k) | (J << 8);
           }
-          codes = (u - start + J + 1) | (J << 8);
         }
       }
       v4d T[S], Td[SD];
@@ -494,65 +526,87 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
 #pragma unroll
         for (int r = 0; r < 4; r++) Td[q][r] = v.fix(Td[q][r], J * LEAF + li, J * LEAF + lk + 4 * r);
       }
-      for (int p = -1; p < np; p++) {
-        if (p >= 0) {
-          bad = flag;
-          if (bad != -1) break;
-          // panel of column p: T_Ip <- inv(L_pp) T_Ip, to LDS and to the matrix
-          // this lane's fragments of inv(L_pp), read once for the wave's panel
-          // tiles in the largest class (the smaller ones have no registers
-          // to spare and re-read them)
-          double lf[4];
+      if (wave == 0) {  // tile (0, 0) to the leaf wave
 #pragma unroll
-          for (int r = 0; r < 4; r++) lf[r] = CLS == 2 ? Li[r * 4 * LL + lo] : 0.0;
-          t16::static_for<0, S>([&](auto Sc) {
-            constexpr int s = decltype(Sc)::value;
-            const int c = slot_code(codes, s);
-            if (c >= 0 && (c >> 8) == p) {
-              v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-              for (int r = 0; r < 4; r++)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CLS == 2 ? lf[r] : Li[r * 4 * LL + lo], T[s][r], acc, 0, 0, 0);
-              T[s] = acc;
-              const int I = c & 255;
-              double *Pt = P + I * LEAF * LL;
-#pragma unroll
-              for (int r = 0; r < 4; r++) {
-                Pt[r * 4 * LL + lo] = acc[r];
-                v.put(I * LEAF + li, p * LEAF + lk + 4 * r, acc[r]);  // L_Ip(li, lk + 4 r)
-              }
-            }
-          });
-          __syncthreads();
-        }
+        for (int r = 0; r < 4; r++) Dg[r * 4 * LL + lo] = Td[0][r];  // A_00(li, lk + 4 r)
+        if (lane == 0) __hip_atomic_store(&dready, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      // the last column has no panel tiles and no trailing update
+      for (int p = 0; p + 1 < np; p++) {
         const int D = p + 1;
-        if (D < np && wave == D % NWT) {
-          // tile (D, D): its last update, then to the leaf wave
+        if (!wave_wait([&] { return lds_acquire(&lready) >= p; })) {
+          if (lane == 0) werr = 1;
+          break;
+        }
+        // leaf p failed (a later leaf may already have failed too: the
+        // decision is made on the step, the same in every tile wave)
+        if (fleaf <= p) break;
+        const double *Lp = Li + (p & 1) * LEAF * LL;
+        double *Pp = P + (p & 1) * PB;
+        // panel of column p: T_Ip <- inv(L_pp) T_Ip, to LDS and to the
+        // matrix; the sub-diagonal tile (D, p) comes first (lowest slot), and
+        // its owner updates tile (D, D) with it and hands that over at once
+        // this lane's fragments of inv(L_pp), read once for the wave's panel
+        // tiles in the largest class (the smaller ones have no registers
+        // to spare and re-read them)
+        double lf[4];
 #pragma unroll
-          for (int q = 0; q < SD; q++)
-            if (q == D / NWT) {
-              if (p >= 0) tile_update(Td[q], P + D * LEAF * LL, P + D * LEAF * LL, lo);
+        for (int r = 0; r < 4; r++) lf[r] = CLS == 2 ? Lp[r * 4 * LL + lo] : 0.0;
+        t16::static_for<0, S>([&](auto Sc) {
+          constexpr int s = decltype(Sc)::value;
+          const int c = slot_code(codes, s);
+          if (c >= 0 && (c >> 8) == p) {
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-              for (int r = 0; r < 4; r++) Dg[r * 4 * LL + lo] = Td[q][r];  // A_DD(li, lk + 4 r)
+            for (int r = 0; r < 4; r++)
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CLS == 2 ? lf[r] : Lp[r * 4 * LL + lo], T[s][r], acc, 0, 0, 0);
+            T[s] = acc;
+            const int I = c & 255;
+            if (I == D) {
+              // tile (D, D) -= L_Dp L_Dp^T: acc[r] is the operand fragment of
+              // L_Dp for k-step r, on both sides
+#pragma unroll
+              for (int q = 0; q < SD; q++)
+                if (q == D / NWT) {
+#pragma unroll
+                  for (int r = 0; r < 4; r++)
+                    Td[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[r], acc[r], Td[q], 0, 0, kNegA);
+#pragma unroll
+                  for (int r = 0; r < 4; r++) Dg[r * 4 * LL + lo] = Td[q][r];  // A_DD(li, lk + 4 r)
+                }
+              if (lane == 0) __hip_atomic_store(&dready, D, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-          if (lane == 0) __hip_atomic_store(&dready, D, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-        if (p >= 0) {
-          // the other trailing tiles
+            double *Pt = Pp + I * LEAF * LL;
 #pragma unroll
-          for (int q = 0; q < SD; q++) {
-            const int J = q * NWT + wave;
-            if (J > D && J < np) tile_update(Td[q], P + J * LEAF * LL, P + J * LEAF * LL, lo);
+            for (int r = 0; r < 4; r++) {
+              Pt[r * 4 * LL + lo] = acc[r];
+              v.put(I * LEAF + li, p * LEAF + lk + 4 * r, acc[r]);  // L_Ip(li, lk + 4 r)
+            }
           }
-          t16::static_for<0, S>([&](auto Sc) {
-            constexpr int s = decltype(Sc)::value;
-            const int c = slot_code(codes, s);
-            if (c >= 0 && (c >> 8) > p) tile_update(T[s], P + (c & 255) * LEAF * LL, P + (c >> 8) * LEAF * LL, lo);
-          });
+        });
+        // tile-wave barrier: every panel tile of column p is in LDS
+        if (lane == 0) __hip_atomic_fetch_add(&tbar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!wave_wait([&] { return lds_acquire(&tbar) >= NWT * D; })) {
+          if (lane == 0) werr = 1;
+          break;
         }
-        __syncthreads();
+        // the other trailing tiles
+#pragma unroll
+        for (int q = 0; q < SD; q++) {
+          const int J = q * NWT + wave;
+          if (J > D && J < np) tile_update(Td[q], Pp + J * LEAF * LL, Pp + J * LEAF * LL, lo);
+        }
+        t16::static_for<0, S>([&](auto Sc) {
+          constexpr int s = decltype(Sc)::value;
+          const int c = slot_code(codes, s);
+          if (c >= 0 && (c >> 8) > p) tile_update(T[s], Pp + (c & 255) * LEAF * LL, Pp + (c >> 8) * LEAF * LL, lo);
+        });
       }
     }
+    __syncthreads();  // the leaf wave is done: flag, rpiv and werr are final
+    const int fl = flag;
+    // bad: failing column, -1 (none) or -2 (a wait was abandoned)
+    const int bad = fl != -1 ? fl : (werr != 0 ? -2 : -1);
     // log-determinant: 2 sum log L_rr = -2 sum log(1 / L_rr), one pivot per thread
     double lg = 0.0;
     if (bad == -1)
@@ -560,16 +614,21 @@ potrf_batched_reg_kernel(const int *cnt, const int *nvec, const long *offs, cons
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
     if (lane == 0) red[wave] = lg;
-    if (tid == 0) dready = -1;  // every wave is past its last use of the flag
     __syncthreads();
     if (tid == 0) {
       double sum = 0.0;
       for (int w = 0; w < NW; w++) sum += red[w];
-      // bad: failing column, -1 (none) or -2 (a wait was abandoned)
       info[idx] = bad >= 0 ? bad + 1 : (bad == -2 ? -1 : 0);
       if (logdet != nullptr) logdet[idx] = bad != -1 ? __builtin_nan("") : 2.0 * sum;
+      // every wave is past its last use of the flags
+      flag = -1;
+      fleaf = 1 << 30;
+      dready = -1;
+      lready = -1;
+      tbar = 0;
+      werr = 0;
     }
-    __syncthreads();  // flag, rpiv and LDS tiles are reused by the next matrix
+    __syncthreads();  // flags, rpiv and LDS tiles are reused by the next matrix
   }
 }
 
