@@ -328,6 +328,15 @@ hipError_t basis_put(hipStream_t s, long n, long nrhs, int nv, int slot, const d
                      const double *scale, double *V);
 hipError_t basis_combine(hipStream_t s, long n, long nrhs, int nv, int nvec, const double *V,
                          const double *y, double *X, long ldx);
+// GMRES bookkeeping on the device (Hessenberg columns, Givens rotations, the
+// rotated right-hand side, the basis scales and the convergence count).
+hipError_t gmres_start(hipStream_t s, long nrhs, int nv, const double *nrm2, double *beta, double *g,
+                       double *scale);
+hipError_t gmres_rotate(hipStream_t s, long nrhs, int nv, int m, int j, const double *hd, const double *nrm2,
+                        double *H, double *cs, double *sn, double *g, const double *beta, double *scale,
+                        double tol, int *active);
+hipError_t gmres_solve_y(hipStream_t s, long nrhs, int nv, int m, int used, const double *H, const double *g,
+                         double *y);
 
 // Tile choice of a GEMM kernel family: a launch takes the family's larger
 // tile when it has at least this many of them (tuning.cpp; measured by the

@@ -314,7 +314,96 @@ __global__ void basis_combine_kernel(int n, int nv, int nvec, const double *V, c
   X[(long)k * ldx + r] += s;
 }
 
+// GMRES bookkeeping on the device: right-hand side k is thread k.  H_k is
+// stored by columns (column j of H_k at H + (k m + j) nv), with the Givens
+// rotations (cs, sn: k m + j) and the rotated right-hand side g_k (k nv + i).
+
+// Start of a cycle: nrm2[k nv] = ||w_k||^2 of the preconditioned residual.
+__global__ void gmres_start_kernel(int K, int nv, const double *nrm2, double *beta, double *g,
+                                   double *scale) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const double b = sqrt(nrm2[(long)k * nv]);
+  beta[k] = b;
+  scale[k] = b > 0.0 ? 1.0 / b : 0.0;
+  for (int i = 0; i < nv; i++) g[(long)k * nv + i] = i == 0 ? b : 0.0;
+}
+
+// Column j of H_k: the two Gram-Schmidt passes' coefficients (hd, hd + K nv)
+// and the new vector's norm (nrm2[k nv + j + 1]); the previous rotations, a
+// new one that zeroes H(j + 1, j), and g_k rotated with it.  scale[k] =
+// 1 / H(j + 1, j) normalises the new basis vector; active[j] counts the
+// right-hand sides whose estimated residual |g(j + 1)| is still above
+// tol * beta (and whose Krylov space has not broken down).
+__global__ void gmres_rotate_kernel(int K, int nv, int m, int j, const double *hd, const double *nrm2,
+                                    double *H, double *cs, double *sn, double *g, const double *beta,
+                                    double *scale, double tol, int *active) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  double *col = H + ((long)k * m + j) * nv;
+  for (int i = 0; i <= j; i++) col[i] = hd[(long)k * nv + i] + hd[(long)K * nv + (long)k * nv + i];
+  const double hn = sqrt(nrm2[(long)k * nv + j + 1]);
+  col[j + 1] = hn;
+  scale[k] = hn > 0.0 ? 1.0 / hn : 0.0;
+  double *c = cs + (long)k * m, *sv = sn + (long)k * m, *gk = g + (long)k * nv;
+  for (int i = 0; i < j; i++) {  // previous rotations
+    const double a = col[i], b = col[i + 1];
+    col[i] = c[i] * a + sv[i] * b;
+    col[i + 1] = -sv[i] * a + c[i] * b;
+  }
+  const double a = col[j], b = col[j + 1], rr = hypot(a, b);
+  c[j] = rr > 0.0 ? a / rr : 1.0;
+  sv[j] = rr > 0.0 ? b / rr : 0.0;
+  col[j] = rr;
+  col[j + 1] = 0.0;
+  gk[j + 1] = -sv[j] * gk[j];
+  gk[j] = c[j] * gk[j];
+  if (!(fabs(gk[j + 1]) <= tol * beta[k]) && hn > 0.0) atomicAdd(active + j, 1);
+}
+
+// y_k = H_k^-1 g_k over the first `used` columns (upper triangular after
+// the rotations); a zero pivot (breakdown) gives a zero coefficient.
+__global__ void gmres_solve_y_kernel(int K, int nv, int m, int used, const double *H, const double *g,
+                                     double *y) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const double *Hk = H + (long)k * m * nv;
+  double *yk = y + (long)k * nv;
+  for (int i = nv - 1; i >= used; i--) yk[i] = 0.0;
+  for (int i = used - 1; i >= 0; i--) {
+    double t = g[(long)k * nv + i];
+    for (int c = i + 1; c < used; c++) t -= Hk[(long)c * nv + i] * yk[c];
+    const double d = Hk[(long)i * nv + i];
+    yk[i] = d != 0.0 ? t / d : 0.0;
+  }
+}
+
 }  // namespace
+
+hipError_t gmres_start(hipStream_t s, long nrhs, int nv, const double *nrm2, double *beta, double *g,
+                       double *scale) {
+  if (nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gmres_start_kernel, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, (int)nrhs, nv,
+                     nrm2, beta, g, scale);
+  return hipGetLastError();
+}
+
+hipError_t gmres_rotate(hipStream_t s, long nrhs, int nv, int m, int j, const double *hd, const double *nrm2,
+                        double *H, double *cs, double *sn, double *g, const double *beta, double *scale,
+                        double tol, int *active) {
+  if (nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gmres_rotate_kernel, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, (int)nrhs, nv,
+                     m, j, hd, nrm2, H, cs, sn, g, beta, scale, tol, active);
+  return hipGetLastError();
+}
+
+hipError_t gmres_solve_y(hipStream_t s, long nrhs, int nv, int m, int used, const double *H, const double *g,
+                         double *y) {
+  if (nrhs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gmres_solve_y_kernel, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, (int)nrhs, nv,
+                     m, used, H, g, y);
+  return hipGetLastError();
+}
 
 hipError_t basis_dots(hipStream_t s, long n, long nrhs, int nv, int first, int nvec, const double *V,
                       const double *W, long ldw, double *out) {

@@ -81,8 +81,16 @@ constexpr long kGmresOuterMax = 10;
 
 // Workspace layout of the mixed solver (byte offsets, 256-aligned).
 struct SposvLayout {
-  size_t S, W, T, Y, R, RS, PART, NORM, DEV, P16, V, WV, HD, SYNC, total;
+  size_t S, W, T, Y, R, RS, PART, NORM, DEV, P16, V, WV, HD, GM, SYNC, total;
 };
+
+// Device GMRES state per right-hand side (doubles): ones, beta, g (nv), y
+// (nv), H (m nv), cs (m), sn (m); then m ints of convergence counts.
+size_t gmres_state_bytes(long nrhs) {
+  constexpr int NV = kGmresM + 1;
+  return (size_t)nrhs * (2 + 2 * NV + (size_t)kGmresM * NV + 2 * kGmresM) * sizeof(double) +
+         kGmresM * sizeof(int);
+}
 
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -103,6 +111,7 @@ SposvLayout sposv_layout(CULAPACKhandle h, long n, long nrhs) {
   L.V = off;    off = align256(off + (size_t)nrhs * (kGmresM + 1) * n * sizeof(double));  // Krylov bases
   L.WV = off;   off = align256(off + (size_t)n * nrhs * sizeof(double));
   L.HD = off;   off = align256(off + 3 * (size_t)nrhs * (kGmresM + 1) * sizeof(double));
+  L.GM = off;   off = align256(off + gmres_state_bytes(nrhs));
   L.SYNC = off; off = align256(off + hc::trsm_multi_sync_bytes(n, nrhs));
   L.total = off;
   return L;
@@ -273,39 +282,33 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
       HC_CHECK(hc::s2d(s, N, K, RS, N, w, N));
       return CUDA_SUCCESS;
     };
-    std::vector<double> hbuf(2 * (size_t)K * NV), scale(K), beta(K);
-    auto fetch = [&](const double *d, size_t count, double *hst) -> CUresult {
-      HC_CHECK(hipMemcpyAsync(hst, d, count * sizeof(double), hipMemcpyDeviceToHost, s));
-      HC_CHECK(hipStreamSynchronize(s));
-      return CUDA_SUCCESS;
-    };
-    auto upload = [&](double *d, const double *hst, size_t count) -> CUresult {
-      HC_CHECK(hipMemcpyAsync(d, hst, count * sizeof(double), hipMemcpyHostToDevice, s));
-      HC_CHECK(hipStreamSynchronize(s));
-      return CUDA_SUCCESS;
-    };
-    // One restarted GMRES cycle on inv(M) A d = inv(M) r (M = S S^T), x += d.
-    // Classical Gram-Schmidt with one reorthogonalisation, Givens rotations on
-    // the Hessenberg matrix, every right-hand side its own Krylov space.
+    // GMRES state on the device (gmres_* kernels): the Hessenberg columns,
+    // rotations and basis scales never leave it.  The host reads one int per
+    // inner iteration, the number of right-hand sides not yet converged, to
+    // decide whether to queue the next one (reading it one iteration late
+    // instead cost a whole extra iteration per cycle: ~5 ms at n = 32768).
+    double *Gs = reinterpret_cast<double *>(wsb + L.GM);
+    double *g_scale = HD + 2 * K * NV;  // basis scales
+    double *ones = Gs, *beta = ones + K, *gv = beta + K, *yv = gv + K * NV, *Hm = yv + K * NV;
+    double *csv = Hm + K * kGmresM * NV, *snv = csv + K * kGmresM;
+    int *act = reinterpret_cast<int *>(snv + K * kGmresM);
+    bool ones_ready = false;
+    hipEvent_t ev = nullptr;
     auto gmres_cycle = [&]() -> CUresult {
+      if (!ones_ready) {
+        const std::vector<double> one(K, 1.0);
+        HC_CHECK(hipMemcpyAsync(ones, one.data(), K * sizeof(double), hipMemcpyHostToDevice, s));
+        HC_CHECK(hipStreamSynchronize(s));
+        HC_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        ones_ready = true;
+      }
       HC_CHECK(hipMemcpyAsync(WV, R, (size_t)N * K * sizeof(double), hipMemcpyDeviceToDevice, s));
       CU_TRY(precondition(WV));
-      std::fill(scale.begin(), scale.end(), 1.0);
-      CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
-      HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, HD + 2 * K * NV, V));
+      HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, ones, V));
       HC_CHECK(hc::basis_dots(s, N, K, NV, 0, 1, V, WV, N, HD));
-      CU_TRY(fetch(HD, (size_t)K * NV, hbuf.data()));
-      std::vector<std::vector<double>> H(K, std::vector<double>((size_t)NV * kGmresM, 0.0));
-      std::vector<std::vector<double>> g(K, std::vector<double>(NV, 0.0)), cs(K), sn(K);
-      for (long k = 0; k < K; k++) {
-        beta[k] = std::sqrt(hbuf[k * NV]);
-        scale[k] = beta[k] > 0.0 ? 1.0 / beta[k] : 0.0;
-        g[k][0] = beta[k];
-        cs[k].assign(kGmresM, 1.0);
-        sn[k].assign(kGmresM, 0.0);
-      }
-      CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
-      HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, HD + 2 * K * NV, V));
+      HC_CHECK(hc::gmres_start(s, K, NV, HD, beta, gv, g_scale));
+      HC_CHECK(hipMemsetAsync(act, 0, kGmresM * sizeof(int), s));
+      HC_CHECK(hc::basis_put(s, N, K, NV, 0, WV, N, g_scale, V));
       int used_j = 0;
       for (int j = 0; j < kGmresM; j++) {
         HC_CHECK(symm_apply(s, ul, N, K, Ad, (long)lda, V + (long)j * N, (long)NV * N, WV, N, PART));
@@ -315,53 +318,28 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
           HC_CHECK(hc::basis_dots(s, N, K, NV, 0, j + 1, V, WV, N, hd));
           HC_CHECK(hc::basis_sub(s, N, K, NV, j + 1, V, hd, WV, N));
         }
-        CU_TRY(fetch(HD, 2 * (size_t)K * NV, hbuf.data()));
-        std::fill(scale.begin(), scale.end(), 1.0);
-        CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
-        HC_CHECK(hc::basis_put(s, N, K, NV, j + 1, WV, N, HD + 2 * K * NV, V));
-        HC_CHECK(hc::basis_dots(s, N, K, NV, j + 1, 1, V, WV, N, HD));
-        std::vector<double> nrm(K * NV);
-        CU_TRY(fetch(HD, (size_t)K * NV, nrm.data()));
-        bool all_done = true;
-        for (long k = 0; k < K; k++) {
-          double *col = H[k].data() + (size_t)j * NV;  // column j of H_k
-          for (int i = 0; i <= j; i++) col[i] = hbuf[k * NV + i] + hbuf[K * NV + k * NV + i];
-          const double hn = std::sqrt(nrm[k * NV + j + 1]);
-          col[j + 1] = hn;
-          scale[k] = hn > 0.0 ? 1.0 / hn : 0.0;
-          for (int i = 0; i < j; i++) {  // previous rotations
-            const double a = col[i], b = col[i + 1];
-            col[i] = cs[k][i] * a + sn[k][i] * b;
-            col[i + 1] = -sn[k][i] * a + cs[k][i] * b;
-          }
-          const double a = col[j], b = col[j + 1], rr = std::hypot(a, b);
-          cs[k][j] = rr > 0.0 ? a / rr : 1.0;
-          sn[k][j] = rr > 0.0 ? b / rr : 0.0;
-          col[j] = rr;
-          col[j + 1] = 0.0;
-          g[k][j + 1] = -sn[k][j] * g[k][j];
-          g[k][j] = cs[k][j] * g[k][j];
-          if (!(std::fabs(g[k][j + 1]) <= kGmresTol * beta[k]) && hn > 0.0) all_done = false;
-        }
-        CU_TRY(upload(HD + 2 * K * NV, scale.data(), K));
-        HC_CHECK(hc::basis_put(s, N, K, NV, j + 1, WV, N, HD + 2 * K * NV, V));
+        HC_CHECK(hc::basis_put(s, N, K, NV, j + 1, WV, N, ones, V));
+        HC_CHECK(hc::basis_dots(s, N, K, NV, j + 1, 1, V, WV, N, HD));  // ||w||^2 at slot j + 1
+        HC_CHECK(hc::gmres_rotate(s, K, NV, kGmresM, j, HD, HD, Hm, csv, snv, gv, beta, g_scale, kGmresTol,
+                                  act));
+        HC_CHECK(hc::basis_put(s, N, K, NV, j + 1, WV, N, g_scale, V));
+        HC_CHECK(hipMemcpyAsync(h->hinfo + 1, act + j, sizeof(int), hipMemcpyDeviceToHost, s));
+        HC_CHECK(hipEventRecord(ev, s));
         used_j = j + 1;
-        if (all_done) break;
+        HC_CHECK(hipEventSynchronize(ev));
+        if (h->hinfo[1] == 0) break;  // every right-hand side converged
       }
       // y = H^-1 g (upper triangular), then x += V y
-      std::vector<double> y((size_t)K * NV, 0.0);
-      for (long k = 0; k < K; k++) {
-        for (int i = used_j - 1; i >= 0; i--) {
-          double t = g[k][i];
-          for (int c = i + 1; c < used_j; c++) t -= H[k][(size_t)c * NV + i] * y[k * NV + c];
-          const double d = H[k][(size_t)i * NV + i];
-          y[k * NV + i] = d != 0.0 ? t / d : 0.0;
-        }
-      }
-      CU_TRY(upload(HD, y.data(), (size_t)K * NV));
-      HC_CHECK(hc::basis_combine(s, N, K, NV, used_j, V, HD, Xd, (long)ldx));
+      HC_CHECK(hc::gmres_solve_y(s, K, NV, kGmresM, used_j, Hm, gv, yv));
+      HC_CHECK(hc::basis_combine(s, N, K, NV, used_j, V, yv, Xd, (long)ldx));
       return CUDA_SUCCESS;
     };
+    struct EventGuard {
+      hipEvent_t *e;
+      ~EventGuard() {
+        if (*e != nullptr) (void)hipEventDestroy(*e);
+      }
+    } event_guard{&ev};
 
     Norms nm;
     const long itmax = refine == 'G' ? kGmresOuterMax : kIterMax;

@@ -353,21 +353,29 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
     }
   }
   __syncthreads();
-  acc.zero();
+  // four independent accumulators (k = 16 u' + 4 t + g, t = u mod 4): the
+  // product is on the chain, and one accumulator would serialise its 32
+  // matrix-core steps
+  {
+    Acc16<T> a4[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) a4[t].zero();
 #pragma unroll 8
-  for (int u = 0; u < SB / 4; u++) {
-    const int k = 4 * u + lg;
-    acc.mma(sm.W[k * WLD + row], sm.Y[lc * (SB + 1) + k]);
+    for (int u = 0; u < SB / 4; u++) {
+      const int k = 4 * u + lg;
+      a4[u & 3].mma(sm.W[k * WLD + row], sm.Y[lc * (SB + 1) + k]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc.v[r] = (a4[0].v[r] + a4[1].v[r]) + (a4[2].v[r] + a4[3].v[r]);
   }
-  // X_i to X and to the staging copy (rows >= ni and columns >= nrhs are
-  // exact zeros: B, the sum and inv(M_ii) are zero there)
+  // X_i to the staging copy (rows >= ni and columns >= nrhs are exact zeros:
+  // B, the sum and inv(M_ii) are zero there)
   {
     T *xt = stage_block<T>(sync, gridDim.y, nblk, y, i);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int k = 16 * wave + Acc16<T>::row(lane, r), col = c0 + lc;
+      const int k = 16 * wave + Acc16<T>::row(lane, r);
       xt[stage_index<T, KS>(k, lc)] = acc.v[r];
-      if (k < ni && col < nrhs) X[(long)col * ldx + ri + k] = acc.v[r];
     }
   }
   // publish: every wave waits for its own global stores to reach the L2 (the
@@ -378,6 +386,13 @@ trsm_chain_kernel(int lowerlike, int n, int nrhs, const T *A, long lda, const T 
   if (wave == 0) {
     publish_release();
     if (lane == 0) __hip_atomic_store(ready + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // X_i in the caller's layout after the flag: no workgroup reads it, so its
+  // stores stay off the chain
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int k = 16 * wave + Acc16<T>::row(lane, r), col = c0 + lc;
+    if (k < ni && col < nrhs) X[(long)col * ldx + ri + k] = acc.v[r];
   }
 }
 
