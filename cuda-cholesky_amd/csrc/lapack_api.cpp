@@ -115,11 +115,12 @@ hipError_t factor_panel(CULAPACKhandle h, hipStream_t s, bool lower, long n, T *
 
 // Right-looking blocked Cholesky with look-ahead (the reference's loop is
 // left-looking, lapack/dpotrf.c:248-286; both produce the same factor).
-// With panel j factored, the main stream first applies it to the next
-// panel's columns only, signals, and then runs the large trailing SYRK/HERK
-// on everything beyond; meanwhile the look-ahead stream factors the next
-// panel.  The two streams touch disjoint column ranges, and each panel is
-// joined back into the main stream before its own trailing update.
+// With panel j factored, the main stream applies it to the next panel's
+// columns only (complex: the look-ahead stream does, see below) and runs the
+// large trailing SYRK/HERK on everything beyond; the look-ahead stream
+// factors the next panel.  The two streams touch disjoint column ranges, and
+// each panel is joined back into the main stream before its own trailing
+// update.
 // Event pair around one trailing update when the handle's timing is on; the
 // event pool grows on demand (outside any timed work it is already sized).
 hipError_t timing_begin(CULAPACKhandle h, hipStream_t s, double flops) {
@@ -197,27 +198,46 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     hc::GemmOpts u;
     u.abort_flag = h->dinfo;
     u.bf16 = bf16;
+    // The next panel's update needs this panel and the previous trailing
+    // update; the trailing update beyond the next panel needs only those
+    // too.  On the look-ahead stream (ahead of the next panel's
+    // factorisation) it runs beside the trailing update instead of before
+    // it.  Measured on MI355X that pays for the complex factorisation
+    // (zpotrf n = 8192: 20.1 -> 19.9 ms) but not for the real one, whose
+    // look-ahead chain is the longer branch late in the factorisation and
+    // then starts behind a slowed update (dpotrf n = 16384: 33.5 -> 34.2 ms;
+    // 33.6-33.8 ms when only the early, long trailing updates get it); the
+    // FP32 / 16-bit factorisation's look-ahead stream is already the busier.
+    const bool upd_side = !bf16 && hc::is_complex<T>::value;
+    hipStream_t su = s0;
+    if (upd_side) {
+      HC_TRY(hipEventRecord(h->ev_update, s0));
+      HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
+      su = s1;
+    }
     if (lower) {  // next panel columns: A[jn:, jn:jn+jbn] -= P[jn:] P[jn:jn+jbn]^H
       u.tri_out = hc::TRI_LOWER;
       bool done = false;
       if constexpr (std::is_same<T, float>::value) {
         if (bf16) {
           const unsigned short *Pn = Pt[pj] + (jn - j) * kk, *Qn = Qt[pj] + (jn - j) * kk;
-          HC_TRY(hc::gemm_bf16_nt(s0, n - jn, jbn, kk, lp_alpha, Pn, kk, Qn, kk, 1.0f, at(jn, jn), lda,
+          HC_TRY(hc::gemm_bf16_nt(su, n - jn, jbn, kk, lp_alpha, Pn, kk, Qn, kk, 1.0f, at(jn, jn), lda,
                                   at(jn, jn), lda, hc::TRI_LOWER, false, h->dinfo, f16));
           done = true;
         }
       }
       if (!done)
-        HC_TRY(hc::gemm<T>(s0, hc::OP_N, conj_op<T>(), n - jn, jbn, jb, mone, at(jn, j), lda, at(jn, j),
+        HC_TRY(hc::gemm<T>(su, hc::OP_N, conj_op<T>(), n - jn, jbn, jb, mone, at(jn, j), lda, at(jn, j),
                            lda, one, at(jn, jn), lda, at(jn, jn), lda, u));
     } else {
       u.tri_out = hc::TRI_UPPER;
-      HC_TRY(hc::gemm<T>(s0, conj_op<T>(), hc::OP_N, jbn, n - jn, jb, mone, at(j, jn), lda, at(j, jn),
+      HC_TRY(hc::gemm<T>(su, conj_op<T>(), hc::OP_N, jbn, n - jn, jb, mone, at(j, jn), lda, at(j, jn),
                          lda, one, at(jn, jn), lda, at(jn, jn), lda, u));
     }
-    HC_TRY(hipEventRecord(h->ev_update, s0));
-    HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
+    if (!upd_side) {
+      HC_TRY(hipEventRecord(h->ev_update, s0));
+      HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
+    }
     HC_TRY(factor_panel<T>(h, s1, lower, n, A, lda, W, jn, jbn));
     if (bf16) HC_TRY(convert(s1, n - jn, jbn, at(jn, jn), pj ^ 1, jn));
     HC_TRY(hipEventRecord(h->ev_panel, s1));
