@@ -157,8 +157,8 @@ def tune_batched(torch, pkg, h, count, reps, per_cu_max=6):
 
 # family: (routine, orders, candidate thresholds); 1 << 30: never the larger tile
 TILE_SWEEP = {
-    "D": ("D", [4096, 8192, 16384], [60, 120, 240, 480, 1 << 30]),
-    "Z": ("Z", [2048, 4096, 8192], [60, 120, 240, 480, 1 << 30]),
+    "D": ("D", [4096, 8192, 16384], [120, 240, 480, 960, 1920, 1 << 30]),
+    "Z": ("Z", [2048, 4096, 8192], [120, 240, 480, 960, 1440, 1920, 1 << 30]),
     "S": ("S", [4096, 8192, 16384], [60, 120, 240, 480, 1 << 30]),
     "H": ("H", [8192, 16384], [120, 240, 480, 1 << 30]),
     "W": ("H", [16384, 32768], [128, 256, 512, 1024, 1 << 30]),
