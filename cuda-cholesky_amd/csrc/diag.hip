@@ -15,6 +15,7 @@
 //   logdet       2 sum log(Re x_i)  (reference lapack/dlogdet.cu:53-118)
 #include "internal.h"
 #include "tile16.h"
+#include "publish.h"
 
 #include <climits>
 
@@ -361,31 +362,21 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
 template <typename T>
 constexpr int diag_waves_per_eu() { return is_complex<T>::value ? 2 : 4; }
 
+// Factor and invert the lower-form block held in S (the packed tiles above,
+// already loaded, identity padding beyond n; `flag` an LDS word).  The factor
+// goes to A (lower form: A(r, c), or conj into A(c, r) when !lower) and
+// inv(L) densely to W (ld ldw, zeros above the diagonal, n x n only).
+// Returns -1, or the first non-positive / NaN pivot (0-based, the same value
+// in every thread; the factor is then only partly written).
 template <typename T, typename ST, int NMAX>
-__global__ void __launch_bounds__(BLK_THREADS)
-__attribute__((amdgpu_waves_per_eu(diag_waves_per_eu<T>(), diag_waves_per_eu<T>())))
-diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info, long info_off) {
-  if (*info != 0) return;
-  // This block is on the critical path of the factorisation while trailing-
-  // update waves share its SIMDs: take instruction issue ahead of them.
-  __builtin_amdgcn_s_setprio(3);
+__device__ __forceinline__ int diag_block_factor(T *S, int &flag, int lower, int n, ST *A, long lda, ST *W,
+                                                 long ldw) {
   constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL, NW = BLK_THREADS / 64;
-  __shared__ T S[NPMAX * (NPMAX + 1) / 2 * TS];
-  __shared__ int flag;
+  static_assert(NPMAX <= NW + 1, "the inverse gives each wave at most one tile");
   const int tid = threadIdx.x, wave = tid >> 6;
-  const int np = (n + LEAF - 1) / LEAF, npad = np * LEAF;
+  const int np = (n + LEAF - 1) / LEAF;
   auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
   auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
-  // identity padding (rows / columns n .. npad - 1 of the lower form)
-  for (int c = tid; c < npad; c += BLK_THREADS)
-    for (int r = max(n, c); r < npad; r++) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
-  // lower form, read along memory's contiguous index (i), the loads of a
-  // thread in flight together
-  // every load of a thread in flight at once (NMAX^2 / BLK_THREADS of them)
-  gather_block<NMAX, BLK_THREADS, NMAX * NMAX / BLK_THREADS>(n, A, lda, [&](int i, int k, ST x) {
-    const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
-    if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
-  });
   if (tid == 0) flag = -1;
   __syncthreads();
   // One leaf of look-ahead: while wave 0 updates tile (p+1, p+1) and factors
@@ -425,10 +416,8 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
       }
     }
     __syncthreads();
-    if (flag >= 0) {  // leaf p + 1 failed (padding never fails, so flag < n)
-      if (tid == 0) *info = (int)(info_off + flag + 1);
-      return;
-    }
+    const int f = flag;
+    if (f >= 0) return f;  // leaf p + 1 failed (padding never fails, so f < n)
   }
   // off-diagonal tiles of the factor (the diagonal ones are already out)
   for_block<NMAX, BLK_THREADS>(n, [&](int i, int k) {
@@ -463,6 +452,363 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
   for_block<NMAX, BLK_THREADS>(n, [&](int r, int c) {
     W[(long)c * ldw + r] = convert<ST>(r >= c ? elem(r, c) : from_real<T>(0.0));
   });
+  return -1;
+}
+
+// Bytes of the packed lower-triangle tiles of a diagonal block of order NMAX.
+template <typename T, int NMAX>
+constexpr int diag_tiles_bytes() {
+  return (NMAX / LEAF) * (NMAX / LEAF + 1) / 2 * LEAF * (LEAF + 1) * (int)sizeof(T);
+}
+
+template <typename T, typename ST, int NMAX>
+__global__ void __launch_bounds__(BLK_THREADS)
+__attribute__((amdgpu_waves_per_eu(diag_waves_per_eu<T>(), diag_waves_per_eu<T>())))
+diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info, long info_off) {
+  if (*info != 0) return;
+  // This block is on the critical path of the factorisation while trailing-
+  // update waves share its SIMDs: take instruction issue ahead of them.
+  __builtin_amdgcn_s_setprio(3);
+  constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL;
+  __shared__ T S[NPMAX * (NPMAX + 1) / 2 * TS];
+  __shared__ int flag;
+  const int tid = threadIdx.x;
+  const int np = (n + LEAF - 1) / LEAF, npad = np * LEAF;
+  auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
+  auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
+  // identity padding (rows / columns n .. npad - 1 of the lower form)
+  for (int c = tid; c < npad; c += BLK_THREADS)
+    for (int r = max(n, c); r < npad; r++) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
+  // lower form, read along memory's contiguous index (i); every load of a
+  // thread in flight at once (NMAX^2 / BLK_THREADS of them)
+  gather_block<NMAX, BLK_THREADS, NMAX * NMAX / BLK_THREADS>(n, A, lda, [&](int i, int k, ST x) {
+    const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
+    if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
+  });
+  const int bad = diag_block_factor<T, ST, NMAX>(S, flag, lower, n, A, lda, W, ldw);
+  if (bad >= 0 && tid == 0) *info = (int)(info_off + bad + 1);
+}
+
+// ---------------------------------------------------------------------------
+// A whole panel of the blocked factorisation in one launch (lower, real):
+// the m x jb panel at A, its top jb x jb block on the diagonal, becomes L,
+// left-looking by 128-column blocks J = 0 .. C-1 (C = ceil(jb / 128)).
+// The reference factors a panel as a chain of diagonal-block, panel-solve
+// and panel-update calls (lapack/dpotrf.c:268-326); here one workgroup owns
+// each 128-row block: row block b < C is the diagonal part (rows 128 b ..),
+// row block b >= C the rows jb + 128 (b - C) .. below it.  For every column
+// block J left of its rows, workgroup b
+//   * once row block J has published its off-diagonal blocks L_Jk (k < J),
+//     accumulates S = sum_{k<J} L_bk L_Jk^T (K = 128 J, its own L_bk
+//     re-read from memory);
+//   * once L_JJ and inv(L_JJ) are published, stores X = A_bJ - S in place and
+//     then L_bJ = X inv(L_JJ)^T.
+// A diagonal workgroup b then publishes its off-diagonal blocks, factors and
+// inverts A_bb - sum_k L_bk L_bk^T with diag_block_factor (FP64 for FP32
+// storage; the partial sum over k < b - 1 is formed ahead of the wait for
+// block b - 1 and parked in the workspace) and publishes L_bb and inv(L_bb).
+// The critical path per column block is one flag hand-off, two 128^3
+// products and the diagonal block; the other workgroups' accumulations run
+// while a diagonal block is factored.
+//
+// Dependencies only point to lower workgroup indices, which the hardware
+// dispatches first, so every awaited workgroup is resident or finished.
+// Flags are set to the launch's epoch (a per-handle counter, so they never
+// need resetting); a wait gives up when the info word becomes nonzero (a
+// failed pivot anywhere) or after about four seconds, which it reports as
+// kInfoAbandoned.  Publishing: every wave waits for its own stores, then one
+// wave writes the L2 back once and sets the flag; waiting: one wave polls,
+// then acquires with one agent-scope fence before the workgroup barrier.
+// ---------------------------------------------------------------------------
+typedef float pv4f __attribute__((ext_vector_type(4)));
+typedef double pv4d __attribute__((ext_vector_type(4)));
+typedef double pv2d __attribute__((ext_vector_type(2)));
+
+constexpr int PK_THREADS = 512;
+constexpr int PK_B = 128;          // rows and columns of a block
+constexpr int PK_FLAG_STRIDE = 32; // ints between flags (one 128-byte line each)
+
+// Matrix-core arithmetic of the panel products per storage type: K step KT
+// staged in LDS as rows of LK elements (K contiguous), a 16-byte fragment of
+// FV consecutive k per lane, and the result column of register r in lane
+// group g (the row is lane % 16).
+template <typename ST> struct PanelMma;
+template <> struct PanelMma<float> {
+  using V = pv4f;
+  using F = pv4f;
+  static constexpr int KT = 32, LK = KT + 4, FV = 4;
+  __device__ static V mma(float b, float a, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c, 0, 0, 0); }
+  __device__ static int col(int g, int r) { return 4 * g + r; }
+};
+template <> struct PanelMma<double> {
+  using V = pv4d;
+  using F = pv2d;
+  static constexpr int KT = 16, LK = KT + 2, FV = 2;
+  __device__ static V mma(double b, double a, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, c, 0, 0, 0); }
+  __device__ static int col(int g, int r) { return g + 4 * r; }
+};
+
+// 128 x 128 accumulator of the workgroup: 8 waves in 2 x 4, each 64 x 32 as
+// 4 x 2 blocks of 16 x 16.
+template <typename ST> struct PanelAcc {
+  typename PanelMma<ST>::V v[4][2];
+  __device__ void zero() {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[i][j][r] = ST(0);
+  }
+  // element <- fn(row, col, element) over the thread's 32 elements
+  template <typename Fn> __device__ void map(Fn fn) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // indices formed per call, not hoisted by the caller
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lc = lane & 15, lg = lane >> 4, wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[i][j][r] = fn(wm0 + 16 * i + lc, wn0 + 16 * j + PanelMma<ST>::col(lg, r), v[i][j][r]);
+  }
+  // fn(row, col, C(row, col) - element) where ok(row, col), C(r, c) at
+  // Cp[c * ldc + r]; C is read in two halves with all loads of a half in
+  // flight before its results are used (fn may store into C)
+  template <typename Ok, typename Fn> __device__ void sub_from(const ST *Cp, long ldc, Ok ok, Fn fn) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lc = lane & 15, lg = lane >> 4, wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      ST cv[16];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int row = wm0 + 16 * (2 * h + i) + lc, col = wn0 + 16 * j + PanelMma<ST>::col(lg, r);
+            cv[8 * i + 4 * j + r] = Cp[ok(row, col) ? (long)col * ldc + row : 0];
+          }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int row = wm0 + 16 * (2 * h + i) + lc, col = wn0 + 16 * j + PanelMma<ST>::col(lg, r);
+            fn(row, col, cv[8 * i + 4 * j + r] - v[2 * h + i][j][r]);
+          }
+    }
+  }
+};
+
+// acc += P Q^T over k < K: P(r, k) = Pp[k * ldp + r] (r < rp), Q(c, k) =
+// Qp[k * ldq + c] (c < rq); k >= klim reads as zero.  K is a multiple of KT.
+// Both operands are staged through LDS (double-buffered, the next step's
+// global loads in registers during this step's products); returns with the
+// LDS free (after a barrier).
+template <typename ST>
+__device__ __forceinline__ void panel_nt(PanelAcc<ST> &acc, const ST *Pp, long ldp, int rp, const ST *Qp, long ldq,
+                                         int rq, int K, int klim, ST *lds) {
+  using M = PanelMma<ST>;
+  constexpr int KT = M::KT, LK = M::LK, FV = M::FV, E = KT / 4;
+  using F = typename M::F;
+  ST *sP = lds, *sQ = lds + 2 * PK_B * LK;
+  int tid = threadIdx.x;
+  // thread-derived values (and the addresses built from them) are formed here
+  // at every call, not hoisted out of the caller's loop and kept live across it
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = tid & (PK_B - 1), kq = tid >> 7;  // loader: row lr, k = kq E .. kq E + E - 1
+  const int lc = lane & 15, lg = lane >> 4, wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
+  ST rp_[E], rq_[E];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int k = k0 + kq * E + e;
+      const bool okp = lr < rp && k < klim, okq = lr < rq && k < klim;
+      const ST a = Pp[okp ? (long)k * ldp + lr : 0];
+      const ST b = Qp[okq ? (long)k * ldq + lr : 0];
+      rp_[e] = okp ? a : ST(0);
+      rq_[e] = okq ? b : ST(0);
+    }
+  };
+  auto store = [&](int buf) {
+    ST *dp = sP + buf * PK_B * LK + lr * LK + kq * E, *dq = sQ + buf * PK_B * LK + lr * LK + kq * E;
+#pragma unroll
+    for (int e = 0; e < E; e += FV) {
+      F a, b;
+#pragma unroll
+      for (int t = 0; t < FV; t++) { a[t] = rp_[e + t]; b[t] = rq_[e + t]; }
+      *reinterpret_cast<F *>(dp + e) = a;
+      *reinterpret_cast<F *>(dq + e) = b;
+    }
+  };
+  const int nk = K / KT;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const bool more = kt + 1 < nk;
+    if (more) load((kt + 1) * KT);
+    const ST *cP = sP + (kt & 1) * PK_B * LK, *cQ = sQ + (kt & 1) * PK_B * LK;
+#pragma unroll
+    for (int h = 0; h < KT; h += 4 * FV) {
+      F fp[4], fq[2];
+#pragma unroll
+      for (int i = 0; i < 4; i++) fp[i] = *reinterpret_cast<const F *>(cP + (wm0 + 16 * i + lc) * LK + h + FV * lg);
+#pragma unroll
+      for (int j = 0; j < 2; j++) fq[j] = *reinterpret_cast<const F *>(cQ + (wn0 + 16 * j + lc) * LK + h + FV * lg);
+#pragma unroll
+      for (int t = 0; t < FV; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) acc.v[i][j] = M::mma(fq[j][t], fp[i][t], acc.v[i][j]);
+    }
+    if (more) store((kt + 1) & 1);
+    __syncthreads();
+  }
+}
+
+// One wave polls flag f until it holds `epoch` (or info becomes nonzero, or
+// the wait times out), then acquires; the result (true: proceed) is the
+// same in every thread.
+__device__ __forceinline__ bool panel_wait(const int *f, int epoch, int *info, int &sh) {
+  if (threadIdx.x < 64) {
+    int st = 0;
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+      const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      const int iv = __builtin_amdgcn_readfirstlane(__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (iv != 0) { st = 2; break; }
+      if (v == epoch) { st = 1; break; }
+      if (__builtin_amdgcn_s_memrealtime() - t0 > 400000000ull) {  // ~4 s at 100 MHz
+        if (threadIdx.x == 0) {
+          int expected = 0;
+          __hip_atomic_compare_exchange_strong(info, &expected, kInfoAbandoned, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        st = 2;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (threadIdx.x == 0) sh = st;
+  }
+  __syncthreads();
+  return sh == 1;
+}
+
+__device__ __forceinline__ void panel_publish(int *f, int epoch) {
+  publish_wait_own();
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    publish_release();
+    if (threadIdx.x == 0) __hip_atomic_store(f, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename ST>
+constexpr int panel_lds_bytes() {
+  constexpr int g = 4 * PK_B * PanelMma<ST>::LK * (int)sizeof(ST);
+  constexpr int d = diag_tiles_bytes<double, PK_B>();
+  return g > d ? g : d;
+}
+
+// ws: C inverses inv(L_JJ) (128 x 128, ld 128), then C partial sums (128 x 128).
+template <typename ST>
+__global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
+panel_potrf_kernel(int m, int jb, ST *A, long lda, ST *ws, int *flags, int epoch, int *info, long info_off) {
+  using T = double;  // the diagonal blocks are factored in FP64 for both storage types
+  if (*info != 0) return;
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ __attribute__((aligned(16))) char lds_raw[panel_lds_bytes<ST>()];
+  __shared__ int sh_wait, sh_flag;
+  ST *lds = reinterpret_cast<ST *>(lds_raw);
+  const int C = (jb + PK_B - 1) / PK_B, b = blockIdx.x;
+  const bool dg = b < C;
+  const int r0 = dg ? PK_B * b : jb + PK_B * (b - C);
+  const int ra = dg ? min(PK_B, jb - r0) : min(PK_B, m - r0);
+  const int nJ = dg ? b : C;
+  constexpr long BB = (long)PK_B * PK_B;
+  ST *Winv = ws, *Psum = ws + C * BB;
+  int *const off_flag = flags, *const diag_flag = flags + PK_FLAG_STRIDE;
+  auto fo = [&](int J) { return off_flag + 2 * PK_FLAG_STRIDE * J; };
+  auto fd = [&](int J) { return diag_flag + 2 * PK_FLAG_STRIDE * J; };
+  PanelAcc<ST> acc;
+  for (int J = 0; J < nJ; J++) {
+    const int cw = min(PK_B, jb - PK_B * J);
+    ST *AbJ = A + (long)PK_B * J * lda + r0;
+    if (dg && J == b - 1 && b >= 2) {  // P = sum_{k < b-1} L_bk L_bk^T (own rows, no wait)
+      acc.zero();
+      panel_nt<ST>(acc, A + r0, lda, ra, A + r0, lda, ra, PK_B * J, PK_B * J, lds);
+      ST *P = Psum + b * BB;
+      acc.map([&](int r, int c, ST x) { P[(long)c * PK_B + r] = x; return x; });
+    }
+    acc.zero();
+    if (J > 0) {
+      if (!panel_wait(fo(J), epoch, info, sh_wait)) return;
+      panel_nt<ST>(acc, A + r0, lda, ra, A + PK_B * J, lda, cw, PK_B * J, PK_B * J, lds);
+    }
+    if (!panel_wait(fd(J), epoch, info, sh_wait)) return;
+    // X = A_bJ - S in place (C and the sum kept apart: they nearly cancel)
+    acc.sub_from(AbJ, lda, [&](int r, int c) { return r < ra && c < cw; }, [&](int r, int c, ST v) {
+      if (r < ra && c < cw) AbJ[(long)c * lda + r] = v;
+    });
+    publish_wait_own();
+    __syncthreads();
+    // L_bJ = X inv(L_JJ)^T
+    acc.zero();
+    panel_nt<ST>(acc, AbJ, lda, ra, Winv + J * BB, PK_B, cw, PK_B, cw, lds);
+    acc.map([&](int r, int c, ST x) {
+      if (r < ra && c < cw) AbJ[(long)c * lda + r] = x;
+      return x;
+    });
+    publish_wait_own();
+    __syncthreads();
+  }
+  if (!dg) return;
+  // the diagonal workgroup's row block left of the diagonal is final
+  if (b > 0) panel_publish(fo(b), epoch);
+  // A_bb - sum_{k<b} L_bk L_bk^T into the packed FP64 tiles
+  ST *Abb = A + (long)r0 * lda + r0;
+  acc.zero();
+  if (b >= 2) {
+    const ST *P = Psum + b * BB;
+    acc.map([&](int r, int c, ST) { return P[(long)c * PK_B + r]; });
+  }
+  if (b >= 1) {
+    const ST *Lp = A + (long)PK_B * (b - 1) * lda + r0;
+    panel_nt<ST>(acc, Lp, lda, ra, Lp, lda, ra, PK_B, PK_B, lds);
+  }
+  {
+    constexpr int LL = LEAF + 1, TS = LEAF * LL;
+    T *S = reinterpret_cast<T *>(lds_raw);
+    const int npad = (ra + LEAF - 1) / LEAF * LEAF;
+    acc.sub_from(Abb, lda, [&](int r, int c) { return r < ra && c < ra && r >= c; }, [&](int r, int c, ST v) {
+      if (r >= c && r < npad && c < npad) {
+        T *tl = S + ((r / LEAF) * (r / LEAF + 1) / 2 + c / LEAF) * TS;
+        tl[(c % LEAF) * LL + r % LEAF] = (r < ra && c < ra) ? convert<T>(v) : from_real<T>(r == c ? 1.0 : 0.0);
+      }
+    });
+  }
+  const int bad = diag_block_factor<T, ST, PK_B>(reinterpret_cast<T *>(lds_raw), sh_flag, 1, ra, Abb, lda,
+                                                 Winv + b * BB, PK_B);
+  if (bad >= 0) {
+    if (threadIdx.x == 0) {
+      int expected = 0;
+      __hip_atomic_compare_exchange_strong(info, &expected, (int)(info_off + r0 + bad + 1), __ATOMIC_RELAXED,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  panel_publish(fd(b), epoch);
 }
 
 // Workgroup b inverts the diagonal block of order min(nb, ntot - b nb) at
@@ -729,6 +1075,20 @@ hipError_t logdet(hipStream_t s, const T *x, long incx, long n, double *partials
   hipLaunchKernelGGL(logdet_final_kernel, dim3(1), dim3(256), 0, s, partials, (int)blocks, out);
   return hipGetLastError();
 }
+
+template <typename ST>
+hipError_t panel_potrf(hipStream_t s, long m, long jb, ST *A, long lda, ST *ws, int *flags, int epoch, int *info,
+                       long info_off) {
+  if (jb <= 0) return hipSuccess;
+  if (jb > kPanelMaxCols || m < jb || m > (long)INT32_MAX) return hipErrorInvalidValue;
+  const long C = (jb + PK_B - 1) / PK_B, grid = C + (m - jb + PK_B - 1) / PK_B;
+  hipLaunchKernelGGL((panel_potrf_kernel<ST>), dim3((unsigned)grid), dim3(PK_THREADS), 0, s, (int)m, (int)jb, A,
+                     lda, ws, flags, epoch, info, info_off);
+  return hipGetLastError();
+}
+template hipError_t panel_potrf<float>(hipStream_t, long, long, float *, long, float *, int *, int, int *, long);
+template hipError_t panel_potrf<double>(hipStream_t, long, long, double *, long, double *, int *, int, int *,
+                                        long);
 
 #define HC_INST(T)                                                                            \
   template hipError_t diag_potrf<T>(hipStream_t, char, int, T *, long, T *, long, int *, long); \

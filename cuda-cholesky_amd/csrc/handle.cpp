@@ -43,6 +43,7 @@ static CUresult lapack_release(__culapackhandle_st *h) {
   std::free(h->tev);
   std::free(h->tflops);
   if (h->dinfo != nullptr) keep(hipFree(h->dinfo));
+  if (h->pflags != nullptr) keep(hipFree(h->pflags));
   if (h->hinfo != nullptr) keep(hipHostFree(h->hinfo));
   if (h->ev_panel != nullptr) keep(hipEventDestroy(h->ev_panel));
   if (h->ev_update != nullptr) keep(hipEventDestroy(h->ev_update));

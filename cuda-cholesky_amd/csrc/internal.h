@@ -186,6 +186,19 @@ template <typename T> constexpr int diag_max() { return is_complex<T>::value ? 6
 template <typename T>
 hipError_t diag_potrf(hipStream_t s, char uplo, int n, T *A, long lda, T *W, long ldw, int *info,
                       long info_off);
+// A whole panel in one launch (lower, FP32 or FP64 storage): the m x jb
+// panel at A (top jb x jb block on the diagonal, m >= jb, jb <=
+// kPanelMaxCols) becomes L.  ws: panel_ws_elems(jb) elements; flags:
+// panel_flag_ints ints, zero before the first launch, each launch with a
+// larger epoch than the last.  A failed pivot sets *info = info_off + k + 1;
+// a dependency wait abandoned after ~4 s sets kInfoAbandoned.
+constexpr long kPanelMaxCols = 2048;
+constexpr int kPanelFlagInts = 2 * 32 * (kPanelMaxCols / 128);
+constexpr int kInfoAbandoned = -2147483647 - 1;
+inline long panel_ws_elems(long jb) { return 2 * ((jb + 127) / 128) * 128L * 128L; }
+template <typename T>
+hipError_t panel_potrf(hipStream_t s, long m, long jb, T *A, long lda, T *ws, int *flags, int epoch, int *info,
+                       long info_off);
 // Invert the triangular block in place (B = inv(A), B may equal A) and, when
 // W != nullptr, also write inv of its lower form densely into W.
 template <typename T>
@@ -344,6 +357,9 @@ hipError_t gmres_solve_y(hipStream_t s, long nrhs, int nv, int m, int used, cons
 // Z (complex tile kernel, 64 vs 32), S (FP32 kernel, 128 vs 64), H (16-bit
 // kernel, 128 vs 64), W (16-bit kernel, 256 vs 128).
 long tuned_tile_min(char family, long dflt);
+// Whether each panel of the lower real factorisation of precision prec is
+// factored in one launch (panel_potrf).
+bool tuned_panel_launch(char prec, bool dflt);
 
 // Measured outer block size for precision D / Z / S / B at order n, 0 when
 // the tuning table has no entry (tuning.cpp).
@@ -405,6 +421,9 @@ struct __culapackhandle_st {
   size_t ws_bytes;
   int *dinfo;             // device info word
   int *hinfo;             // pinned host copy
+  int *pflags;            // one-launch panel factorisation: flags (zeroed once), then
+  void *pws;              // its workspace (allocated on first use)
+  int pepoch;             // epoch of the last panel launch
   long nb_override[4];    // outer block size set by cuLAPACKSetBlockSize (D, Z, S, B), 0 = tuned
   // optional timing of the factorisation's trailing updates (event pairs on
   // the main stream and the algorithmic flops of each launch)

@@ -157,6 +157,25 @@ hipError_t timing_end(CULAPACKhandle h, hipStream_t s) {
 // products from those copies on the BF16 matrix cores.  split: each copy is
 // the pair of rows [hi | hi | lo] and [hi | lo | hi] (x = hi + lo), and the
 // products run over 3 ldt: three BF16 products, ~16 significant bits.
+// Flags and workspace of the one-launch panel factorisation, allocated once
+// per handle (the flags zeroed then; every launch uses a new epoch).
+hipError_t panel_resources(CULAPACKhandle h) {
+  if (h->pflags != nullptr) return hipSuccess;
+  const size_t flag_bytes = (size_t)hc::kPanelFlagInts * sizeof(int);
+  const size_t ws_bytes = (size_t)hc::panel_ws_elems(hc::kPanelMaxCols) * sizeof(double);
+  void *p = nullptr;
+  HC_TRY(hipMalloc(&p, flag_bytes + ws_bytes));
+  const hipError_t e = hipMemset(p, 0, flag_bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return e;
+  }
+  h->pflags = static_cast<int *>(p);
+  h->pws = static_cast<char *>(p) + flag_bytes;
+  h->pepoch = 0;
+  return hipSuccess;
+}
+
 template <typename T>
 hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false,
                         void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f) {
@@ -187,7 +206,26 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     }
     return hipErrorInvalidValue;
   };
-  HC_TRY(factor_panel<T>(h, s0, lower, n, A, lda, W, 0, std::min(NB, n)));
+  // Lower real panels in one launch (panel_potrf) when the tuning table asks
+  // for it; by default a panel is the chain of diagonal-block and
+  // panel-product launches (factor_panel).  Measured on MI355X the one-launch
+  // panel is slower beside the trailing update: its resident workgroups keep
+  // the CUs the 16-bit update's workgroups need (one per CU, all of the LDS
+  // and registers), and its left-looking steps put two 128^3 products on one
+  // CU on the critical path (dpotrf n = 16384 34.5 ms against 33.5 ms; FP16
+  // solve n = 32768 59.8 ms against 57.6 ms).
+  const bool one_launch = lower && !hc::is_complex<T>::value && NB <= hc::kPanelMaxCols &&
+                          hc::tuned_panel_launch(prec_of<T>(bf16), false);
+  if (one_launch) HC_TRY(panel_resources(h));
+  auto panel = [&](hipStream_t s, long jj, long jbb) -> hipError_t {
+    if constexpr (!hc::is_complex<T>::value) {
+      if (one_launch)
+        return hc::panel_potrf<T>(s, n - jj, jbb, at(jj, jj), lda, static_cast<T *>(h->pws), h->pflags,
+                                  ++h->pepoch, h->dinfo, jj);
+    }
+    return factor_panel<T>(h, s, lower, n, A, lda, W, jj, jbb);
+  };
+  HC_TRY(panel(s0, 0, std::min(NB, n)));
   if (bf16) HC_TRY(convert(s0, n, std::min(NB, n), A, 0, 0));
   for (long j = 0; j < n; j += NB) {
     const int pj = (int)((j / NB) & 1);
@@ -238,7 +276,7 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
       HC_TRY(hipEventRecord(h->ev_update, s0));
       HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
     }
-    HC_TRY(factor_panel<T>(h, s1, lower, n, A, lda, W, jn, jbn));
+    HC_TRY(panel(s1, jn, jbn));
     if (bf16) HC_TRY(convert(s1, n - jn, jbn, at(jn, jn), pj ^ 1, jn));
     HC_TRY(hipEventRecord(h->ev_panel, s1));
     const long t0 = jn + jbn, t = n - t0;   // trailing matrix beyond the next panel
@@ -338,6 +376,10 @@ CUresult finish(CULAPACKhandle h, long *info) {
   HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HC_CHECK(hipStreamSynchronize(h->stream));
   *info = (long)h->hinfo[0];
+  if (h->hinfo[0] == hc::kInfoAbandoned) {  // a dependency wait of the panel kernel gave up
+    *info = 0;
+    return CUDA_ERROR_LAUNCH_FAILED;
+  }
   return CUDA_SUCCESS;
 }
 

@@ -265,6 +265,7 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipStreamSynchronize(s));
     finfo = h->hinfo[0];
+    if (finfo == hc::kInfoAbandoned) return CUDA_ERROR_LAUNCH_FAILED;  // a dependency wait gave up
     if (finfo != 0) { *code = -3; return CUDA_SUCCESS; }  // low-precision factorisation failed
     // x0 = inv(S S^T) b in FP32; the diagonal-block inverses serve every solve
     HC_CHECK(potrs_prepare<float>(s, 'L', N, S, N, static_cast<float *>(T)));
@@ -384,6 +385,10 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   HC_CHECK(hipStreamSynchronize(s));
   *info = h->hinfo[0];
+  if (*info == hc::kInfoAbandoned) {  // a dependency wait gave up
+    *info = 0;
+    return CUDA_ERROR_LAUNCH_FAILED;
+  }
   if (*info != 0) return CUDA_SUCCESS;
   HC_CHECK(hc::copy2d<double>(s, N, K, Bd, (long)ldb, Xd, (long)ldx));
   HC_CHECK(potrs_prepare<double>(s, ul, N, Aw, (long)lda, static_cast<double *>(T)));

@@ -2,19 +2,29 @@
 // and a few right-hand sides X (n x nrhs, 16 columns per slice): the
 // residual and the operator of the mixed-precision solver.
 //
-// Memory-bound (A is read once; 16 columns make 32 flops per element of A),
-// so the stored triangle is read ONCE: every 128 x 64 tile T of the lower
-// form (T(r, c) = A(r, c), or A(c, r) for upper storage) serves both
+// Bound by the matrix cores and HBM together (A is read once; 16 columns make
+// 32 multiply-adds per element of A, and the FP64 matrix cores of a CU consume
+// 16 bytes of A per clock at that rate), so the stored triangle is read ONCE:
+// every 64 x 64 tile T of the lower form (T(r, c) = A(r, c), or A(c, r) for
+// upper storage) serves both
 //   U: rows r  += T X_c         (the stored triangle)
 //   V: rows c  += T^T X_r       (its mirror image; strict part on the diagonal)
-// One workgroup per 512 x 512 region of the lower form keeps its U sums
-// (512 rows) and V sums (512 columns) in registers and writes them once as
-// partial sums; a second pass adds a row block's partials in a fixed order
-// (deterministic, no atomics).  Partials are 1/16 of the region's bytes.
+// One workgroup per 512 x 512 region of the lower form: waves 0-3 form U (16
+// rows of the tile each, one accumulator per row tile), waves 4-7 form V (16
+// columns each, one accumulator per column tile of the region); both write
+// their sums once as partial sums and a second pass adds a row block's
+// partials in a fixed order (deterministic, no atomics).  Partials are 1/16
+// of the region's bytes.
 //
-// Tiles go global -> registers -> LDS (double-buffered, the next tile's loads
-// in flight during the current tile's products); X comes from a compact copy
-// (16 consecutive values per row) so that its operand loads are contiguous.
+// Pipeline: the region's X rows (the V operand, 64 KiB) sit in LDS for the
+// whole region; each tile and its X columns (the U operand) go global ->
+// registers -> LDS, two tiles ahead in two register stages, so every product
+// reads LDS only and the only waits on global loads come after a tile's
+// products (the load counter completes in order: an operand load issued
+// after a prefetch would otherwise wait for the prefetch).  Row tiles are the
+// outer (rolled) loop and the 8 column tiles of a row the inner (unrolled)
+// one, so the V accumulators are indexed at compile time.  Tiles of a
+// diagonal region above its diagonal are neither loaded nor multiplied.
 // Products use v_mfma_f64_16x16x4_f64.
 #include "internal.h"
 
@@ -22,10 +32,11 @@ namespace hc {
 namespace {
 
 constexpr int RG = 512;               // region order
-constexpr int TR = 128, TC = 64;      // tile rows x columns
-constexpr int LDT = TR + 2;           // LDS column stride of a tile
+constexpr int TB = 64;                // tile order
+constexpr int NTB = RG / TB;          // 8 tiles per region side
+constexpr int LDT = TB + 2;           // LDS column stride of a tile (conflict-free strided reads)
 constexpr int SY_THREADS = 512;       // 8 waves
-constexpr int NTP = RG / TR, NTQ = RG / TC;  // 4 row tiles, 8 column tiles per region
+constexpr int TV = TB * TB / 2 / SY_THREADS;  // 16-byte tile loads per thread (4)
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
@@ -51,13 +62,21 @@ __device__ __forceinline__ void region_of(int t, int nreg, int &a, int &b) {
   a = bb + rem;
 }
 
+// One prefetch stage: a thread's share of a tile and of its X columns.
+struct Stage {
+  v2d a[TV];
+  v2d x;
+};
+
 template <bool LOWER>
 __global__ void __launch_bounds__(SY_THREADS)
 symm_region_kernel(int n, int nreg, const double *A, long lda, int vec, const double *Xc, long npad,
                    double *part) {
-  __shared__ double Ts[2][TC * LDT];
-  __shared__ double Vx[TC * 16];  // V of one column tile from waves 4..7
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double Xr[RG * 16];        // X rows R0 .. R0 + 511 (V operand)
+  __shared__ double Ts[2][TB * LDT];    // tiles, column-major
+  __shared__ double Xt[2][TB * 16];     // X rows of a tile's columns (U operand)
+  int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int lc = lane & 15, lg = lane >> 4;
   const int y = blockIdx.y;
   int a, b;
@@ -66,63 +85,58 @@ symm_region_kernel(int n, int nreg, const double *A, long lda, int vec, const do
   const long R0 = (long)a * RG, C0 = (long)b * RG;
   const double *xc = Xc + (long)y * npad * 16;
 
-  // global -> registers: thread owns 16 elements of the 128 x 64 tile
-  auto load_tile = [&](v2d (&reg)[8], int p, int q) __attribute__((always_inline)) {
-    const long r0 = R0 + (long)p * TR, c0 = C0 + (long)q * TC;
-    if (vec && r0 + TR <= n && c0 + TC <= n) {  // interior tile: plain 16-byte loads
-      const double *base = LOWER ? A + (c0 + wave) * lda + r0 + 2 * lane
-                                 : A + (r0 + (tid >> 5)) * lda + c0 + 2 * (lane & 31);
+  // X rows of the region into LDS (rows beyond n are zero in the compact copy)
 #pragma unroll
-      for (int u = 0; u < 8; u++)
-        reg[u] = *reinterpret_cast<const v2d *>(base + (LOWER ? 8L * u * lda : 16L * u * lda));
+  for (int u = 0; u < RG * 16 / 2 / SY_THREADS; u++) {
+    const int e = tid + u * SY_THREADS, row = e >> 3, pr = 2 * (e & 7);
+    *reinterpret_cast<v2d *>(Xr + row * 16 + pr) = *reinterpret_cast<const v2d *>(xc + (R0 + row) * 16 + pr);
+  }
+
+  // tile t = 8 p + q: row tile p, column tile q; above the diagonal of a
+  // diagonal region (q > p) it is skipped
+  auto skip = [&](int t) { return diag && (t & (NTB - 1)) > (t >> 3); };
+  auto load = [&](Stage &st, int t) __attribute__((always_inline)) {
+    if (t >= NTB * NTB || skip(t)) return;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // addresses formed here, not hoisted and kept live across the loop
+    const int p = t >> 3, q = t & (NTB - 1);
+    const long r0 = R0 + (long)p * TB, c0 = C0 + (long)q * TB;
+    st.x = *reinterpret_cast<const v2d *>(xc + (c0 + (tid >> 3)) * 16 + 2 * (tid & 7));
+    if (vec && r0 + TB <= n && c0 + TB <= n) {  // interior tile: plain 16-byte loads
+      const double *base = LOWER ? A + (c0 + (tid >> 5)) * lda + r0 + 2 * (tid & 31)
+                                 : A + (r0 + (tid >> 5)) * lda + c0 + 2 * (tid & 31);
+#pragma unroll
+      for (int u = 0; u < TV; u++) st.a[u] = *reinterpret_cast<const v2d *>(base + 16L * u * lda);
       return;
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int v = tid + u * SY_THREADS;  // vector index 0 .. 4095
-      int r, c;
-      if (LOWER) { r = 2 * (v & 63); c = v >> 6; }   // pairs along rows (contiguous)
-      else { c = 2 * (v & 31); r = v >> 5; }         // pairs along columns (contiguous)
-      const long gr = r0 + r, gc = c0 + c;
-      v2d x;
-      if (LOWER) {
-        const long cc = min(gc, (long)n - 1);
-        if (vec && gr + 1 < n && gc < n) {
-          x = *reinterpret_cast<const v2d *>(A + cc * lda + gr);
-        } else {
-          const double e0 = A[cc * lda + min(gr, (long)n - 1)];
-          const double e1 = A[cc * lda + min(gr + 1, (long)n - 1)];
-          x = (v2d){e0, e1};
-        }
-        x[0] = (gr < n && gc < n) ? x[0] : 0.0;
-        x[1] = (gr + 1 < n && gc < n) ? x[1] : 0.0;
-      } else {
-        const long rr = min(gr, (long)n - 1);
-        if (vec && gc + 1 < n && gr < n) {
-          x = *reinterpret_cast<const v2d *>(A + rr * lda + gc);
-        } else {
-          const double e0 = A[rr * lda + min(gc, (long)n - 1)];
-          const double e1 = A[rr * lda + min(gc + 1, (long)n - 1)];
-          x = (v2d){e0, e1};
-        }
-        x[0] = (gr < n && gc < n) ? x[0] : 0.0;
-        x[1] = (gr < n && gc + 1 < n) ? x[1] : 0.0;
-      }
-      reg[u] = x;
+    for (int u = 0; u < TV; u++) {
+      const int v = tid + u * SY_THREADS;      // vector index 0 .. 2047
+      const int i2 = 2 * (v & 31), j = v >> 5;  // pair along the contiguous index, other index
+      // lower: (r, c) = (i2, j) and (i2 + 1, j); upper: (r, c) = (j, i2) and (j, i2 + 1)
+      const long gr0 = LOWER ? r0 + i2 : r0 + j, gc0 = LOWER ? c0 + j : c0 + i2;
+      const long gr1 = LOWER ? gr0 + 1 : gr0, gc1 = LOWER ? gc0 : gc0 + 1;
+      const bool ok0 = gr0 < n && gc0 < n, ok1 = gr1 < n && gc1 < n;
+      auto at = [&](long r, long c) {  // memory element of lower-form (r, c)
+        const long rr = min(r, (long)n - 1), cc = min(c, (long)n - 1);
+        return LOWER ? A[cc * lda + rr] : A[rr * lda + cc];
+      };
+      const double e0 = at(gr0, gc0), e1 = at(gr1, gc1);
+      st.a[u] = (v2d){ok0 ? e0 : 0.0, ok1 ? e1 : 0.0};
     }
   };
-  auto store_tile = [&](const v2d (&reg)[8], int buf) __attribute__((always_inline)) {
-    double *T = Ts[buf];
+  auto store = [&](const Stage &st, int t) __attribute__((always_inline)) {
+    if (t >= NTB * NTB || skip(t)) return;
+    double *T = Ts[t & 1];
+    *reinterpret_cast<v2d *>(Xt[t & 1] + (tid >> 3) * 16 + 2 * (tid & 7)) = st.x;
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int v = tid + u * SY_THREADS;
+    for (int u = 0; u < TV; u++) {
+      const int v = tid + u * SY_THREADS, i2 = 2 * (v & 31), j = v >> 5;
       if (LOWER) {
-        const int r = 2 * (v & 63), c = v >> 6;
-        *reinterpret_cast<v2d *>(T + c * LDT + r) = reg[u];
+        *reinterpret_cast<v2d *>(T + j * LDT + i2) = st.a[u];
       } else {
-        const int c = 2 * (v & 31), r = v >> 5;
-        T[c * LDT + r] = reg[u][0];
-        T[(c + 1) * LDT + r] = reg[u][1];
+        T[i2 * LDT + j] = st.a[u][0];
+        T[(i2 + 1) * LDT + j] = st.a[u][1];
       }
     }
   };
@@ -131,83 +145,69 @@ symm_region_kernel(int n, int nreg, const double *A, long lda, int vec, const do
   double *pu = part + ((long)y * nreg * (nreg + 1) + 2L * blockIdx.x) * RG * 16;
   double *pv = pu + RG * 16;
 
-  v4d U[NTP], V;
+  const bool uwave = wave < 4;
+  const int w4 = wave & 3;
+  v4d V[NTB];
 #pragma unroll
-  for (int p = 0; p < NTP; p++) U[p] = (v4d){0.0, 0.0, 0.0, 0.0};
-  V = (v4d){0.0, 0.0, 0.0, 0.0};
-  const int vr = 64 * (wave >> 2);      // V: this wave's half of the tile rows
-  const int vc = 16 * (wave & 3) + lc;  // V: this lane's tile column
+  for (int q = 0; q < NTB; q++) V[q] = (v4d){0.0, 0.0, 0.0, 0.0};
 
-  // products of the tile in buffer buf (row tile p, column tile q)
-  auto tile_products = [&](int buf, int p, int q, v4d &Up) __attribute__((always_inline)) {
-    const double *T = Ts[buf];
-    const long gr0 = R0 + (long)p * TR, gc0 = C0 + (long)q * TC;
-    {  // U rows 16 wave + lc: sum over the tile's 64 columns
-      const int r = 16 * wave + lc;
-      v4d acc = Up;
-#pragma unroll
-      for (int u = 0; u < TC / 4; u++) {
-        const int c = 4 * u + lg;
-        double av = T[c * LDT + r];
-        if (diag && gr0 + r < gc0 + c) av = 0.0;  // above the diagonal
-        const double bv = xc[(gc0 + c) * 16 + lc];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-      Up = acc;
-    }
-    {  // V column vc: sum over this wave's 64 tile rows of T(r, vc) X(r)
-      v4d acc = V;
-#pragma unroll
-      for (int u = 0; u < 64 / 4; u++) {
-        const int r = vr + 4 * u + lg;
-        double av = T[vc * LDT + r];
-        if (diag && gr0 + r <= gc0 + vc) av = 0.0;  // strict lower part only
-        const double bv = xc[(gr0 + r) * 16 + lc];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-      V = acc;
-    }
-  };
-
-  // The 32 tiles of the region, q outer and p inner: the next tile's loads
-  // are in flight during this tile's products.  On a diagonal region the
-  // masks zero what lies above the diagonal.
-  v2d ra[8];
-  load_tile(ra, 0, 0);
-  store_tile(ra, 0);
+  Stage s0, s1;
+  load(s0, 0);
+  load(s1, 1);
+  store(s0, 0);
   __syncthreads();
 #pragma unroll 1
-  for (int t = 0; t < NTP * NTQ; t++) {
-    const int p = t % NTP, q = t / NTP;
-    const bool more = t + 1 < NTP * NTQ;
-    if (more) load_tile(ra, (t + 1) % NTP, (t + 1) / NTP);
-    tile_products(t & 1, p, q, U[p]);
-    // column tile q complete: waves 4..7 hand their half to waves 0..3
-    if (p == NTP - 1 && wave >= 4) {
+  for (int p = 0; p < NTB; p++) {
+    v4d U = (v4d){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int r = 0; r < 4; r++) Vx[(16 * (wave & 3) + lg + 4 * r) * 16 + lc] = V[r];
-    }
-    // the other buffer was last read in the previous iteration, which every
-    // wave finished before that iteration's barrier
-    if (more) store_tile(ra, (t + 1) & 1);
-    __syncthreads();
-    if (p == NTP - 1) {
-      if (wave < 4) {
+    for (int q = 0; q < NTB; q++) {
+      const int t = NTB * p + q;
+      Stage &cur = (q & 1) ? s1 : s0, &nxt = (q & 1) ? s0 : s1;
+      // tile t + 2 into the stage tile t came through (stored last iteration)
+      load(cur, t + 2);
+      asm volatile("" ::: "memory");  // keep the prefetch ahead of the products
+      if (!(diag && q > p)) {
+        const double *T = Ts[t & 1];
+        const bool md = diag && q == p;  // diagonal tile: lower part only
+        if (uwave) {  // U rows 16 w4 + lc: sum over the tile's 64 columns
+          const int r = 16 * w4 + lc;
+          const double *xt = Xt[t & 1];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int row = 16 * wave + lg + 4 * r;
-          pv[(q * TC + row) * 16 + lc] = V[r] + Vx[row * 16 + lc];
+          for (int u = 0; u < TB / 4; u++) {
+            const int c = 4 * u + lg;
+            double av = T[c * LDT + r];
+            if (md && r < c) av = 0.0;
+            U = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xt[c * 16 + lc], U, 0, 0, 0);
+          }
+        } else {      // V column 16 w4 + lc: sum over the tile's 64 rows (strict part on the diagonal)
+          const int c = 16 * w4 + lc;
+          const double *xr = Xr + p * TB * 16;
+#pragma unroll
+          for (int u = 0; u < TB / 4; u++) {
+            const int r = 4 * u + lg;
+            double av = T[c * LDT + r];
+            if (md && r <= c) av = 0.0;
+            V[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xr[r * 16 + lc], V[q], 0, 0, 0);
+          }
         }
       }
-      V = (v4d){0.0, 0.0, 0.0, 0.0};
+      // tile t + 1 (loaded one iteration ago) into the buffer tile t - 1 used,
+      // which every wave finished reading before the last barrier
+      store(nxt, t + 1);
+      __syncthreads();
+    }
+    // U of row tile p: wave w4 rows 64 p + 16 w4 + (lane / 16 + 4 r), column lane % 16
+    if (uwave) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) pu[(p * TB + 16 * w4 + lg + 4 * r) * 16 + lc] = U[r];
     }
   }
-
-  // U: wave w, row-tile p: rows 128 p + 16 w + (lane / 16 + 4 r), column lane % 16
+  if (!uwave) {
 #pragma unroll
-  for (int p = 0; p < NTP; p++)
+    for (int q = 0; q < NTB; q++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) pu[(p * TR + 16 * wave + lg + 4 * r) * 16 + lc] = U[p][r];
+      for (int r = 0; r < 4; r++) pv[(q * TB + 16 * w4 + lg + 4 * r) * 16 + lc] = V[q][r];
+  }
 }
 
 // R(rows of block a) = beta C + alpha (sum_b U(a, b) + sum_{a' >= a} V(a', a));

@@ -7,10 +7,14 @@
 //     potrf_nb <precision> <n_max> <nb>
 //     batched_per_cu <precision> <inverse> <workgroups per CU>
 //     gemm_tile <family> <min_tiles>
+//     panel_launch <precision> <0|1>
 // precision: D (FP64), Z (complex FP64), S (FP32 products / storage), B
 // (BF16 products).  For an order n the first potrf_nb line of that precision
 // with n <= n_max applies.  gemm_tile: a launch of kernel family F takes its
-// larger tile when it has at least min_tiles of them (families below).  The
+// larger tile when it has at least min_tiles of them (families below).
+// panel_launch: 1 factors each panel of the lower real factorisation in one
+// launch (panel_potrf), 0 as a chain of diagonal-block and panel-product
+// launches.  The
 // table is read once, from $CUCHOLESKY_TUNING or else tuning.txt beside the
 // library; without it the built-in defaults hold.
 #include "internal.h"
@@ -44,6 +48,10 @@ int g_batched_per_cu[2][2] = {{0, 0}, {0, 0}};
 // gemm_tile entries by family letter; 0 = none
 long g_tile_min[26] = {};
 
+// panel_launch entries by precision letter; -1 = none
+int g_panel_launch[26] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                          -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+
 void load() {
   std::string path;
   if (const char *e = std::getenv("CUCHOLESKY_TUNING")) {
@@ -67,6 +75,10 @@ void load() {
     long tmin = 0;
     if (std::sscanf(line, "gemm_tile %7s %ld", prec, &tmin) == 2) {
       if (prec[0] >= 'A' && prec[0] <= 'Z' && tmin > 0) g_tile_min[prec[0] - 'A'] = tmin;
+      continue;
+    }
+    if (std::sscanf(line, "panel_launch %7s %ld", prec, &tmin) == 2) {
+      if (prec[0] >= 'A' && prec[0] <= 'Z' && (tmin == 0 || tmin == 1)) g_panel_launch[prec[0] - 'A'] = (int)tmin;
       continue;
     }
     if (std::sscanf(line, "%63s %7s %ld %ld", key, prec, &n_max, &nb) != 4) continue;
@@ -109,6 +121,19 @@ long tuned_tile_min(char family, long dflt) {
   std::call_once(g_once, load);
   const long t = g_tile_min[family - 'A'];
   return t > 0 ? t : dflt;
+}
+
+bool tuned_panel_launch(char prec, bool dflt) {
+  if (prec < 'A' || prec > 'Z') return dflt;
+  // $CUCHOLESKY_PANEL_LAUNCH_<precision> (read on every call, for the
+  // autotuner), else the table, else the default
+  char name[40] = "CUCHOLESKY_PANEL_LAUNCH_";
+  name[24] = prec;
+  name[25] = '\0';
+  if (const char *e = std::getenv(name)) return std::atoi(e) != 0;
+  std::call_once(g_once, load);
+  const int v = g_panel_launch[prec - 'A'];
+  return v < 0 ? dflt : v != 0;
 }
 
 }  // namespace hc
