@@ -273,6 +273,24 @@ long to_float_norm_ws_doubles(long n);
 hipError_t diag_absmax(hipStream_t s, long n, const double *A, long lda, double *out);
 hipError_t to_float_norm(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
                          int *overflow, double *ws, double *anorm);
+// The same in two parts: the copy and the partial sums of the lower-form
+// columns [col0, col1) (multiples of 64, or col1 = n) on at most max_wgs
+// workgroups (0: one per 64 x 64 tile), then, once every column is done, the
+// norm from the partial sums.
+hipError_t to_float_norm_cols(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
+                              int *overflow, double *ws, long col0, long col1, long max_wgs);
+hipError_t to_float_norm_finish(hipStream_t s, long n, double *ws, double *anorm);
+// The FP32 copy (with the norm) made by the blocked factorisation itself: its
+// first block column before the first panel, the rest on the look-ahead
+// stream beside that panel.
+struct PreConvert {
+  char uplo;            // triangle of A
+  const double *A;      // FP64 source
+  long lda;
+  int *overflow;        // set to 1 when a value exceeds the FP32 range
+  double *ws;           // to_float_norm_ws_doubles(n) doubles
+  double *anorm;        // ||A||_inf (device)
+};
 
 // ---------------------------------------------------------------------------
 // symmetric matrix times a few right-hand sides (symm.hip)
@@ -374,9 +392,11 @@ hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out);
 // bf16: FP32 data only, trailing updates with BF16 products (split: three
 // BF16 products of a two-term split of every panel value; f16_scale > 0:
 // FP16 products of the panel values times f16_scale, a power of two).
+// pre (FP32 data only): A is not yet the FP32 copy; the factorisation makes it.
 template <typename T>
 hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
-                           void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f);
+                           void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f,
+                           const PreConvert *pre = nullptr);
 // Workspace of the bf16 factorisation (two bfloat16 panel copies; six with split).
 size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n, bool split = false);
 

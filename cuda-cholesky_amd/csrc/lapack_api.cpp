@@ -157,6 +157,9 @@ hipError_t timing_end(CULAPACKhandle h, hipStream_t s) {
 // products from those copies on the BF16 matrix cores.  split: each copy is
 // the pair of rows [hi | hi | lo] and [hi | lo | hi] (x = hi + lo), and the
 // products run over 3 ldt: three BF16 products, ~16 significant bits.
+// Workgroups of the FP32 copy made beside the first panel (four per CU).
+constexpr long kConvertWorkgroups = 1024;
+
 // Flags and workspace of the one-launch panel factorisation, allocated once
 // per handle (the flags zeroed then; every launch uses a new epoch).
 hipError_t panel_resources(CULAPACKhandle h) {
@@ -178,7 +181,8 @@ hipError_t panel_resources(CULAPACKhandle h) {
 
 template <typename T>
 hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16 = false,
-                        void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f) {
+                        void *lp_ws = nullptr, bool split = false, float f16_scale = 0.0f,
+                        const hc::PreConvert *pre = nullptr) {
   const long NB = potrf_nb(h, prec_of<T>(bf16), n);
   hipStream_t s0 = h->stream, s1 = h->side;
   const bool lower = (uplo == 'L');
@@ -225,8 +229,27 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
     }
     return factor_panel<T>(h, s, lower, n, A, lda, W, jj, jbb);
   };
+  if (pre != nullptr) {
+    // FP32 copy of A made here: the first block column before the first
+    // panel, the rest (and the norm) on the look-ahead stream beside it; the
+    // main stream waits for it before the first update beyond that panel.
+    if constexpr (!std::is_same<T, float>::value) return hipErrorInvalidValue;
+    else {
+      const long csplit = std::min(n, (std::min(NB, n) + 63) / 64 * 64);
+      HC_TRY(hc::to_float_norm_cols(s0, pre->uplo, n, pre->A, pre->lda, A, lda, pre->overflow, pre->ws, 0,
+                                    csplit, 0));
+      HC_TRY(hipEventRecord(h->ev_update, s0));
+      HC_TRY(hipStreamWaitEvent(s1, h->ev_update, 0));
+      // a bounded grid, so that the panel's launches find room beside it
+      HC_TRY(hc::to_float_norm_cols(s1, pre->uplo, n, pre->A, pre->lda, A, lda, pre->overflow, pre->ws, csplit,
+                                    n, kConvertWorkgroups));
+      HC_TRY(hc::to_float_norm_finish(s1, n, pre->ws, pre->anorm));
+      HC_TRY(hipEventRecord(h->ev_panel, s1));
+    }
+  }
   HC_TRY(panel(s0, 0, std::min(NB, n)));
   if (bf16) HC_TRY(convert(s0, n, std::min(NB, n), A, 0, 0));
+  if (pre != nullptr) HC_TRY(hipStreamWaitEvent(s0, h->ev_panel, 0));
   for (long j = 0; j < n; j += NB) {
     const int pj = (int)((j / NB) & 1);
     const long jb = std::min(NB, n - j);
@@ -536,13 +559,13 @@ hipError_t lapack_workspace(CULAPACKhandle h, size_t bytes, void **out) {
 
 template <typename T>
 hipError_t potrf_on_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *W, bool bf16,
-                           void *lp_ws, bool split, float f16_scale) {
-  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws, split, f16_scale);
+                           void *lp_ws, bool split, float f16_scale, const PreConvert *pre) {
+  return potrf_device<T>(h, uplo, n, A, lda, W, bf16, lp_ws, split, f16_scale, pre);
 }
 template hipError_t potrf_on_device<double>(CULAPACKhandle, char, long, double *, long, double *, bool,
-                                            void *, bool, float);
+                                            void *, bool, float, const PreConvert *);
 template hipError_t potrf_on_device<float>(CULAPACKhandle, char, long, float *, long, float *, bool,
-                                           void *, bool, float);
+                                           void *, bool, float, const PreConvert *);
 
 size_t potrf_bf16_ws_bytes(CULAPACKhandle h, long n, bool split) {
   const long ldt = (potrf_nb(h, 'B', n) + 63) / 64 * 64;

@@ -5,6 +5,7 @@
 // reductions are deterministic (fixed trees, no atomics).
 #include "internal.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace hc {
@@ -108,20 +109,27 @@ __global__ void __launch_bounds__(256) colmax_kernel(int m, const double *X, lon
 }
 
 // One 64 x 64 lower-form tile per workgroup (a 1-D grid over the lower
-// tiles): the FP32 copy as in to_float_lower_kernel, and the tile's share of
-// the infinity norm of the symmetric matrix -- row sums of |a| over the tile
-// (diagonal included) to rowpart[bj][row], column sums of |a| strictly below
-// the diagonal to colpart[bi][column] -- so the matrix is read once for both.
+// tiles, column-major: tile columns bj = 0, 1, ... each from the diagonal
+// down, so that a range of tile columns is a range of tiles): the FP32 copy
+// as in to_float_lower_kernel, and the tile's share of the infinity norm of
+// the symmetric matrix -- row sums of |a| over the tile (diagonal included)
+// to rowpart[bj][row], column sums of |a| strictly below the diagonal to
+// colpart[bi][column] -- so the matrix is read once for both.
 constexpr int NT = 64;
 __global__ void __launch_bounds__(256)
 to_float_norm_kernel(int lower, int n, const double *A, long lda, float *S, long lds, int *overflow,
-                     double *rowpart, double *colpart) {
+                     double *rowpart, double *colpart, long t0, long count) {
   __shared__ double t[NT][NT + 1];  // t[c][r]: lower-form element (r0 + r, c0 + c), 0 outside
-  const long tix = blockIdx.x;
-  int bi = (int)((sqrt(8.0 * (double)tix + 1.0) - 1.0) * 0.5);
-  while ((long)bi * (bi + 1) / 2 > tix) bi--;
-  while ((long)(bi + 1) * (bi + 2) / 2 <= tix) bi++;
-  const int bj = (int)(tix - (long)bi * (bi + 1) / 2);
+  bool ovf = false;
+  // tiles t0 .. t0 + count - 1, strided over the grid
+  for (long tt = blockIdx.x; tt < count; tt += gridDim.x) {
+  // counted from the last tile, the triangle is row-major: tile column
+  // nt - 1 - cr holds cr + 1 tiles
+  const long nt = (n + NT - 1) / NT, total = nt * (nt + 1) / 2, rev = total - 1 - (t0 + tt);
+  int cr = (int)((sqrt(8.0 * (double)rev + 1.0) - 1.0) * 0.5);
+  while ((long)cr * (cr + 1) / 2 > rev) cr--;
+  while ((long)(cr + 1) * (cr + 2) / 2 <= rev) cr++;
+  const int bj = (int)(nt - 1 - cr), bi = (int)(nt - 1 - (rev - (long)cr * (cr + 1) / 2));
   const int r0 = bi * NT, c0 = bj * NT;
   const int li = threadIdx.x & (NT - 1), g = threadIdx.x >> 6;
   // lower: lanes over rows of A (lower-form rows); upper: lanes over rows of A,
@@ -142,7 +150,6 @@ to_float_norm_kernel(int lower, int n, const double *A, long lda, float *S, long
     t[c][r] = (R < n && C < n && R >= C) ? v[u] : 0.0;
   }
   __syncthreads();
-  bool ovf = false;
 #pragma unroll
   for (int u = 0; u < 16; u++) {  // S column-major, lanes over rows
     const int c = g + 4 * u, R = r0 + li, C = c0 + c;
@@ -164,6 +171,8 @@ to_float_norm_kernel(int lower, int n, const double *A, long lda, float *S, long
       if (r0 + r > C) sum += fabs(t[c][r]);
     if (C < n) colpart[(long)bi * n + C] = sum;
   }
+  __syncthreads();  // t is refilled by the next tile
+  }
   if (ovf) *overflow = 1;
 }
 
@@ -174,9 +183,14 @@ tile_norm_rows_kernel(int n, int nt, const double *rowpart, const double *colpar
   const int i = blockIdx.x * 256 + threadIdx.x;
   double v = 0.0;
   if (i < n) {
+    // four independent sums (more loads in flight), added in a fixed order
     const int b = i / NT;
-    for (int j = 0; j <= b; j++) v += rowpart[(long)j * n + i];
-    for (int j = b; j < nt; j++) v += colpart[(long)j * n + i];
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int j = 0; j <= b; j++) a4[j & 3] += rowpart[(long)j * n + i];
+#pragma unroll 4
+    for (int j = b; j < nt; j++) a4[j & 3] += colpart[(long)j * n + i];
+    v = (a4[0] + a4[1]) + (a4[2] + a4[3]);
   }
   const double r = block_max<256>(v);
   if (threadIdx.x == 0) bmax[blockIdx.x] = r;
@@ -213,20 +227,37 @@ long to_float_norm_ws_doubles(long n) {
   return 2 * nt * n + (n + 255) / 256 + 1;
 }
 
-hipError_t to_float_norm(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
-                         int *overflow, double *ws, double *anorm) {
+hipError_t to_float_norm_cols(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
+                              int *overflow, double *ws, long col0, long col1, long max_wgs) {
+  if (n <= 0 || col1 <= col0) return hipSuccess;
+  if (col0 % NT != 0 || col0 < 0 || col1 > n || (col1 % NT != 0 && col1 != n)) return hipErrorInvalidValue;
+  const long nt = (n + NT - 1) / NT, b0 = col0 / NT, b1 = (col1 + NT - 1) / NT;
+  // tiles before tile column b: sum_{k<b} (nt - k)
+  auto before = [&](long b) { return b * nt - b * (b - 1) / 2; };
+  const long t0 = before(b0), count = before(b1) - t0;
+  double *rowpart = ws, *colpart = ws + nt * n;
+  const long grid = max_wgs > 0 ? std::min(count, max_wgs) : count;
+  hipLaunchKernelGGL(to_float_norm_kernel, dim3((unsigned)grid), dim3(256), 0, s, uplo == 'L' ? 1 : 0,
+                     (int)n, A, lda, S, lds, overflow, rowpart, colpart, t0, count);
+  return hipGetLastError();
+}
+
+hipError_t to_float_norm_finish(hipStream_t s, long n, double *ws, double *anorm) {
   if (n <= 0) return hipSuccess;
   const long nt = (n + NT - 1) / NT;
   double *rowpart = ws, *colpart = ws + nt * n, *bmax = ws + 2 * nt * n;
   const unsigned nb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(to_float_norm_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s,
-                     uplo == 'L' ? 1 : 0, (int)n, A, lda, S, lds, overflow, rowpart, colpart);
-  HC_TRY(hipGetLastError());
   hipLaunchKernelGGL(tile_norm_rows_kernel, dim3(nb), dim3(256), 0, s, (int)n, (int)nt, rowpart, colpart,
                      bmax);
   HC_TRY(hipGetLastError());
   hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, s, (int)nb, bmax, anorm);
   return hipGetLastError();
+}
+
+hipError_t to_float_norm(hipStream_t s, char uplo, long n, const double *A, long lda, float *S, long lds,
+                         int *overflow, double *ws, double *anorm) {
+  HC_TRY(to_float_norm_cols(s, uplo, n, A, lda, S, lds, overflow, ws, 0, n, 0));
+  return to_float_norm_finish(s, n, ws, anorm);
 }
 
 hipError_t to_float_lower(hipStream_t s, char uplo, long n, const double *A, long lda, float *S,

@@ -205,15 +205,17 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   // ||A||_inf; refinement stops when every column has
   // ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n)   (LAPACK dsposv, eps = 2^-53).
   // One read of A gives both the norm and the FP32 copy the first attempt
-  // factors (a later attempt converts again: the factor overwrites it).
+  // factors, made by that factorisation itself (its first block column ahead
+  // of the first panel, the rest beside it); a later attempt converts again:
+  // the factor overwrites the copy.
   HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
-  HC_CHECK(hc::to_float_norm(s, ul, N, Ad, (long)lda, S, N, ovf, NW, dnorm));
-  bool converted = true;
+  bool first = true;
   double anorm = 0.0, amax_diag = 0.0;
-  if (factor == 'H') HC_CHECK(hc::diag_absmax(s, N, Ad, (long)lda, dmax));
-  HC_CHECK(hipMemcpyAsync(&anorm, dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
-  if (factor == 'H') HC_CHECK(hipMemcpyAsync(&amax_diag, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
-  HC_CHECK(hipStreamSynchronize(s));
+  if (factor == 'H') {
+    HC_CHECK(hc::diag_absmax(s, N, Ad, (long)lda, dmax));
+    HC_CHECK(hipMemcpyAsync(&amax_diag, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
+    HC_CHECK(hipStreamSynchronize(s));
+  }
   // FP16 panel copies are scaled by 2^e with every entry of the factor below
   // 2^15: |l_ij| <= sqrt(a_ii) <= sqrt(max a_ii) for an SPD matrix
   float f16_scale = 1.0f;
@@ -222,15 +224,17 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     std::frexp(std::sqrt(amax_diag), &e2);   // sqrt(max a_ii) < 2^e2
     f16_scale = std::ldexp(1.0f, std::max(-60, std::min(60, 15 - e2)));
   }
-  const double cte = anorm * std::ldexp(1.0, -53) * std::sqrt((double)N);
+  double cte = 0.0;  // anorm eps sqrt(n), once the first factorisation has formed anorm
 
-  auto read_norms = [&](Norms &nm) -> CUresult {
+  // flag != nullptr: the overflow word comes back with the norms (one wait)
+  auto read_norms = [&](Norms &nm, int *flag) -> CUresult {
     HC_CHECK(hc::colmax(s, N, K, R, N, rn));
     HC_CHECK(hc::colmax(s, N, K, Xd, (long)ldx, xn));
     nm.r.resize(K);
     nm.x.resize(K);
     HC_CHECK(hipMemcpyAsync(nm.r.data(), rn, K * sizeof(double), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipMemcpyAsync(nm.x.data(), xn, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (flag != nullptr) HC_CHECK(hipMemcpyAsync(flag, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipStreamSynchronize(s));
     return CUDA_SUCCESS;
   };
@@ -250,28 +254,35 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
   // One attempt with a low-precision factor; returns the LAPACK ITER code.
   auto attempt = [&](char kind, long *code) -> CUresult {
     const bool bf16 = kind != 'S', split = kind == 'X';
-    if (!converted) {
+    const bool pre = first;  // this factorisation makes the FP32 copy and the norm
+    first = false;
+    int flag = 0;
+    if (!pre) {
       HC_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
       HC_CHECK(hc::to_float_lower(s, ul, N, Ad, (long)lda, S, N, ovf));
+      CU_TRY(read_flag(&flag));
+      if (flag) { *code = -2; return CUDA_SUCCESS; }     // narrowing overflowed
     }
-    converted = false;
-    int flag = 0;
-    CU_TRY(read_flag(&flag));
-    if (flag) { *code = -2; return CUDA_SUCCESS; }       // narrowing overflowed
+    const hc::PreConvert pc{ul, Ad, (long)lda, ovf, NW, dnorm};
     HC_CHECK(hipMemsetAsync(h->dinfo, 0, sizeof(int), s));
     HC_CHECK(hc::potrf_on_device<float>(h, 'L', N, S, N, W, bf16, wsb + L.P16, split,
-                                        kind == 'H' ? f16_scale : 0.0f));
+                                        kind == 'H' ? f16_scale : 0.0f, pre ? &pc : nullptr));
+    // x0 = inv(S S^T) b in FP32; the diagonal-block inverses serve every
+    // solve.  They and the FP32 right-hand sides are queued behind the
+    // factorisation before its info word is read (wasted if it failed), so
+    // that one wait covers info, the norm and every overflow.
+    HC_CHECK(potrs_prepare<float>(s, 'L', N, S, N, static_cast<float *>(T)));
+    HC_CHECK(hc::d2s(s, N, K, Bd, (long)ldb, RS, N, ovf));
     long finfo = 0;
     HC_CHECK(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (pre) HC_CHECK(hipMemcpyAsync(&anorm, dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+    HC_CHECK(hipMemcpyAsync(&flag, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
     HC_CHECK(hipStreamSynchronize(s));
+    if (pre) cte = anorm * std::ldexp(1.0, -53) * std::sqrt((double)N);
+    if (flag) { *code = -2; return CUDA_SUCCESS; }       // narrowing overflowed (any factor is void)
     finfo = h->hinfo[0];
     if (finfo == hc::kInfoAbandoned) return CUDA_ERROR_LAUNCH_FAILED;  // a dependency wait gave up
     if (finfo != 0) { *code = -3; return CUDA_SUCCESS; }  // low-precision factorisation failed
-    // x0 = inv(S S^T) b in FP32; the diagonal-block inverses serve every solve
-    HC_CHECK(potrs_prepare<float>(s, 'L', N, S, N, static_cast<float *>(T)));
-    HC_CHECK(hc::d2s(s, N, K, Bd, (long)ldb, RS, N, ovf));
-    CU_TRY(read_flag(&flag));
-    if (flag) { *code = -2; return CUDA_SUCCESS; }
     HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
                                  static_cast<float *>(Yw), sync));
     HC_CHECK(hc::s2d(s, N, K, RS, N, Xd, (long)ldx));
@@ -346,15 +357,16 @@ CUresult sposv(const char *name, CULAPACKhandle h, CBlasUplo uplo, size_t n, siz
     const long itmax = refine == 'G' ? kGmresOuterMax : kIterMax;
     for (long it = 0;; it++) {
       HC_CHECK(residual(s, ul, N, K, Ad, (long)lda, Bd, (long)ldb, Xd, (long)ldx, R, N, PART));
-      CU_TRY(read_norms(nm));
+      // the FP32 residual of the next correction, formed ahead of the norms'
+      // wait (unused once converged)
+      if (refine != 'G') HC_CHECK(hc::d2s(s, N, K, R, N, RS, N, ovf));
+      CU_TRY(read_norms(nm, refine != 'G' ? &flag : nullptr));
       if (converged(nm)) { *code = it; return CUDA_SUCCESS; }
       if (it == itmax) { *code = -(itmax + 1); return CUDA_SUCCESS; }
       if (refine == 'G') {
         CU_TRY(gmres_cycle());
         continue;
       }
-      HC_CHECK(hc::d2s(s, N, K, R, N, RS, N, ovf));
-      CU_TRY(read_flag(&flag));
       if (flag) { *code = -2; return CUDA_SUCCESS; }
       HC_CHECK(potrs_device<float>(s, 'L', N, K, S, N, static_cast<const float *>(T), RS, N,
                                    static_cast<float *>(Yw), sync));
