@@ -18,7 +18,8 @@
 //
 // Pipeline: the region's X rows (the V operand, 64 KiB) sit in LDS for the
 // whole region; each tile and its X columns (the U operand) go global ->
-// registers -> LDS, two tiles ahead in two register stages, so every product
+// registers -> LDS, two tiles ahead in two register stages (measured on
+// MI355X: four stages are no faster), so every product
 // reads LDS only and the only waits on global loads come after a tile's
 // products (the load counter completes in order: an operand load issued
 // after a prefetch would otherwise wait for the prefetch).  Row tiles are the
@@ -34,7 +35,8 @@ namespace {
 constexpr int RG = 512;               // region order
 constexpr int TB = 64;                // tile order
 constexpr int NTB = RG / TB;          // 8 tiles per region side
-constexpr int LDT = TB + 2;           // LDS column stride of a tile (conflict-free strided reads)
+constexpr int LDT = TB + 2;           // LDS column stride of a tile (see the products below)
+constexpr int LDX = 16 + 2;           // LDS row stride of a tile's X rows
 constexpr int SY_THREADS = 512;       // 8 waves
 constexpr int TV = TB * TB / 2 / SY_THREADS;  // 16-byte tile loads per thread (4)
 
@@ -74,7 +76,7 @@ symm_region_kernel(int n, int nreg, const double *A, long lda, int vec, const do
                    double *part) {
   __shared__ double Xr[RG * 16];        // X rows R0 .. R0 + 511 (V operand)
   __shared__ double Ts[2][TB * LDT];    // tiles, column-major
-  __shared__ double Xt[2][TB * 16];     // X rows of a tile's columns (U operand)
+  __shared__ double Xt[2][TB * LDX];    // X rows of a tile's columns (U operand)
   int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int lc = lane & 15, lg = lane >> 4;
@@ -128,7 +130,7 @@ symm_region_kernel(int n, int nreg, const double *A, long lda, int vec, const do
   auto store = [&](const Stage &st, int t) __attribute__((always_inline)) {
     if (t >= NTB * NTB || skip(t)) return;
     double *T = Ts[t & 1];
-    *reinterpret_cast<v2d *>(Xt[t & 1] + (tid >> 3) * 16 + 2 * (tid & 7)) = st.x;
+    *reinterpret_cast<v2d *>(Xt[t & 1] + (tid >> 3) * LDX + 2 * (tid & 7)) = st.x;
 #pragma unroll
     for (int u = 0; u < TV; u++) {
       const int v = tid + u * SY_THREADS, i2 = 2 * (v & 31), j = v >> 5;
@@ -151,43 +153,57 @@ symm_region_kernel(int n, int nreg, const double *A, long lda, int vec, const do
 #pragma unroll
   for (int q = 0; q < NTB; q++) V[q] = (v4d){0.0, 0.0, 0.0, 0.0};
 
-  Stage s0, s1;
-  load(s0, 0);
-  load(s1, 1);
-  store(s0, 0);
+  // NST register stages: tiles t + 1 .. t + NST - 1 in flight while tile t
+  // is multiplied (t mod NST = q mod NST: the stage is known at compile time)
+  constexpr int NST = 2;
+  Stage st[NST];
+#pragma unroll
+  for (int k = 0; k < NST; k++) load(st[k], k);
+  store(st[0], 0);
   __syncthreads();
 #pragma unroll 1
   for (int p = 0; p < NTB; p++) {
     v4d U = (v4d){0.0, 0.0, 0.0, 0.0};
+    // V operand of every tile of row tile p: X rows 64 p + 4 u + lane / 16
+    double xv[TB / 4];
+    if (!uwave) {
+#pragma unroll
+      for (int u = 0; u < TB / 4; u++) xv[u] = Xr[(p * TB + 4 * u + lg) * 16 + lc];
+    }
 #pragma unroll
     for (int q = 0; q < NTB; q++) {
       const int t = NTB * p + q;
-      Stage &cur = (q & 1) ? s1 : s0, &nxt = (q & 1) ? s0 : s1;
-      // tile t + 2 into the stage tile t came through (stored last iteration)
-      load(cur, t + 2);
+      Stage &cur = st[q % NST], &nxt = st[(q + 1) % NST];
+      // tile t + NST into the stage tile t came through (stored last iteration)
+      load(cur, t + NST);
       asm volatile("" ::: "memory");  // keep the prefetch ahead of the products
       if (!(diag && q > p)) {
         const double *T = Ts[t & 1];
         const bool md = diag && q == p;  // diagonal tile: lower part only
+        // LDS banks: a half-wave (lane groups g = lane / 16 of 0, 1 or 2, 3)
+        // reads 32 doubles at once.  V lanes differ in the column (stride
+        // LDT = 66 doubles: 16 distinct even bank pairs) and g in the row.
+        // U lanes differ in the row, so the two lane groups of a half-wave
+        // take columns 8 apart (8 LDT doubles: the other 32 banks) and their
+        // X rows are LDX = 18 doubles apart for the same reason.
         if (uwave) {  // U rows 16 w4 + lc: sum over the tile's 64 columns
           const int r = 16 * w4 + lc;
           const double *xt = Xt[t & 1];
 #pragma unroll
           for (int u = 0; u < TB / 4; u++) {
-            const int c = 4 * u + lg;
+            const int c = (u & 7) + 8 * lg + 32 * (u >> 3);  // a permutation of the 64 columns
             double av = T[c * LDT + r];
             if (md && r < c) av = 0.0;
-            U = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xt[c * 16 + lc], U, 0, 0, 0);
+            U = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xt[c * LDX + lc], U, 0, 0, 0);
           }
         } else {      // V column 16 w4 + lc: sum over the tile's 64 rows (strict part on the diagonal)
           const int c = 16 * w4 + lc;
-          const double *xr = Xr + p * TB * 16;
 #pragma unroll
           for (int u = 0; u < TB / 4; u++) {
             const int r = 4 * u + lg;
             double av = T[c * LDT + r];
             if (md && r <= c) av = 0.0;
-            V[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xr[r * 16 + lc], V[q], 0, 0, 0);
+            V[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xv[u], V[q], 0, 0, 0);
           }
         }
       }
