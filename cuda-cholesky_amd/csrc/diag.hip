@@ -318,13 +318,7 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
   for (int c = 0; c < LEAF; c++) a[c] = S[c * ls + r];
   T x[4];
   int bad;
-  if constexpr (is_complex<T>::value) {
-    // the complex kernel is already at its register limit: the fused pass
-    // keeps more values live, so factor and invert one after the other
-    double unused = 0.0;
-    bad = t16::leaf_factor_rows<T>(a, 0, &unused);
-    if (bad < 0) t16::leaf_inverse_cols<T>(a, x);
-  } else {
+  {
     double rpiv;
     bad = t16::leaf_factor_invert<T>(a, x, rpiv);
   }
