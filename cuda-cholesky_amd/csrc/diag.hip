@@ -318,7 +318,13 @@ __device__ __forceinline__ int leaf_factor_inverse(T *S, int ls, bool lower, int
   for (int c = 0; c < LEAF; c++) a[c] = S[c * ls + r];
   T x[4];
   int bad;
-  {
+  if constexpr (is_complex<T>::value) {
+    // complex: factor and invert one after the other (the fused pass and the
+    // inverse-row accumulator of the caller do not fit the registers together)
+    double unused = 0.0;
+    bad = t16::leaf_factor_rows<T>(a, 0, &unused);
+    if (bad < 0) t16::leaf_inverse_cols<T>(a, x);
+  } else {
     double rpiv;
     bad = t16::leaf_factor_invert<T>(a, x, rpiv);
   }
@@ -366,7 +372,7 @@ template <typename T, typename ST, int NMAX>
 __device__ __forceinline__ int diag_block_factor(T *S, int &flag, int lower, int n, ST *A, long lda, ST *W,
                                                  long ldw) {
   constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL, NW = BLK_THREADS / 64;
-  static_assert(NPMAX <= NW + 1, "the inverse gives each wave at most one tile");
+  static_assert(NPMAX <= NW, "each wave finishes at most one tile of a row of the inverse");
   const int tid = threadIdx.x, wave = tid >> 6;
   const int np = (n + LEAF - 1) / LEAF;
   auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
@@ -374,15 +380,50 @@ __device__ __forceinline__ int diag_block_factor(T *S, int &flag, int lower, int
   if (tid == 0) flag = -1;
   __syncthreads();
   // One leaf of look-ahead: while wave 0 updates tile (p+1, p+1) and factors
-  // leaf p + 1, waves 1 .. NW-1 apply the rest of step p's trailing update.
+  // leaf p + 1, waves 1 .. NW-1 apply the rest of step p's trailing update
+  // and form S_{p+1,J} = sum_{K=J..p} L_{p+1,K} X_KJ (wave 1 + J) for row
+  // p + 1 of X = inv(L).  At the start of step p + 1 (phase A) the same wave
+  // writes L_{p+1,J} to A and finishes X_{p+1,J} = -X_{p+1,p+1} S_{p+1,J} in
+  // its tile (L X = I row by row), beside the panel solves below leaf p + 1:
+  // the inverse is complete when the last leaf is, with no pass of its own.
   // Step p = -1 only factors leaf 0 (one call site keeps the leaf inlined once).
+  TileAcc<T> sacc;  // S_{p+1, wave-1}, carried from phase B to the next phase A
+  acc_zero(sacc);
   for (int p = -1; p < np; p++) {
     if (p >= 0) {
-      for (int I = p + 1 + wave; I < np; I += NW) {  // L_Ip = A_Ip inv(L_pp)^H
+      // phase A: panel item k (L_Ip = A_Ip X_pp^H, I = p + 1 + k) on wave 0
+      // (k = 0) or wave p + k; row p of the inverse on waves 1 .. p
+      const int k = wave == 0 ? 0 : (wave > p ? wave - p : -1);
+      if (k >= 0 && p + 1 + k < np) {
+        const int I = p + 1 + k;
         TileAcc<T> acc;
         acc_zero(acc);
         tile_mma<true>(acc, tile(I, p), LL, tile(p, p), LL, 1.0);
         acc_store(acc, tile(I, p), LL);
+      }
+      if (wave >= 1 && wave <= p) {
+        const int J = wave - 1;
+        T *tpj = tile(p, J);
+        {  // L_pJ to its place in A (lower form; padding rows are not written)
+          const int l = tid & 63, rr = l & 15, cq = l >> 4, r = p * LEAF + rr;
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int c = J * LEAF + cq + 4 * j;
+            const T v = tpj[(cq + 4 * j) * LL + rr];
+            if (r < n) {
+              if (lower) A[(long)c * lda + r] = convert<ST>(v);
+              else A[(long)r * lda + c] = convert<ST>(conjx(v));
+            }
+          }
+        }
+        wave_lds_fence();
+        acc_store(sacc, tpj, LL);
+        wave_lds_fence();
+        TileAcc<T> acc;
+        acc_zero(acc);
+        tile_mma<false>(acc, tile(p, p), LL, tpj, LL, -1.0);
+        wave_lds_fence();
+        acc_store(acc, tpj, LL);
       }
       __syncthreads();
     }
@@ -407,40 +448,16 @@ __device__ __forceinline__ int diag_block_factor(T *S, int &flag, int lower, int
               tile_mma<true>(acc, tile(I, p), LL, tile(J, p), LL, -1.0);
               acc_store(acc, tile(I, J), LL);
             }
+        if (wave - 1 <= p) {  // S_{p+1,J}, J = wave - 1: row p + 1 of L is complete
+          const int J = wave - 1;
+          acc_zero(sacc);
+          for (int K = J; K <= p; K++) tile_mma<false>(sacc, tile(p + 1, K), LL, tile(K, J), LL, 1.0);
+        }
       }
     }
     __syncthreads();
     const int f = flag;
     if (f >= 0) return f;  // leaf p + 1 failed (padding never fails, so f < n)
-  }
-  // off-diagonal tiles of the factor (the diagonal ones are already out)
-  for_block<NMAX, BLK_THREADS>(n, [&](int i, int k) {
-    const int r = lower ? i : k, c = lower ? k : i;
-    if (r / LEAF <= c / LEAF) return;
-    const T v = elem(r, c);
-    if (lower) A[(long)c * lda + r] = convert<ST>(v);
-    else A[(long)r * lda + c] = convert<ST>(conjx(v));
-  });
-  __syncthreads();
-  for (int p = np - 2; p >= 0; p--) {
-    // column block p below the diagonal: B_I <- sum_{J = p+1..I} X_IJ L_Jp,
-    // then X_Ip <- -B_I inv(L_pp); X_JJ for J > p is already in place
-    const int I = p + 1 + wave;  // at most one tile per wave (np <= NW + 1)
-    TileAcc<T> acc;
-    acc_zero(acc);
-    if (I < np)
-      for (int J = p + 1; J <= I; J++) tile_mma<false>(acc, tile(I, J), LL, tile(J, p), LL, 1.0);
-    __syncthreads();
-    if (I < np) {
-      acc_store(acc, tile(I, p), LL);
-      wave_lds_fence();
-      TileAcc<T> acc2;
-      acc_zero(acc2);
-      tile_mma<false>(acc2, tile(I, p), LL, tile(p, p), LL, -1.0);
-      wave_lds_fence();
-      acc_store(acc2, tile(I, p), LL);
-    }
-    __syncthreads();
   }
   // W = inv(L) dense, zeros above the diagonal
   for_block<NMAX, BLK_THREADS>(n, [&](int r, int c) {
