@@ -368,7 +368,10 @@ constexpr int diag_waves_per_eu() { return is_complex<T>::value ? 2 : 4; }
 // inv(L) densely to W (ld ldw, zeros above the diagonal, n x n only).
 // Returns -1, or the first non-positive / NaN pivot (0-based, the same value
 // in every thread; the factor is then only partly written).
-template <typename T, typename ST, int NMAX>
+// EARLY: wave 0 has written tile (0, 0) itself and set flag = -1, and the
+// other waves may still be filling the rest of S (leaf 0 overlaps their loads;
+// the barrier after it completes S).
+template <typename T, typename ST, int NMAX, bool EARLY = false>
 __device__ __forceinline__ int diag_block_factor(T *S, int &flag, int lower, int n, ST *A, long lda, ST *W,
                                                  long ldw) {
   constexpr int NPMAX = NMAX / LEAF, LL = LEAF + 1, TS = LEAF * LL, NW = BLK_THREADS / 64;
@@ -377,8 +380,10 @@ __device__ __forceinline__ int diag_block_factor(T *S, int &flag, int lower, int
   const int np = (n + LEAF - 1) / LEAF;
   auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
   auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
-  if (tid == 0) flag = -1;
-  __syncthreads();
+  if constexpr (!EARLY) {
+    if (tid == 0) flag = -1;
+    __syncthreads();
+  }
   // One leaf of look-ahead: while wave 0 updates tile (p+1, p+1) and factors
   // leaf p + 1, waves 1 .. NW-1 apply the rest of step p's trailing update
   // and form S_{p+1,J} = sum_{K=J..p} L_{p+1,K} X_KJ (wave 1 + J) for row
@@ -487,16 +492,53 @@ diag_potrf_kernel(int lower, int n, ST *A, long lda, ST *W, long ldw, int *info,
   const int np = (n + LEAF - 1) / LEAF, npad = np * LEAF;
   auto tile = [&](int I, int J) { return S + (I * (I + 1) / 2 + J) * TS; };
   auto elem = [&](int r, int c) -> T & { return tile(r / LEAF, c / LEAF)[(c % LEAF) * LL + r % LEAF]; };
-  // identity padding (rows / columns n .. npad - 1 of the lower form)
-  for (int c = tid; c < npad; c += BLK_THREADS)
-    for (int r = max(n, c); r < npad; r++) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
-  // lower form, read along memory's contiguous index (i); every load of a
-  // thread in flight at once (NMAX^2 / BLK_THREADS of them)
-  gather_block<NMAX, BLK_THREADS, NMAX * NMAX / BLK_THREADS>(n, A, lda, [&](int i, int k, ST x) {
-    const int r = lower ? i : k, c = lower ? k : i;  // lower-form, r >= c wanted
-    if (r >= c) elem(r, c) = lower ? convert<T>(x) : conjx(convert<T>(x));
-  });
-  const int bad = diag_block_factor<T, ST, NMAX>(S, flag, lower, n, A, lda, W, ldw);
+  if (tid < 64) {
+    // wave 0: tile (0, 0) only (lower form, identity padding), then leaf 0
+    // at once while the other waves load the rest of the block
+    const int rr = tid & 15, cq = tid >> 4, rc = min(rr, n - 1);
+    T v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int cc = min(cq + 4 * j, n - 1);
+      const ST x = lower ? A[(long)cc * lda + rc] : A[(long)rc * lda + cc];
+      v[j] = lower ? convert<T>(x) : conjx(convert<T>(x));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int c = cq + 4 * j;
+      if (rr >= c) tile(0, 0)[c * LL + rr] = (rr < n && c < n) ? v[j] : from_real<T>(rr == c ? 1.0 : 0.0);
+    }
+    if (tid == 0) flag = -1;
+    wave_lds_fence();
+  } else {
+    // waves 1 .. 7: identity padding and the lower form of every other tile,
+    // read along memory's contiguous index (i), every load of a thread in
+    // flight at once (clamped, always valid addresses)
+    constexpr int T2 = BLK_THREADS - 64, U = (NMAX * NMAX + T2 - 1) / T2;
+    const int t2 = tid - 64;
+    for (int c = t2; c < npad; c += T2)
+      for (int r = max(n, c); r < npad; r++)
+        if (r >= LEAF || c >= LEAF) elem(r, c) = from_real<T>(r == c ? 1.0 : 0.0);
+    // in batches that fit the registers (one for 4-byte data, two for 8-byte)
+    constexpr int NB = sizeof(ST) > 4 ? 2 : 1, UB = (U + NB - 1) / NB;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      ST x[UB];
+#pragma unroll
+      for (int u = 0; u < UB; u++) {
+        const int e = min(t2 + T2 * (b * UB + u), NMAX * NMAX - 1), i = e % NMAX, k = e / NMAX;
+        x[u] = A[(long)min(k, n - 1) * lda + min(i, n - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < UB; u++) {
+        const int e = t2 + T2 * (b * UB + u), i = e % NMAX, k = e / NMAX;
+        const int r = lower ? i : k, c = lower ? k : i;  // lower form, r >= c wanted
+        if (b * UB + u < U && e < NMAX * NMAX && i < n && k < n && r >= c && (r >= LEAF || c >= LEAF))
+          elem(r, c) = lower ? convert<T>(x[u]) : conjx(convert<T>(x[u]));
+      }
+    }
+  }
+  const int bad = diag_block_factor<T, ST, NMAX, true>(S, flag, lower, n, A, lda, W, ldw);
   if (bad >= 0 && tid == 0) *info = (int)(info_off + bad + 1);
 }
 
