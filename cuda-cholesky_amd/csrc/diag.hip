@@ -286,9 +286,9 @@ __device__ __forceinline__ void diag_reciprocals(const T *S, T *dinv, int n) {
 // workgroup barrier.  Everything else is 16 x 16 x 16 tile products on the
 // matrix cores spread over the eight waves: the panel below the leaf
 // (L_Ip = A_Ip inv(L_pp)^H), the trailing update (A_IJ -= L_Ip L_Jp^H), and
-// afterwards the in-place blocked inverse, column blocks from last to first
-// (X_Ip = -(sum_J X_IJ L_Jp) inv(L_pp), as in LAPACK xTRTRI).  Three
-// barriers per leaf for the factor and two for the inverse.
+// the inverse X = inv(L) row block by row block as the factor's rows
+// complete (X_IJ = -X_II sum_{K=J..I-1} L_IK X_KJ, from L X = I), formed by
+// the waves that are not factoring the next leaf.  Two barriers per leaf.
 // ---------------------------------------------------------------------------
 using t16::LEAF;
 using t16::TileAcc;
