@@ -21,6 +21,14 @@
   S (FP32 kernel, 128 vs 64: cuSpotrf), H and W (16-bit kernel, 128 vs 64 and
   256 vs 128: the FP16-product factor of cuDsposvx).  A threshold above any
   launch's tile count means "always the smaller tile".
+* Workgroup cap of the factorisation's late trailing updates: a trailing
+  update of order t <= t_max runs on at most `workgroups` workgroups, each
+  taking several tiles, so that the look-ahead chain (diagonal blocks and
+  panel products on the other stream) finds free room on the CUs.  Each
+  candidate (t_max, workgroups) is set through
+  $CUCHOLESKY_TRAILING_CAP_<precision> and the factorisation timed
+  (median of a few runs, candidates interleaved, since the differences are
+  about 1-2 %).
 
 The table goes to tuning.txt beside the library (or --out), where the library
 reads it at first use:
@@ -28,8 +36,9 @@ reads it at first use:
     potrf_nb <precision> <n_max> <nb>
     batched_per_cu <precision> <inverse 0|1> <workgroups per CU>
     gemm_tile <family> <min_tiles>
+    trailing_cap <precision> <t_max> <workgroups>
 
-    python cuda-cholesky_amd/autotune.py [--quick] [--only nb|batched|tiles] [--out PATH]
+    python cuda-cholesky_amd/autotune.py [--quick] [--only nb|batched|tiles|cap] [--out PATH]
 """
 import argparse
 import importlib.util
@@ -222,12 +231,50 @@ def tune_tiles(torch, pkg, h, sweep, reps):
     return lines
 
 
+# precision: (order, candidate (t_max, workgroups) pairs; (0, 0) = no cap)
+CAP_SWEEP = {
+    "D": (16384, [(0, 0), (8192, 384), (8192, 448), (12288, 448), (16384, 448), (16384, 480)]),
+    "Z": (8192, [(0, 0), (8192, 384), (8192, 448)]),
+}
+CAP_QUICK = {"D": (16384, [(0, 0), (12288, 448)])}
+
+
+def tune_cap(torch, pkg, h, sweep, reps):
+    """trailing_cap lines: per precision, the candidate with the least median
+    time, the candidates timed in turn reps times."""
+    lines = []
+    for prec, (n, cands) in sweep.items():
+        dtype = {"D": torch.float64, "Z": torch.complex128}[prec]
+        A0 = spd(torch, n, dtype, seed=n)
+        var = f"CUCHOLESKY_TRAILING_CAP_{prec}"
+        times = {c: [] for c in cands}
+        for c in cands:  # warm-up
+            os.environ[var] = f"{c[0]}:{c[1]}"
+            time_routine(torch, pkg, h, prec, n, A0, None, 1)
+        for _ in range(max(3, reps)):
+            for c in cands:
+                os.environ[var] = f"{c[0]}:{c[1]}"
+                times[c].append(time_routine(torch, pkg, h, prec, n, A0, None, 1))
+        os.environ.pop(var, None)
+        med = {c: sorted(v)[len(v) // 2] for c, v in times.items()}
+        best = min(med, key=med.get)
+        detail = " ".join(f"{c[0]}:{c[1]}:{t * 1e3:.2f}ms" for c, t in med.items())
+        if best[1] > 0:
+            lines.append(f"trailing_cap {prec} {best[0]} {best[1]}   # n={n} median {detail}")
+        else:
+            lines.append(f"# trailing_cap {prec}: no cap   # n={n} median {detail}")
+        print(lines[-1], flush=True)
+        del A0
+        torch.cuda.empty_cache()
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--quick", action="store_true", help="a reduced sweep (a minute or two)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.txt"))
-    ap.add_argument("--only", choices=["nb", "batched", "tiles"], help="run one of the sweeps")
+    ap.add_argument("--only", choices=["nb", "batched", "tiles", "cap"], help="run one of the sweeps")
     args = ap.parse_args()
     import torch
     pkg = package()
@@ -237,8 +284,9 @@ def main():
     lines = ["# schedule parameters measured by autotune.py on " + torch.cuda.get_device_name(0),
              "# potrf_nb <precision> <n_max> <nb>   (seconds per candidate in the trailing comment)",
              "# batched_per_cu <precision> <inverse> <workgroups per CU>",
-             "# gemm_tile <family> <min_tiles>   (milliseconds per order and candidate in the trailing comment)"]
-    if args.only in ("batched", "tiles"):
+             "# gemm_tile <family> <min_tiles>   (milliseconds per order and candidate in the trailing comment)",
+             "# trailing_cap <precision> <t_max> <workgroups>   (median milliseconds per candidate in the trailing comment)"]
+    if args.only in ("batched", "tiles", "cap"):
         sweep = {}
     for prec, (orders, cands) in sweep.items():
         dtype = {"D": torch.float64, "Z": torch.complex128, "S": torch.float32, "B": torch.float64}[prec]
@@ -262,10 +310,12 @@ def main():
         lines += tune_batched(torch, pkg, h, 4000 if args.quick else 20000, args.reps + 2)
     if args.only in (None, "tiles"):
         lines += tune_tiles(torch, pkg, h, TILE_QUICK if args.quick else TILE_SWEEP, args.reps)
+    if args.only in (None, "cap"):
+        lines += tune_cap(torch, pkg, h, CAP_QUICK if args.quick else CAP_SWEEP, args.reps + 2)
     h.close()
     if args.only is not None and os.path.exists(args.out):
         # keep the entries of the sweeps that did not run
-        ran = {"nb": "potrf_nb", "batched": "batched_per_cu", "tiles": "gemm_tile"}[args.only]
+        ran = {"nb": "potrf_nb", "batched": "batched_per_cu", "tiles": "gemm_tile", "cap": "trailing_cap"}[args.only]
         with open(args.out) as f:
             lines += [ln.rstrip("\n") for ln in f
                       if not ln.startswith("#") and not ln.startswith(ran) and ln.strip()]

@@ -145,9 +145,10 @@ hipError_t trsm(hipStream_t s, char side, char uplo, char trans, char diag, long
 // only the tiles meeting the stored triangle are launched.
 template <typename T>
 hipError_t herk(hipStream_t s, char uplo, char trans, long n, long k, double alpha, const T *A,
-                long lda, double beta, T *C, long ldc, const int *abort_flag, bool bf16) {
+                long lda, double beta, T *C, long ldc, const int *abort_flag, bool bf16, int max_wgs) {
   GemmOpts o;
   o.bf16 = bf16;
+  o.max_wgs = max_wgs;
   o.tri_out = (uplo == 'L') ? TRI_LOWER : TRI_UPPER;
   o.tri_grid = true;
   o.real_diag = is_complex<T>::value;
@@ -168,11 +169,11 @@ template hipError_t trsm<double>(hipStream_t, char, char, char, char, long, long
 template hipError_t trsm<zd>(hipStream_t, char, char, char, char, long, long, zd, const zd *, long,
                              zd *, long, zd *);
 template hipError_t herk<double>(hipStream_t, char, char, long, long, double, const double *, long,
-                                 double, double *, long, const int *, bool);
+                                 double, double *, long, const int *, bool, int);
 template hipError_t herk<zd>(hipStream_t, char, char, long, long, double, const zd *, long, double,
-                             zd *, long, const int *, bool);
+                             zd *, long, const int *, bool, int);
 template hipError_t herk<float>(hipStream_t, char, char, long, long, double, const float *, long,
-                                double, float *, long, const int *, bool);
+                                double, float *, long, const int *, bool, int);
 template hipError_t trsm<float>(hipStream_t, char, char, char, char, long, long, float,
                                 const float *, long, float *, long, float *);
 // single and single complex (FP32 matrix cores; diagonal blocks inverted in
@@ -184,6 +185,6 @@ template hipError_t trmm<zf>(hipStream_t, char, char, char, char, long, long, zf
 template hipError_t trsm<zf>(hipStream_t, char, char, char, char, long, long, zf, const zf *, long,
                              zf *, long, zf *);
 template hipError_t herk<zf>(hipStream_t, char, char, long, long, double, const zf *, long, double,
-                             zf *, long, const int *, bool);
+                             zf *, long, const int *, bool, int);
 
 }  // namespace hc

@@ -86,6 +86,9 @@ struct KParams {
   // its raw sum to part + z * m * n (m x n column-major, ld m)
   int kchunk;
   T *part;
+  // tri-grid launches: the number of tiles when the grid is smaller (each
+  // workgroup then takes tiles w, w + gridDim.x, ...); 0: one per tile
+  int vtiles;
 };
 
 // Element (r, c) of op(X).
@@ -182,8 +185,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   // renumber so that every XCD owns a contiguous range of logical tiles, and
   // walk tiles in groups of GROUP tile rows, column by column inside a group,
   // so the tiles in flight on one XCD share their op(A) and op(B) panels.
-  const int nwg = TG ? (int)gridDim.x : (int)(gridDim.x * gridDim.y);
-  const int wid = TG ? (int)blockIdx.x : (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  auto tile_body = [&](const int wid, const int nwg) __attribute__((always_inline)) {
   const int logical = xcd_contiguous(wid, nwg);
   int bm, bn;
   if constexpr (TG) {
@@ -246,7 +248,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
   kbeg &= ~(KT - 1);
   if (is_zero(p.alpha)) kend = kbeg;
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));  // thread-derived values are formed per tile, not kept across tiles
+  const int lane = tid & 63, wave = tid >> 6;
   const int lc = lane & 15, lg = lane >> 4;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
 
@@ -483,6 +487,16 @@ __global__ void __launch_bounds__(NTHREADS, 2) gemm_kernel(KParams<T> p) {
       }
     }
   }
+  };
+  if (TG && p.vtiles > 0) {
+    // fewer workgroups than tiles: a workgroup's tiles keep its XCD
+    // (gridDim.x is a multiple of 8) and reuse its LDS after the last
+    // barrier of the previous tile
+    for (int w = (int)blockIdx.x; w < p.vtiles; w += (int)gridDim.x) tile_body(w, p.vtiles);
+  } else {
+    tile_body(TG ? (int)blockIdx.x : (int)(blockIdx.x + blockIdx.y * gridDim.x),
+              TG ? (int)gridDim.x : (int)(gridDim.x * gridDim.y));
+  }
 }
 
 // D = alpha * sum_z part[z] + beta * C (split-K epilogue, fixed order).
@@ -516,22 +530,27 @@ hipError_t launch(hipStream_t s, const KParams<T> &p) {
 }
 
 template <typename T, int TA, int TB, int MT>
-hipError_t launch_tri(hipStream_t s, const KParams<T> &p) {
-  const long t = (p.m + MT - 1) / MT;
-  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, MT, true>), dim3((unsigned)(t * (t + 1) / 2)),
-                     dim3(NTHREADS), 0, s, p);
+hipError_t launch_tri(hipStream_t s, KParams<T> p, int max_wgs) {
+  const long t = (p.m + MT - 1) / MT, tiles = t * (t + 1) / 2;
+  long grid = tiles;
+  p.vtiles = 0;
+  if (max_wgs > 0 && max_wgs < tiles) {
+    grid = std::max(NXCD, max_wgs / NXCD * NXCD);
+    p.vtiles = (int)tiles;
+  }
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, MT, MT, true>), dim3((unsigned)grid), dim3(NTHREADS), 0, s, p);
   return hipGetLastError();
 }
 
 template <typename T, int TA, int TB>
-hipError_t dispatch_tile(hipStream_t s, const KParams<T> &p, int mt, int nt) {
+hipError_t dispatch_tile(hipStream_t s, const KParams<T> &p, int mt, int nt, int max_wgs) {
   constexpr bool CPLX = is_complex<T>::value;
   constexpr int BIG = CPLX ? 64 : 128, SMALL = CPLX ? 32 : 64;
   if (p.tri_grid) {
     // SYRK / HERK: op(B) = op(A)^H, so only (N, T/C) and (T/C, N) occur
     if constexpr ((TA == OP_N) != (TB == OP_N)) {
-      if (mt == BIG) return launch_tri<T, TA, TB, BIG>(s, p);
-      return launch_tri<T, TA, TB, SMALL>(s, p);
+      if (mt == BIG) return launch_tri<T, TA, TB, BIG>(s, p, max_wgs);
+      return launch_tri<T, TA, TB, SMALL>(s, p, max_wgs);
     }
     return hipErrorInvalidValue;
   }
@@ -545,17 +564,17 @@ hipError_t dispatch_tile(hipStream_t s, const KParams<T> &p, int mt, int nt) {
 }
 
 template <typename T>
-hipError_t dispatch_ops(hipStream_t s, int ta, int tb, const KParams<T> &p, int mt, int nt) {
+hipError_t dispatch_ops(hipStream_t s, int ta, int tb, const KParams<T> &p, int mt, int nt, int max_wgs) {
   if constexpr (!is_complex<T>::value) {
     // for real data C and T coincide
     const bool a = ta != OP_N, b = tb != OP_N;
-    if (!a && !b) return dispatch_tile<T, OP_N, OP_N>(s, p, mt, nt);
-    if (!a && b) return dispatch_tile<T, OP_N, OP_T>(s, p, mt, nt);
-    if (a && !b) return dispatch_tile<T, OP_T, OP_N>(s, p, mt, nt);
-    return dispatch_tile<T, OP_T, OP_T>(s, p, mt, nt);
+    if (!a && !b) return dispatch_tile<T, OP_N, OP_N>(s, p, mt, nt, max_wgs);
+    if (!a && b) return dispatch_tile<T, OP_N, OP_T>(s, p, mt, nt, max_wgs);
+    if (a && !b) return dispatch_tile<T, OP_T, OP_N>(s, p, mt, nt, max_wgs);
+    return dispatch_tile<T, OP_T, OP_T>(s, p, mt, nt, max_wgs);
   } else {
 #define HC_OPS_CASE(X, Y) \
-    if (ta == X && tb == Y) return dispatch_tile<T, X, Y>(s, p, mt, nt);
+    if (ta == X && tb == Y) return dispatch_tile<T, X, Y>(s, p, mt, nt, max_wgs);
     HC_OPS_CASE(OP_N, OP_N) HC_OPS_CASE(OP_N, OP_T) HC_OPS_CASE(OP_N, OP_C)
     HC_OPS_CASE(OP_T, OP_N) HC_OPS_CASE(OP_T, OP_T) HC_OPS_CASE(OP_T, OP_C)
     HC_OPS_CASE(OP_C, OP_N) HC_OPS_CASE(OP_C, OP_T) HC_OPS_CASE(OP_C, OP_C)
@@ -589,6 +608,7 @@ hipError_t gemm_tile(hipStream_t s, int ta, int tb, long m, long n, long k, T al
   p.abort_flag = o.abort_flag;
   p.kchunk = 0;
   p.part = nullptr;
+  p.vtiles = 0;
 
   // Tile choice: the large tile when the launch has at least the measured
   // number of them (family D / Z of the tuning table; about one workgroup per
@@ -625,7 +645,7 @@ hipError_t gemm_tile(hipStream_t s, int ta, int tb, long m, long n, long k, T al
     if (m != n) return hipErrorInvalidValue;
     mt = nt = (mt > nt ? mt : nt);
   }
-  return dispatch_ops<T>(s, ta, tb, p, mt, nt);
+  return dispatch_ops<T>(s, ta, tb, p, mt, nt, o.tri_grid ? o.max_wgs : 0);
 }
 
 template <typename T>

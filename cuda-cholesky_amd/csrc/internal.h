@@ -125,6 +125,7 @@ struct GemmOpts {
   bool bf16 = false;        // float data only: products in BF16 (FP32 accumulate)
   void *splitk_ws = nullptr;  // optional workspace: few-column products may split K
   long splitk_elems = 0;      // ... of this many elements
+  int max_wgs = 0;            // tri_grid: at most this many workgroups (0: one per tile)
 };
 
 template <typename T>
@@ -173,7 +174,7 @@ hipError_t trsm(hipStream_t s, char side, char uplo, char trans, char diag, long
 template <typename T>
 hipError_t herk(hipStream_t s, char uplo, char trans, long n, long k, double alpha,
                 const T *A, long lda, double beta, T *C, long ldc,
-                const int *abort_flag = nullptr, bool bf16 = false);
+                const int *abort_flag = nullptr, bool bf16 = false, int max_wgs = 0);
 
 // ---------------------------------------------------------------------------
 // diagonal-block kernels (diag.hip); the block is held in LDS
@@ -378,6 +379,9 @@ long tuned_tile_min(char family, long dflt);
 // Whether each panel of the lower real factorisation of precision prec is
 // factored in one launch (panel_potrf).
 bool tuned_panel_launch(char prec, bool dflt);
+// Workgroup cap of the factorisation's trailing update of order t in
+// precision prec (tuning table entry trailing_cap), 0 for none.
+int tuned_trailing_cap(char prec, long t);
 
 // Measured outer block size for precision D / Z / S / B at order n, 0 when
 // the tuning table has no entry (tuning.cpp).

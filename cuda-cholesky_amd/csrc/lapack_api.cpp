@@ -315,13 +315,15 @@ hipError_t potrf_device(CULAPACKhandle h, char uplo, long n, T *A, long lda, T *
           done = true;
         }
       }
+      // late, shorter trailing updates leave room for the look-ahead chain
+      const int max_wgs = hc::tuned_trailing_cap(prec_of<T>(bf16), t);
       if (done) {
       } else if (lower)
         HC_TRY(hc::herk<T>(s0, 'L', 'N', t, jb, -1.0, at(t0, j), lda, 1.0, at(t0, t0), lda, h->dinfo,
-                           bf16));
+                           bf16, max_wgs));
       else
         HC_TRY(hc::herk<T>(s0, 'U', 'C', t, jb, -1.0, at(j, t0), lda, 1.0, at(t0, t0), lda, h->dinfo,
-                           bf16));
+                           bf16, max_wgs));
       HC_TRY(timing_end(h, s0));
     }
     HC_TRY(hipStreamWaitEvent(s0, h->ev_panel, 0));

@@ -8,13 +8,17 @@
 //     batched_per_cu <precision> <inverse> <workgroups per CU>
 //     gemm_tile <family> <min_tiles>
 //     panel_launch <precision> <0|1>
+//     trailing_cap <precision> <t_max> <workgroups>
 // precision: D (FP64), Z (complex FP64), S (FP32 products / storage), B
 // (BF16 products).  For an order n the first potrf_nb line of that precision
 // with n <= n_max applies.  gemm_tile: a launch of kernel family F takes its
 // larger tile when it has at least min_tiles of them (families below).
 // panel_launch: 1 factors each panel of the lower real factorisation in one
 // launch (panel_potrf), 0 as a chain of diagonal-block and panel-product
-// launches.  The
+// launches.  trailing_cap: a trailing update of the factorisation whose
+// order is at most t_max runs on at most that many workgroups (each then
+// takes several tiles), so that the look-ahead chain's launches find free
+// room on the CUs instead of waiting for an update workgroup to end.  The
 // table is read once, from $CUCHOLESKY_TUNING or else tuning.txt beside the
 // library; without it the built-in defaults hold.
 #include "internal.h"
@@ -52,6 +56,9 @@ long g_tile_min[26] = {};
 int g_panel_launch[26] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
                           -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 
+// trailing_cap entries by precision letter; {0, 0} = none
+long g_cap[26][2] = {};
+
 void load() {
   std::string path;
   if (const char *e = std::getenv("CUCHOLESKY_TUNING")) {
@@ -84,6 +91,9 @@ void load() {
     if (std::sscanf(line, "%63s %7s %ld %ld", key, prec, &n_max, &nb) != 4) continue;
     if (std::strcmp(key, "potrf_nb") == 0 && nb > 0 && n_max > 0)
       table().push_back(Entry{prec[0], n_max, nb});
+    else if (std::strcmp(key, "trailing_cap") == 0 && prec[0] >= 'A' && prec[0] <= 'Z' && n_max > 0 &&
+             nb >= 8)
+      g_cap[prec[0] - 'A'][0] = n_max, g_cap[prec[0] - 'A'][1] = nb;
     else if (std::strcmp(key, "batched_per_cu") == 0 && (prec[0] == 'D' || prec[0] == 'S') &&
              (n_max == 0 || n_max == 1) && nb > 0 && nb <= 64)
       g_batched_per_cu[prec[0] == 'S' ? 1 : 0][n_max] = (int)nb;
@@ -134,6 +144,24 @@ bool tuned_panel_launch(char prec, bool dflt) {
   std::call_once(g_once, load);
   const int v = g_panel_launch[prec - 'A'];
   return v < 0 ? dflt : v != 0;
+}
+
+int tuned_trailing_cap(char prec, long t) {
+  if (prec < 'A' || prec > 'Z') return 0;
+  // $CUCHOLESKY_TRAILING_CAP_<precision> = "t_max:workgroups" (read on every
+  // call, for the autotuner; "0:0" switches the cap off), else the table
+  long t_max = 0, wgs = 0;
+  char name[40] = "CUCHOLESKY_TRAILING_CAP_";
+  name[24] = prec;
+  name[25] = '\0';
+  if (const char *e = std::getenv(name)) {
+    if (std::sscanf(e, "%ld:%ld", &t_max, &wgs) != 2) t_max = wgs = 0;
+  } else {
+    std::call_once(g_once, load);
+    t_max = g_cap[prec - 'A'][0];
+    wgs = g_cap[prec - 'A'][1];
+  }
+  return t <= t_max && wgs >= 8 ? (int)wgs : 0;
 }
 
 }  // namespace hc

@@ -61,6 +61,33 @@ def test_dpotrf(gpu, lapack, uplo, n):
     assert np.all(F[n:] == -7.0)
 
 
+@pytest.mark.parametrize("cap", ["100000:8", "100000:64", "1200:448"])
+@pytest.mark.parametrize("prec,uplo", [("D", "L"), ("D", "U"), ("Z", "L"), ("Z", "U")])
+def test_potrf_trailing_cap(gpu, lapack, monkeypatch, prec, uplo, cap):
+    """Trailing updates capped at a few workgroups (each then takes several
+    tiles; CUCHOLESKY_TRAILING_CAP_<prec> = "t_max:workgroups", read on every
+    call): the same tiles computed by other workgroups, so the factor is
+    bit-identical to the uncapped one, and within the backward-error bound."""
+    if prec == "D":
+        n = 2500
+        A, fn = O.spd(n + 1, n, 1.0), gpu.cuDpotrf
+    else:
+        n = 1500
+        A, fn = O.hpd(n, n, 0.1), gpu.cuZpotrf
+    monkeypatch.setenv(f"CUCHOLESKY_TRAILING_CAP_{prec}", "0:0")
+    dA = gpu.to_device(A)
+    st, info = gpu_call(gpu, fn, lapack, ord(uplo), n, gpu.ptr(dA), n)
+    assert st == 0 and info == 0
+    F0 = gpu.from_device(dA)
+    monkeypatch.setenv(f"CUCHOLESKY_TRAILING_CAP_{prec}", cap)
+    dA = gpu.to_device(A)
+    st, info = gpu_call(gpu, fn, lapack, ord(uplo), n, gpu.ptr(dA), n)
+    assert st == 0 and info == 0
+    F = gpu.from_device(dA)
+    np.testing.assert_array_equal(F, F0)
+    assert O.potrf_backward_error(A, F, uplo) <= 10 * n * EPS
+
+
 @pytest.mark.parametrize("n", [2, 100, 1000])
 @pytest.mark.parametrize("uplo", "LU")
 def test_dpotrf_dlatmc(gpu, lapack, uplo, n):
