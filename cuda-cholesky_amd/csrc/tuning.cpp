@@ -92,8 +92,10 @@ void load() {
     if (std::strcmp(key, "potrf_nb") == 0 && nb > 0 && n_max > 0)
       table().push_back(Entry{prec[0], n_max, nb});
     else if (std::strcmp(key, "trailing_cap") == 0 && prec[0] >= 'A' && prec[0] <= 'Z' && n_max > 0 &&
-             nb >= 8)
-      g_cap[prec[0] - 'A'][0] = n_max, g_cap[prec[0] - 'A'][1] = nb;
+             nb >= 8) {
+      g_cap[prec[0] - 'A'][0] = n_max;
+      g_cap[prec[0] - 'A'][1] = nb;
+    }
     else if (std::strcmp(key, "batched_per_cu") == 0 && (prec[0] == 'D' || prec[0] == 'S') &&
              (n_max == 0 || n_max == 1) && nb > 0 && nb <= 64)
       g_batched_per_cu[prec[0] == 'S' ? 1 : 0][n_max] = (int)nb;
